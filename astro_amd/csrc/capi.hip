@@ -1,0 +1,979 @@
+// capi.hip -- libastro's C-ABI: context, buffers, stage calls and the fused step.
+//
+// The context plays the role of the reference's Renderer + MetalKernelsRadixSort objects
+// (Renderer.swift:56-111, RadixSort.swift:24-67): it owns every device buffer, allocated
+// once at creation, and one HIP stream on which every call enqueues its kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/astro/astro.h"
+#include "launch.hpp"
+
+using namespace astro;
+
+struct astro_ctx {
+    astro_config cfg{};
+    astro_params params{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    uint32_t ns = 0, nt = 0, nsys = 1;
+    std::vector<uint32_t> sys_off;     // nsys + 1 particle offsets
+    std::vector<float> colors;         // host copy of ColorType.color (render-only data)
+
+    // particle state (SoA float4: 16-B rows instead of the 32-B reference structs)
+    float4 *pos = nullptr, *vel = nullptr, *force = nullptr;
+    uint32_t* type = nullptr;
+    uint32_t* sys_of = nullptr;
+    // keys and sort
+    uint64_t *codes = nullptr, *unsorted_codes = nullptr, *sorted_codes = nullptr;
+    uint64_t *tmp_keys_a = nullptr, *tmp_keys_b = nullptr;
+    uint32_t *unsorted_idx = nullptr, *sorted_idx = nullptr, *tmp_idx_a = nullptr,
+             *tmp_idx_b = nullptr;
+    uint32_t *unique_idx = nullptr, *flags = nullptr, *scan = nullptr, *ebuf = nullptr,
+             *fbuf = nullptr;
+    uint32_t *parent_count = nullptr, *child_count = nullptr;
+    uint32_t* dev_consts = nullptr;    // [0] = ns, [1] = nt
+    // tree
+    TNode* tnodes = nullptr;
+    OctreeNode* onodes = nullptr;
+    uint64_t node_capacity = 0;
+    uint32_t *layer_counts = nullptr, *layer_counts_prev = nullptr, *leaf_of_sorted = nullptr;
+    float4* cubes = nullptr;
+    uint32_t* minmax = nullptr;
+    uint32_t* err_word = nullptr;
+    unsigned long long* stats = nullptr;
+    uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
+    uint64_t scratch_words = 0;
+    bool emit = true;
+    bool onodes_dirty = true;          // stage calls wrote onodes outside the fused build
+
+    // layer table of the current particles
+    uint32_t layers = 0;
+    uint64_t cap[kMaxLayers] = {}, off[kMaxLayers] = {}, total = 0;
+
+    // graph of one fused step
+    hipGraphExec_t graph = nullptr;
+    float graph_dt = 0.0f;
+    bool graph_valid = false;
+    bool use_graph = true;
+};
+
+// ------------------------------------------------------------------------------- helpers
+static int fail(astro_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define HIPCHK(c, call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), ASTRO_ERR_HIP,                                               \
+                        std::string(#call) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+#define LAUNCHCHK(c)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), ASTRO_ERR_HIP, std::string("kernel launch: ") +              \
+                                                hipGetErrorString(e_));                   \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, uint64_t count) {
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+// cap_l = sum over systems of min(N_s, 8^(B-l)); offsets are their prefix sums
+// (Renderer.swift:373-386 generalised to B bits and several systems).
+static void compute_layers(const std::vector<uint32_t>& sizes, uint32_t bits, uint32_t* layers,
+                           uint64_t* cap, uint64_t* off, uint64_t* total) {
+    *layers = bits + 1;
+    uint64_t o = 0;
+    for (uint32_t l = 0; l < *layers; l++) {
+        uint32_t b = 3 * bits - 3 * l;
+        uint64_t possible = b >= 63 ? (1ull << 63) : (1ull << b);
+        uint64_t c = 0;
+        for (uint32_t n : sizes) c += std::min<uint64_t>(n, possible);
+        cap[l] = c;
+        off[l] = o;
+        o += c;
+    }
+    *total = o;
+}
+
+static uint64_t scratch_need(uint32_t nmax, uint32_t bits, const uint64_t* cap, uint32_t layers) {
+    uint64_t w = (uint64_t)kMaxRadixPasses * 256;                              // histograms
+    w += (uint64_t)kMaxRadixPasses * ((uint64_t)sort_tiles(nmax) * 256 + 1);   // pass status
+    w += scan_tiles(nmax) + 1;                                                 // leaf layer
+    for (uint32_t l = 1; l < layers; l++) w += scan_tiles((uint32_t)cap[l - 1]) + 1;
+    w += scan_tiles(nmax) + 1;                                                 // astro_scan
+    (void)bits;
+    return w;
+}
+
+static void invalidate_graph(astro_ctx* c) {
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_valid = false;
+}
+
+static int refresh_layers(astro_ctx* c) {
+    std::vector<uint32_t> sizes;
+    if (c->nsys <= 1) sizes.push_back(c->nt);
+    else
+        for (uint32_t s = 0; s < c->nsys; s++) sizes.push_back(c->sys_off[s + 1] - c->sys_off[s]);
+    compute_layers(sizes, c->params.bits_per_axis, &c->layers, c->cap, c->off, &c->total);
+    if (c->total > c->node_capacity)
+        return fail(c, ASTRO_ERR_CAPACITY, "tree needs more nodes than the context holds");
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+// ---------------------------------------------------------------------- lifetime/params
+extern "C" void astro_params_compat(astro_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->bits_per_axis = 6;  // MortonCode.metal:43-47
+    p->quantizer = ASTRO_QUANT_OFFSET_WRAP;
+    p->quant_offset = 1000000.0f;
+    p->side_rule = ASTRO_SIDE_SHIFTED;
+    p->theta = 0.7f;        // Renderer.swift:864
+    p->eps2 = 0.01f;        // BarnesHut.metal:40
+    p->G = 1e11f;           // BarnesHut.metal:39
+    p->r2_floor = 1e-6f;    // BarnesHut.metal:59
+    p->r_floor = 1e-6f;     // BarnesHut.metal:61
+    p->leaf_cutoff_r2 = 50.0f;  // BarnesHut.metal:70
+    p->force_clamp = 1e5f;  // BarnesHut.metal:100
+    p->damping = 0.9995f;   // BarnesHut.metal:190
+    p->type_scale[0] = 5.0f;
+    p->type_scale[1] = 5.0f;
+    p->type_scale[2] = 1.0f;  // BarnesHut.metal:182-188
+    p->quirk_last_leaf = 1;   // Octree.metal:71
+    p->fp64_top_layers = 0;
+    p->kahan = 0;
+}
+
+extern "C" void astro_params_physics(astro_params* p, uint32_t bits_per_axis) {
+    astro_params_compat(p);
+    p->bits_per_axis = bits_per_axis;
+    p->quantizer = ASTRO_QUANT_BBOX_POW2;
+    p->quant_offset = 0.0f;
+    p->side_rule = ASTRO_SIDE_EXACT;
+    p->G = 1.0f;
+    p->leaf_cutoff_r2 = 0.0f;
+    p->force_clamp = 0.0f;
+    p->damping = 1.0f;
+    p->type_scale[0] = p->type_scale[1] = p->type_scale[2] = 1.0f;
+    p->quirk_last_leaf = 0;
+    p->fp64_top_layers = 4;
+}
+
+extern "C" int astro_device_count(int* count) {
+    if (!count) return ASTRO_ERR_ARG;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    return e == hipSuccess ? ASTRO_OK : ASTRO_ERR_HIP;
+}
+
+extern "C" int astro_destroy(astro_ctx* c) {
+    if (!c) return ASTRO_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    invalidate_graph(c);
+    void* ptrs[] = {c->pos, c->vel, c->force, c->type, c->sys_of, c->codes, c->unsorted_codes,
+                    c->sorted_codes, c->tmp_keys_a, c->tmp_keys_b, c->unsorted_idx,
+                    c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
+                    c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
+                    c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
+                    c->leaf_of_sorted, c->cubes, c->minmax, c->err_word, c->stats, c->scratch};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return ASTRO_OK;
+}
+
+extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
+    if (!cfg || !out) return ASTRO_ERR_ARG;
+    *out = nullptr;
+    if (cfg->max_particles == 0 || cfg->bits_per_axis == 0 || cfg->bits_per_axis > 21)
+        return ASTRO_ERR_ARG;
+    astro_ctx* c = new astro_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.max_tree == 0 || c->cfg.max_tree > c->cfg.max_particles)
+        c->cfg.max_tree = c->cfg.max_particles;
+    if (c->cfg.max_systems == 0) c->cfg.max_systems = 1;
+    c->device = cfg->device;
+    c->emit = cfg->emit_octree_nodes != 0;
+    astro_params_compat(&c->params);
+    c->params.bits_per_axis = std::min<uint32_t>(c->params.bits_per_axis, cfg->bits_per_axis);
+    const uint32_t NP = c->cfg.max_particles, NT = c->cfg.max_tree;
+
+    // Node capacity: sum over layers of min(N_t, S * 8^(B-l)) bounds every split of N_t
+    // into at most S systems, for every B' <= B (fewer bits only shrink the layers).
+    uint64_t cap_bound = 0;
+    uint64_t cap_max[kMaxLayers] = {};
+    for (uint32_t bits = 1; bits <= cfg->bits_per_axis; bits++) {
+        uint64_t t = 0;
+        for (uint32_t l = 0; l <= bits; l++) {
+            uint32_t b = 3 * bits - 3 * l;
+            uint64_t possible = b >= 60 ? (1ull << 60) : (1ull << b);
+            possible = std::min<uint64_t>(possible * c->cfg.max_systems, 1ull << 60);
+            uint64_t cl = std::min<uint64_t>(NT, possible);
+            t += cl;
+            if (bits == cfg->bits_per_axis) cap_max[l] = cl;
+        }
+        cap_bound = std::max(cap_bound, t);
+    }
+    c->node_capacity = cap_bound;
+    if (cap_bound >= (1ull << 29)) {
+        astro_destroy(c);
+        return ASTRO_ERR_CAPACITY;  // node indices are packed in 29 bits on the stack
+    }
+
+    auto bad = [&](hipError_t e) {
+        if (e != hipSuccess) {
+            astro_destroy(c);
+            return true;
+        }
+        return false;
+    };
+    if (bad(hipSetDevice(c->device))) return ASTRO_ERR_HIP;
+    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
+    hipError_t e = hipSuccess;
+    e = e ? e : dalloc(&c->pos, NP);
+    e = e ? e : dalloc(&c->vel, NP);
+    e = e ? e : dalloc(&c->force, NP);
+    e = e ? e : dalloc(&c->type, NP);
+    e = e ? e : dalloc(&c->sys_of, NP);
+    e = e ? e : dalloc(&c->codes, NP);
+    e = e ? e : dalloc(&c->unsorted_codes, NP);
+    e = e ? e : dalloc(&c->sorted_codes, NP);
+    e = e ? e : dalloc(&c->tmp_keys_a, NP);
+    e = e ? e : dalloc(&c->tmp_keys_b, NP);
+    e = e ? e : dalloc(&c->unsorted_idx, NP);
+    e = e ? e : dalloc(&c->sorted_idx, NP);
+    e = e ? e : dalloc(&c->tmp_idx_a, NP);
+    e = e ? e : dalloc(&c->tmp_idx_b, NP);
+    e = e ? e : dalloc(&c->unique_idx, NP);
+    e = e ? e : dalloc(&c->flags, NP);
+    e = e ? e : dalloc(&c->scan, NP);
+    e = e ? e : dalloc(&c->ebuf, NP);
+    e = e ? e : dalloc(&c->fbuf, NP);
+    e = e ? e : dalloc(&c->parent_count, 1);
+    e = e ? e : dalloc(&c->child_count, 1);
+    e = e ? e : dalloc(&c->dev_consts, 4);
+    e = e ? e : dalloc(&c->tnodes, cap_bound);
+    e = e ? e : dalloc(&c->onodes, cap_bound);
+    e = e ? e : dalloc(&c->layer_counts, kMaxLayers);
+    e = e ? e : dalloc(&c->layer_counts_prev, kMaxLayers);
+    e = e ? e : dalloc(&c->leaf_of_sorted, NT);
+    e = e ? e : dalloc(&c->cubes, c->cfg.max_systems);
+    e = e ? e : dalloc(&c->minmax, 6ull * c->cfg.max_systems);
+    e = e ? e : dalloc(&c->err_word, 1);
+    e = e ? e : dalloc(&c->stats, 4);
+    if (e != hipSuccess) {
+        astro_destroy(c);
+        return ASTRO_ERR_NOMEM;
+    }
+    c->scratch_words = scratch_need(NP, cfg->bits_per_axis, cap_max, cfg->bits_per_axis + 1);
+    if (bad(dalloc(&c->scratch, c->scratch_words))) return ASTRO_ERR_NOMEM;
+    hipStream_t s = c->stream;
+    launch_fill_u32(c->err_word, 0, 1, s);
+    launch_fill_u32(c->layer_counts, 0, kMaxLayers, s);
+    launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
+    launch_fill_u32(c->sys_of, 0, NP, s);
+    launch_fill_u32(c->parent_count, 0, 1, s);
+    launch_fill_u32(c->child_count, 0, 1, s);
+    launch_fill_u32(c->dev_consts, 0, 4, s);
+    launch_clear_onodes(c->onodes, cap_bound, s);
+    c->onodes_dirty = false;
+    if (bad(hipGetLastError()) || bad(hipStreamSynchronize(s))) return ASTRO_ERR_HIP;
+    *out = c;
+    return ASTRO_OK;
+}
+
+extern "C" const char* astro_last_error(const astro_ctx* c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+extern "C" int astro_set_params(astro_ctx* c, const astro_params* p) {
+    if (!c || !p) return ASTRO_ERR_ARG;
+    if (p->bits_per_axis == 0 || p->bits_per_axis > c->cfg.bits_per_axis)
+        return fail(c, ASTRO_ERR_ARG, "bits_per_axis outside [1, configured maximum]");
+    if (p->quantizer > ASTRO_QUANT_BBOX_POW2 || p->side_rule > ASTRO_SIDE_EXACT)
+        return fail(c, ASTRO_ERR_ARG, "unknown quantizer or side rule");
+    if (p->side_rule == ASTRO_SIDE_SHIFTED && p->bits_per_axis > 10)
+        return fail(c, ASTRO_ERR_ARG, "the shifted side rule overflows above 10 layers");
+    c->params = *p;
+    return refresh_layers(c);
+}
+
+extern "C" int astro_get_params(const astro_ctx* c, astro_params* p) {
+    if (!c || !p) return ASTRO_ERR_ARG;
+    *p = c->params;
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_emit_octree_nodes(astro_ctx* c, int on) {
+    if (!c) return ASTRO_ERR_ARG;
+    c->emit = on != 0;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_use_graph(astro_ctx* c, int on) {
+    if (!c) return ASTRO_ERR_ARG;
+    c->use_graph = on != 0;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+// ------------------------------------------------------------------------ state transfer
+static int set_counts(astro_ctx* c, uint32_t ns, uint32_t nt) {
+    c->ns = ns;
+    c->nt = nt;
+    uint32_t consts[4] = {ns, nt, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(c->dev_consts, consts, sizeof(consts), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_upload(astro_ctx* c, const PositionMass* pos, const VelocityRadius* vel,
+                            const ColorType* col, uint32_t n, uint32_t n_tree) {
+    if (!c || !pos) return ASTRO_ERR_ARG;
+    if (n > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "too many particles");
+    if (n_tree == 0 || n_tree > n) n_tree = n;
+    if (n_tree > c->cfg.max_tree) return fail(c, ASTRO_ERR_CAPACITY, "too many tree sources");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<float4> p4(n), v4(n);
+    std::vector<uint32_t> ty(n, 0u);
+    c->colors.assign(4ull * n, 0.0f);
+    for (uint32_t i = 0; i < n; i++) {
+        p4[i] = make_float4(pos[i].position[0], pos[i].position[1], pos[i].position[2],
+                            pos[i].mass);
+        if (vel)
+            v4[i] = make_float4(vel[i].velocity[0], vel[i].velocity[1], vel[i].velocity[2],
+                                vel[i].radius);
+        else
+            v4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (col) {
+            ty[i] = col[i].type;
+            for (int k = 0; k < 4; k++) c->colors[4ull * i + k] = col[i].color[k];
+        }
+    }
+    if (n) {
+        HIPCHK(c, hipMemcpyAsync(c->pos, p4.data(), n * sizeof(float4), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->vel, v4.data(), n * sizeof(float4), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->type, ty.data(), n * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    launch_fill_u32(c->sys_of, 0, c->cfg.max_particles, c->stream);
+    c->nsys = 1;
+    c->sys_off = {0u, n};
+    int rc = set_counts(c, n, n_tree);
+    if (rc) return rc;
+    return refresh_layers(c);
+}
+
+extern "C" int astro_download(astro_ctx* c, PositionMass* pos, VelocityRadius* vel,
+                              ColorType* col) {
+    if (!c) return ASTRO_ERR_ARG;
+    const uint32_t n = c->ns;
+    std::vector<float4> p4(n), v4(n);
+    std::vector<uint32_t> ty(n);
+    if (n) {
+        HIPCHK(c, hipMemcpyAsync(p4.data(), c->pos, n * sizeof(float4), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(v4.data(), c->vel, n * sizeof(float4), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(ty.data(), c->type, n * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n; i++) {
+        if (pos) {
+            std::memset(&pos[i], 0, sizeof(PositionMass));
+            pos[i].position[0] = p4[i].x;
+            pos[i].position[1] = p4[i].y;
+            pos[i].position[2] = p4[i].z;
+            pos[i].mass = p4[i].w;
+        }
+        if (vel) {
+            std::memset(&vel[i], 0, sizeof(VelocityRadius));
+            vel[i].velocity[0] = v4[i].x;
+            vel[i].velocity[1] = v4[i].y;
+            vel[i].velocity[2] = v4[i].z;
+            vel[i].radius = v4[i].w;
+        }
+        if (col) {
+            std::memset(&col[i], 0, sizeof(ColorType));
+            for (int k = 0; k < 4; k++) col[i].color[k] = c->colors[4ull * i + k];
+            col[i].type = ty[i];
+        }
+    }
+    return ASTRO_OK;
+}
+
+extern "C" int astro_download_forces(astro_ctx* c, float* force4) {
+    if (!c || !force4) return ASTRO_ERR_ARG;
+    if (c->ns)
+        HIPCHK(c, hipMemcpyAsync(force4, c->force, c->ns * sizeof(float4), hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t* bytes) {
+    if (!c || !d_ptr) return ASTRO_ERR_ARG;
+    const uint64_t NP = c->cfg.max_particles, NT = c->cfg.max_tree;
+    void* p = nullptr;
+    uint64_t b = 0;
+    switch (which) {
+        case ASTRO_BUF_POSITION_MASS: p = c->pos; b = NP * 16; break;
+        case ASTRO_BUF_VELOCITY_RADIUS: p = c->vel; b = NP * 16; break;
+        case ASTRO_BUF_TYPE: p = c->type; b = NP * 4; break;
+        case ASTRO_BUF_FORCE: p = c->force; b = NP * 16; break;
+        case ASTRO_BUF_MORTON_CODES: p = c->codes; b = NP * 8; break;
+        case ASTRO_BUF_UNSORTED_CODES: p = c->unsorted_codes; b = NP * 8; break;
+        case ASTRO_BUF_UNSORTED_INDICES: p = c->unsorted_idx; b = NP * 4; break;
+        case ASTRO_BUF_SORTED_CODES: p = c->sorted_codes; b = NP * 8; break;
+        case ASTRO_BUF_SORTED_INDICES: p = c->sorted_idx; b = NP * 4; break;
+        case ASTRO_BUF_UNIQUE_INDICES: p = c->unique_idx; b = NP * 4; break;
+        case ASTRO_BUF_PARENT_COUNT: p = c->parent_count; b = 4; break;
+        case ASTRO_BUF_CHILD_COUNT: p = c->child_count; b = 4; break;
+        case ASTRO_BUF_FLAGS: p = c->flags; b = NP * 4; break;
+        case ASTRO_BUF_SCAN: p = c->scan; b = NP * 4; break;
+        case ASTRO_BUF_OCTREE_NODES: p = c->onodes; b = c->node_capacity * sizeof(OctreeNode); break;
+        case ASTRO_BUF_TREE_NODES: p = c->tnodes; b = c->node_capacity * sizeof(TNode); break;
+        case ASTRO_BUF_LAYER_COUNTS: p = c->layer_counts; b = kMaxLayers * 4; break;
+        case ASTRO_BUF_LEAF_OF_SORTED: p = c->leaf_of_sorted; b = NT * 4; break;
+        case ASTRO_BUF_CUBES: p = c->cubes; b = c->cfg.max_systems * 16ull; break;
+        default: return fail(c, ASTRO_ERR_ARG, "unknown buffer id");
+    }
+    *d_ptr = p;
+    if (bytes) *bytes = b;
+    return ASTRO_OK;
+}
+
+extern "C" int astro_memcpy_d2h(astro_ctx* c, void* host, const void* d_src, uint64_t bytes) {
+    if (!c || (!host && bytes) || (!d_src && bytes)) return ASTRO_ERR_ARG;
+    if (bytes)
+        HIPCHK(c, hipMemcpyAsync(host, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_memcpy_h2d(astro_ctx* c, void* d_dst, const void* host, uint64_t bytes) {
+    if (!c || (!host && bytes) || (!d_dst && bytes)) return ASTRO_ERR_ARG;
+    if (bytes)
+        HIPCHK(c, hipMemcpyAsync(d_dst, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_layer_table(const astro_ctx* c, uint32_t* layers, uint64_t* cap,
+                                 uint64_t* off) {
+    if (!c || !layers) return ASTRO_ERR_ARG;
+    *layers = c->layers;
+    for (uint32_t l = 0; l < c->layers; l++) {
+        if (cap) cap[l] = c->cap[l];
+        if (off) off[l] = c->off[l];
+    }
+    return ASTRO_OK;
+}
+
+extern "C" int astro_sync(astro_ctx* c) {
+    if (!c) return ASTRO_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_check(astro_ctx* c) {
+    if (!c) return ASTRO_ERR_ARG;
+    uint32_t w = 0;
+    HIPCHK(c, hipMemcpyAsync(&w, c->err_word, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (w) {
+        launch_fill_u32(c->err_word, 0, 1, c->stream);
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        std::string m = "device error word:";
+        if (w & ERR_SPIN) m += " look-back wait bound exceeded;";
+        if (w & ERR_STACK) m += " traversal stack overflow;";
+        if (w & ERR_CAPACITY) m += " layer capacity exceeded;";
+        return fail(c, ASTRO_ERR_DEVICE, m);
+    }
+    return ASTRO_OK;
+}
+
+// ------------------------------------------------------------------------------- sorting
+// Stable LSD sort of count pairs through the context's two temporaries.  kin/iin are not
+// modified unless they alias the temporaries.
+static int radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
+                      uint32_t* iout, uint32_t cap, const uint32_t* d_count,
+                      const RadixPlan& plan, const uint32_t* seg_of) {
+    hipStream_t s = c->stream;
+    if (cap == 0 || plan.npasses == 0) return ASTRO_OK;
+    const uint64_t per_pass = (uint64_t)sort_tiles(cap) * 256 + 1;
+    const uint64_t need = (uint64_t)plan.npasses * 256 + plan.npasses * per_pass;
+    if (need > c->scratch_words) return fail(c, ASTRO_ERR_CAPACITY, "sort scratch too small");
+    uint32_t* hist = c->scratch;
+    uint32_t* status = c->scratch + (uint64_t)plan.npasses * 256;
+    launch_fill_u32(c->scratch, 0, need, s);
+    launch_radix_hist(kin, d_count, cap, seg_of, plan, hist, s);
+    const uint64_t* ksrc = kin;
+    const uint32_t* isrc = iin;
+    for (uint32_t p = 0; p < plan.npasses; p++) {
+        const bool last = p + 1 == plan.npasses;
+        uint64_t* kdst = last ? kout : (p % 2 == 0 ? c->tmp_keys_a : c->tmp_keys_b);
+        uint32_t* idst = last ? iout : (p % 2 == 0 ? c->tmp_idx_a : c->tmp_idx_b);
+        launch_radix_pass(ksrc, isrc, kdst, idst, d_count, cap, hist + p * 256, plan.pass[p],
+                          seg_of, status + p * per_pass, c->err_word, s);
+        ksrc = kdst;
+        isrc = idst;
+    }
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+static RadixPlan key_plan(uint32_t key_bits, uint32_t nsys) {
+    RadixPlan plan{};
+    uint32_t passes = (key_bits + 7) / 8;
+    for (uint32_t p = 0; p < passes; p++) plan.pass[plan.npasses++] = {8 * p, 0};
+    if (nsys > 1) {
+        uint32_t sb = 0;
+        while ((1ull << sb) < nsys) sb++;
+        for (uint32_t p = 0; p < (sb + 7) / 8; p++) plan.pass[plan.npasses++] = {8 * p, 1};
+    }
+    return plan;
+}
+
+extern "C" int astro_radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_t* iin,
+                                uint64_t* kout, uint32_t* iout, uint32_t estimated_len,
+                                const uint32_t* d_count) {
+    if (!c || !kin || !iin || !kout || !iout || !d_count) return ASTRO_ERR_ARG;
+    if (estimated_len == 0) return ASTRO_OK;  // RadixSort.swift:106
+    if (estimated_len > c->cfg.max_particles)
+        return fail(c, ASTRO_ERR_CAPACITY, "estimated_len beyond the sorter's max length");
+    RadixPlan plan = key_plan(64, 1);
+    if ((const void*)kin == (const void*)kout || (const void*)iin == (const void*)iout) {
+        // in place: stage through the second temporary first
+        launch_copy_u64_count(kin, c->tmp_keys_b, d_count, estimated_len, c->stream);
+        launch_copy_u32_count(iin, c->tmp_idx_b, d_count, estimated_len, c->stream);
+        kin = c->tmp_keys_b;
+        iin = c->tmp_idx_b;
+        // 8 passes: pass 0 writes temp a, so temp b is free to hold the input
+    }
+    return radix_sort(c, kin, iin, kout, iout, estimated_len, d_count, plan, nullptr);
+}
+
+static uint32_t* scan_scratch(astro_ctx* c, uint32_t len) {
+    const uint64_t need = scan_tiles(len) + 1;
+    uint32_t* p = c->scratch + (c->scratch_words - need);
+    launch_fill_u32(p, 0, need, c->stream);
+    return p;
+}
+
+extern "C" int astro_scan(astro_ctx* c, const uint32_t* in, uint32_t* out, uint32_t len) {
+    if (!c || (!in && len) || (!out && len)) return ASTRO_ERR_ARG;
+    if (len == 0) return ASTRO_OK;
+    if (scan_tiles(len) + 1 > c->scratch_words)
+        return fail(c, ASTRO_ERR_CAPACITY, "scan longer than the context supports");
+    uint32_t* st = scan_scratch(c, len);
+    launch_scan(in, out, len, st, c->err_word, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_split_pass(astro_ctx* c, const uint64_t* kin, const uint32_t* iin,
+                                uint64_t* kout, uint32_t* iout, uint32_t bit,
+                                uint32_t estimated_len, const uint32_t* d_count) {
+    if (!c || !kin || !iin || !kout || !iout || !d_count || bit > 63) return ASTRO_ERR_ARG;
+    if (estimated_len == 0) return ASTRO_OK;
+    if (estimated_len > c->cfg.max_particles)
+        return fail(c, ASTRO_ERR_CAPACITY, "estimated_len beyond the sorter's max length");
+    launch_split_prep(kin, c->ebuf, bit, d_count, estimated_len, c->stream);
+    int rc = astro_scan(c, c->ebuf, c->fbuf, estimated_len);
+    if (rc) return rc;
+    launch_split_scatter(kin, iin, kout, iout, bit, c->ebuf, c->fbuf, d_count, estimated_len,
+                         c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// ------------------------------------------------------------------ stage calls (Renderer)
+static int need_particles(astro_ctx* c) {
+    if (c->ns == 0) return fail(c, ASTRO_ERR_STATE, "no particles uploaded");
+    return ASTRO_OK;
+}
+
+static void enqueue_cube(astro_ctx* c, uint32_t count) {
+    hipStream_t s = c->stream;
+    if (c->params.quantizer != ASTRO_QUANT_BBOX_POW2) return;
+    const bool seg = c->nsys > 1;
+    launch_bbox_init(c->minmax, c->nsys, s);
+    launch_bbox(c->pos, count, seg ? c->sys_of : nullptr, c->minmax, s);
+    launch_cube_finalize(c->minmax, c->cubes, c->nsys, s);
+}
+
+extern "C" int astro_generate_morton_codes(astro_ctx* c, uint32_t count) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    if (count > c->nt) return fail(c, ASTRO_ERR_CAPACITY, "count beyond the tree sources");
+    enqueue_cube(c, count);
+    launch_morton(c->pos, count, c->nsys > 1 ? c->sys_of : nullptr, c->cubes, c->params,
+                  c->codes, c->unsorted_codes, nullptr, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_fill_indices(astro_ctx* c, uint32_t* d_buf, uint32_t start, uint32_t end) {
+    if (!c || !d_buf) return ASTRO_ERR_ARG;
+    launch_iota(d_buf, start, end, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_clear_buffer(astro_ctx* c, void* d_buf, uint32_t count,
+                                  uint32_t elem_bytes) {
+    if (!c || !d_buf) return ASTRO_ERR_ARG;
+    if (elem_bytes == 8)
+        launch_fill_u64(static_cast<uint64_t*>(d_buf), ASTRO_INVALID_MORTON_CODE, count,
+                        c->stream);
+    else if (elem_bytes == 4)
+        launch_fill_u32(static_cast<uint32_t*>(d_buf), 0u, count, c->stream);
+    else
+        return fail(c, ASTRO_ERR_ARG, "elem_bytes must be 4 or 8");
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_clear_octree_node_buffer(astro_ctx* c, OctreeNode* d_nodes,
+                                              uint32_t count) {
+    if (!c || !d_nodes) return ASTRO_ERR_ARG;
+    launch_clear_onodes(d_nodes, count, c->stream);
+    if (d_nodes == c->onodes) c->onodes_dirty = true;
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_count_unique_morton_codes(astro_ctx* c, int aggregate, uint32_t count) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (count > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "count too large");
+    if (count == 0) return ASTRO_OK;
+    hipStream_t s = c->stream;
+    launch_fill_u32(c->flags, 0, count, s);  // Renderer.swift:572-574
+    launch_fill_u32(c->scan, 0, count, s);
+    launch_fill_u32(c->unique_idx, 0, count, s);
+    launch_mark_uniques(c->sorted_codes, c->flags, count, aggregate ? 1u : 0u, s);
+    if (int rc = astro_scan(c, c->flags, c->scan, count)) return rc;
+    launch_scatter_uniques(c->flags, c->scan, c->unique_idx, count, s);
+    launch_copy_last_scan(c->scan, c->flags, c->parent_count, c->child_count, count, s);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+static uint32_t fp64_from(const astro_ctx* c) {
+    const uint32_t nl = c->params.bits_per_axis + 1;
+    return c->params.fp64_top_layers >= nl ? 0u : nl - c->params.fp64_top_layers;
+}
+
+extern "C" int astro_aggregate_leaf_nodes(astro_ctx* c, uint32_t count) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    if (count > c->nt) return fail(c, ASTRO_ERR_CAPACITY, "count beyond the tree sources");
+    launch_aggregate_leaf(c->sorted_codes, c->sorted_idx, c->unique_idx, c->pos, c->parent_count,
+                          c->onodes + c->off[0], c->unsorted_codes, c->unsorted_idx, count,
+                          c->params.quirk_last_leaf, count, fp64_from(c) == 0, c->stream);
+    c->onodes_dirty = true;
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_aggregate_internal_nodes(astro_ctx* c, uint32_t layer, uint32_t in_off,
+                                              uint32_t out_off, uint32_t node_count) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (layer == 0 || layer >= c->layers) return fail(c, ASTRO_ERR_ARG, "layer out of range");
+    if ((uint64_t)out_off + node_count > c->node_capacity)
+        return fail(c, ASTRO_ERR_CAPACITY, "output range beyond the node buffer");
+    launch_aggregate_nodes(c->onodes, c->sorted_codes, c->sorted_idx, c->unique_idx,
+                           c->parent_count, in_off, out_off, layer, c->unsorted_codes,
+                           c->unsorted_idx, c->child_count, node_count, layer >= fp64_from(c),
+                           c->stream);
+    c->onodes_dirty = true;
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// Renderer.swift:714-755.  The per-layer sort of already-ascending codes is kept, as in the
+// reference; it is the identity.
+extern "C" int astro_build_internal_nodes(astro_ctx* c, uint32_t count) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    for (uint32_t l = 1; l < c->layers; l++) {
+        int rc;
+        if ((rc = astro_clear_buffer(c, c->sorted_codes, count, 8))) return rc;
+        if ((rc = astro_clear_buffer(c, c->sorted_idx, count, 4))) return rc;
+        if ((rc = astro_radix_sort(c, c->unsorted_codes, c->unsorted_idx, c->sorted_codes,
+                                   c->sorted_idx, (uint32_t)c->cap[l - 1], c->parent_count)))
+            return rc;
+        if ((rc = astro_count_unique_morton_codes(c, 1, count))) return rc;
+        if ((rc = astro_clear_buffer(c, c->unsorted_codes, count, 8))) return rc;
+        if ((rc = astro_clear_buffer(c, c->unsorted_idx, count, 4))) return rc;
+        if ((rc = astro_aggregate_internal_nodes(c, l, (uint32_t)c->off[l - 1],
+                                                 (uint32_t)c->off[l], (uint32_t)c->cap[l])))
+            return rc;
+    }
+    return ASTRO_OK;
+}
+
+static ForceArgs force_args(astro_ctx* c) {
+    ForceArgs a{};
+    a.tnodes = c->tnodes;
+    a.onodes = c->onodes;
+    a.pos = c->pos;
+    a.type = c->type;
+    a.sorted_idx = c->sorted_idx;
+    a.leaf_of_sorted = c->leaf_of_sorted;
+    a.sys_of = c->nsys > 1 ? c->sys_of : nullptr;
+    a.cubes = c->cubes;
+    a.codes = c->codes;
+    a.force = c->force;
+    a.err = c->err_word;
+    a.stats = nullptr;
+    a.off0 = c->off[0];
+    a.root_off = c->off[c->layers - 1];
+    a.ns = c->ns;
+    a.nt = c->nt;
+    a.p = c->params;
+    return a;
+}
+
+extern "C" int astro_barnes_hut(astro_ctx* c) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    if (c->nsys > 1) return fail(c, ASTRO_ERR_STATE, "stage traversal is single-system");
+    launch_force_reference(force_args(c), c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_update_spheres(astro_ctx* c, float dt) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// ------------------------------------------------------------------------ the fused path
+static TreeArgs tree_args(astro_ctx* c) {
+    TreeArgs a{};
+    a.pos = c->pos;
+    a.sys_of = c->sys_of;
+    a.sorted_keys = c->sorted_codes;
+    a.sorted_idx = c->sorted_idx;
+    a.tnodes = c->tnodes;
+    a.onodes = c->emit ? c->onodes : nullptr;
+    a.counts = c->layer_counts;
+    a.leaf_of_sorted = c->leaf_of_sorted;
+    a.err = c->err_word;
+    for (uint32_t l = 0; l < c->layers; l++) {
+        a.off[l] = c->off[l];
+        a.cap[l] = c->cap[l];
+    }
+    a.layers = c->layers;
+    a.bits = c->params.bits_per_axis;
+    a.nt = c->nt;
+    a.fp64_from = fp64_from(c);
+    a.quirk_last_leaf = c->params.quirk_last_leaf;
+    a.segmented = c->nsys > 1;
+    return a;
+}
+
+static int enqueue_bbox_morton(astro_ctx* c) {
+    enqueue_cube(c, c->nt);
+    launch_morton(c->pos, c->nt, c->nsys > 1 ? c->sys_of : nullptr, c->cubes, c->params,
+                  c->codes, c->unsorted_codes, c->unsorted_idx, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+static int enqueue_sort(astro_ctx* c) {
+    RadixPlan plan = key_plan(3 * c->params.bits_per_axis, c->nsys);
+    return radix_sort(c, c->unsorted_codes, c->unsorted_idx, c->sorted_codes, c->sorted_idx,
+                      c->nt, c->dev_consts + 1, plan, c->nsys > 1 ? c->sys_of : nullptr);
+}
+
+static int enqueue_layers(astro_ctx* c) {
+    hipStream_t s = c->stream;
+    TreeArgs a = tree_args(c);
+    // scratch after the sort's region: one status block per layer
+    const RadixPlan plan = key_plan(3 * c->params.bits_per_axis, c->nsys);
+    uint64_t base = (uint64_t)plan.npasses * 256 +
+                    (uint64_t)plan.npasses * ((uint64_t)sort_tiles(c->nt) * 256 + 1);
+    uint64_t need = scan_tiles(c->nt) + 1;
+    for (uint32_t l = 1; l < c->layers; l++) need += scan_tiles((uint32_t)c->cap[l - 1]) + 1;
+    if (base + need > c->scratch_words)
+        return fail(c, ASTRO_ERR_CAPACITY, "layer scratch too small");
+    uint32_t* st = c->scratch + base;
+    launch_fill_u32(st, 0, need, s);
+    if (a.onodes && c->onodes_dirty) {
+        launch_clear_onodes(c->onodes, c->node_capacity, s);
+        launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
+        c->onodes_dirty = false;
+    }
+    launch_leaf_layer(a, st, s);
+    st += scan_tiles(c->nt) + 1;
+    for (uint32_t l = 1; l < c->layers; l++) {
+        launch_internal_layer(a, l, st, s);
+        st += scan_tiles((uint32_t)c->cap[l - 1]) + 1;
+    }
+    if (a.onodes) {
+        launch_clear_stale(a, c->layer_counts_prev, s);
+        launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, s);
+    }
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+static int enqueue_build(astro_ctx* c) {
+    int rc;
+    if ((rc = enqueue_bbox_morton(c))) return rc;
+    if ((rc = enqueue_sort(c))) return rc;
+    return enqueue_layers(c);
+}
+
+static int enqueue_forces(astro_ctx* c, unsigned long long* stats) {
+    ForceArgs a = force_args(c);
+    a.stats = stats;
+    launch_force_compact(a, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_build_tree(astro_ctx* c) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    return enqueue_build(c);
+}
+
+extern "C" int astro_compute_forces(astro_ctx* c) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    return enqueue_forces(c, nullptr);
+}
+
+static int enqueue_step(astro_ctx* c, float dt) {
+    int rc;
+    if ((rc = enqueue_build(c))) return rc;
+    if ((rc = enqueue_forces(c, nullptr))) return rc;
+    launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->use_graph) {
+        for (uint32_t k = 0; k < nsteps; k++)
+            if (int rc = enqueue_step(c, dt)) return rc;
+        return ASTRO_OK;
+    }
+    if (!c->graph_valid || c->graph_dt != dt || c->onodes_dirty) {
+        invalidate_graph(c);
+        if (c->emit && c->onodes_dirty) {
+            // do the one-off full clear eagerly so that the graph holds only the steady step
+            launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
+            launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
+            c->onodes_dirty = false;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(c, dt);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, ASTRO_ERR_HIP, "graph capture failed");
+        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, ASTRO_ERR_HIP, "graph instantiate failed");
+        c->graph_valid = true;
+        c->graph_dt = dt;
+    }
+    for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* stage_ms) {
+    if (!c || !stage_ms) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const uint32_t ne = ASTRO_STAGE_COUNT + 1;
+    std::vector<hipEvent_t> ev((size_t)nsteps * ne);
+    for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+    int rc = ASTRO_OK;
+    for (uint32_t k = 0; k < nsteps && !rc; k++) {
+        hipEvent_t* E = &ev[(size_t)k * ne];
+        hipEventRecord(E[0], c->stream);
+        rc = enqueue_bbox_morton(c);
+        hipEventRecord(E[1], c->stream);
+        if (!rc) rc = enqueue_sort(c);
+        hipEventRecord(E[2], c->stream);
+        if (!rc) rc = enqueue_layers(c);
+        hipEventRecord(E[3], c->stream);
+        if (!rc) rc = enqueue_forces(c, nullptr);
+        hipEventRecord(E[4], c->stream);
+        launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
+        hipEventRecord(E[5], c->stream);
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    for (uint32_t s = 0; s < ASTRO_STAGE_COUNT; s++) stage_ms[s] = 0.0f;
+    if (!rc && se == hipSuccess) {
+        for (uint32_t k = 0; k < nsteps; k++) {
+            hipEvent_t* E = &ev[(size_t)k * ne];
+            for (uint32_t s = 0; s < ASTRO_STAGE_COUNT; s++) {
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, E[s], E[s + 1]);
+                stage_ms[s] += ms;
+            }
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    if (rc) return rc;
+    if (se != hipSuccess) return fail(c, ASTRO_ERR_HIP, hipGetErrorString(se));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_traversal_stats(astro_ctx* c, astro_stats* out) {
+    if (!c || !out) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    launch_fill_u64(reinterpret_cast<uint64_t*>(c->stats), 0, 4, c->stream);
+    // the counting traversal writes the same forces as the plain one
+    if (int rc = enqueue_forces(c, c->stats)) return rc;
+    unsigned long long h[4];
+    HIPCHK(c, hipMemcpyAsync(h, c->stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->visits = h[0];
+    out->leaf_terms = h[1];
+    out->monopole_terms = h[2];
+    out->opened = h[3];
+    return ASTRO_OK;
+}

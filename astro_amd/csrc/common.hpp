@@ -1,0 +1,138 @@
+// common.hpp -- device-side types and helpers shared by the libastro kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/astro/bridge.h"
+
+namespace astro {
+
+// Compact traversal node, 32 B (one half of a 64-B line).  Children of a node are a
+// contiguous run of the layer below (the build writes each layer in ascending key order),
+// so a (first, count) pair replaces the reference's children[8] (Astro/bridge.h:62).
+struct alignas(16) TNode {
+    float x, y, z, m;   // centre of mass, total mass
+    uint32_t first;     // absolute index of the first child (leaves: ASTRO_INVALID_CHILD)
+    uint32_t info;      // layer | nchild << 5 | head level << 9
+    uint32_t head;      // sorted position of the node's first particle
+    uint32_t pad;
+};
+static_assert(sizeof(TNode) == 32, "TNode must be 32 B");
+
+__host__ __device__ inline uint32_t tn_info(uint32_t layer, uint32_t nchild, uint32_t h) {
+    return layer | (nchild << 5) | (h << 9);
+}
+__host__ __device__ inline uint32_t tn_layer(uint32_t info) { return info & 31u; }
+__host__ __device__ inline uint32_t tn_nchild(uint32_t info) { return (info >> 5) & 15u; }
+__host__ __device__ inline uint32_t tn_level(uint32_t info) { return (info >> 9) & 31u; }
+
+// Error word bits raised by kernels (read back by astro_check).
+enum : uint32_t {
+    ERR_SPIN = 1u,       // a look-back wait exceeded its bound
+    ERR_STACK = 2u,      // traversal stack overflow
+    ERR_CAPACITY = 4u,   // a layer had more nodes than its capacity
+};
+
+constexpr uint32_t kMaxLayers = 24;   // B <= 21 -> 22 layers
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// Decoupled look-back status word: 2 flag bits over a 30-bit count.
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1u;
+
+__device__ inline uint32_t ld_acquire(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_release(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Popcount of the lanes below this one in a 64-lane mask.
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Inclusive scan across the 64 lanes of a wave.
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a block of NT threads (NT multiple of 64, <= 1024).  `tmp` is an LDS
+// array of NT/64 + 1 words.  Returns the exclusive prefix; *total receives the block sum.
+template <int NT>
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) tmp[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < NW; w++) {
+            uint32_t t = tmp[w];
+            tmp[w] = acc;
+            acc += t;
+        }
+        tmp[NW] = acc;
+    }
+    __syncthreads();
+    uint32_t r = tmp[wave] + inc - v;
+    *total = tmp[NW];
+    __syncthreads();
+    return r;
+}
+
+// Decoupled look-back for one value per tile (Merrill & Garland, "Single-pass Parallel Prefix
+// Scan with Decoupled Look-back", 2016).  Called by one thread.  Publishes the tile's
+// aggregate, walks back to the nearest inclusive prefix, publishes its own inclusive prefix
+// and returns the exclusive prefix of this tile.  Waits are bounded: on timeout the error
+// word is raised and the partial sum returned, so every wave still exits.
+__device__ inline uint32_t lookback_exclusive(uint32_t* status, uint32_t tile, uint32_t agg,
+                                              uint32_t* err) {
+    if (tile == 0) {
+        st_release(&status[0], kFlagInc | agg);
+        return 0;
+    }
+    st_release(&status[tile], kFlagAgg | agg);
+    uint32_t prefix = 0;
+    int64_t t = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (t >= 0) {
+        uint32_t s = ld_acquire(&status[t]);
+        uint32_t f = s & ~kValMask;
+        if (f == 0u) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, ERR_SPIN);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        prefix += s & kValMask;
+        if (f == kFlagInc) break;
+        t--;
+    }
+    st_release(&status[tile], kFlagInc | (prefix + agg));
+    return prefix;
+}
+
+// Float <-> order-preserving unsigned key, for atomic min/max of floats.
+__device__ inline uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+}  // namespace astro

@@ -1,0 +1,599 @@
+// k_tree.hip -- bounding cube, Morton keys and the layered leaf-to-root octree for gfx950.
+//
+// The reference builds each layer with clears, an identity re-sort, mark/scan/scatter of
+// segment heads, a count copy and an aggregation dispatch (Renderer.swift:570-755).  Here a
+// layer is one kernel: head flags (a node starts a parent when its key differs from its
+// predecessor above bit 3l), a tile scan with decoupled look-back for the parent index, and
+// the head thread aggregating its (at most 8) contiguous children in the reference's serial
+// order.  Aggregation order and arithmetic match the oracle exactly.
+#include "launch.hpp"
+
+namespace astro {
+
+// ------------------------------------------------------------------------- bounding cube
+__global__ void k_bbox_init(uint32_t* mm, uint32_t nsys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsys * 6) mm[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+}
+void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s) {
+    hipLaunchKernelGGL(k_bbox_init, dim3((nsys * 6 + 255) / 256), dim3(256), 0, s, minmax, nsys);
+}
+
+constexpr uint32_t kBoxTile = 4096;
+
+__device__ inline float wave_min(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Per-system min/max of the positions in one tile; one atomic per (tile, system, value).
+__global__ __launch_bounds__(256) void k_bbox(const float4* __restrict__ pos, uint32_t n,
+                                              const uint32_t* __restrict__ sys_of,
+                                              uint32_t* mm) {
+    __shared__ float s_red[6][4];
+    const uint64_t base = (uint64_t)blockIdx.x * kBoxTile;
+    if (base >= n) return;
+    const uint64_t end = base + kBoxTile < n ? base + kBoxTile : n;
+    const uint32_t s0 = sys_of ? sys_of[base] : 0u;
+    const uint32_t s1 = sys_of ? sys_of[end - 1] : 0u;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t sys = s0; sys <= s1; sys++) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint64_t e = base + threadIdx.x; e < end; e += 256) {
+            if (sys_of && sys_of[e] != sys) continue;
+            float4 p = pos[e];
+            lo[0] = fminf(lo[0], p.x);
+            lo[1] = fminf(lo[1], p.y);
+            lo[2] = fminf(lo[2], p.z);
+            hi[0] = fmaxf(hi[0], p.x);
+            hi[1] = fmaxf(hi[1], p.y);
+            hi[2] = fmaxf(hi[2], p.z);
+        }
+        for (int a = 0; a < 3; a++) {
+            lo[a] = wave_min(lo[a]);
+            hi[a] = wave_max(hi[a]);
+        }
+        if (lane == 0)
+            for (int a = 0; a < 3; a++) {
+                s_red[a][wave] = lo[a];
+                s_red[3 + a][wave] = hi[a];
+            }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const int a = threadIdx.x;
+            float v = s_red[a][0];
+            for (int w = 1; w < 4; w++) v = a < 3 ? fminf(v, s_red[a][w]) : fmaxf(v, s_red[a][w]);
+            if (a < 3) {
+                if (v != INFINITY) atomicMin(&mm[sys * 6 + a], f2ord(v));
+            } else {
+                if (v != -INFINITY) atomicMax(&mm[sys * 6 + a], f2ord(v));
+            }
+        }
+        __syncthreads();
+    }
+}
+void launch_bbox(const float4* pos, uint32_t n, const uint32_t* sys_of, uint32_t* minmax,
+                 hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_bbox, dim3((n + kBoxTile - 1) / kBoxTile), dim3(256), 0, s, pos, n,
+                       sys_of, minmax);
+}
+
+// cube = {lo.x, lo.y, lo.z, side}: side = smallest power of two >= the largest extent.
+__global__ void k_cube_finalize(const uint32_t* mm, float4* cubes, uint32_t nsys) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    float lo[3], hi[3];
+    bool empty = false;
+    for (int a = 0; a < 3; a++) {
+        uint32_t kl = mm[s * 6 + a], kh = mm[s * 6 + 3 + a];
+        if (kl == 0xFFFFFFFFu || kh == 0u) empty = true;
+        lo[a] = ord2f(kl);
+        hi[a] = ord2f(kh);
+    }
+    if (empty)
+        for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0.0f;
+    float ext = 0.0f;
+    for (int a = 0; a < 3; a++) {
+        float e = hi[a] - lo[a];
+        ext = e > ext ? e : ext;
+    }
+    float side = 1.0f;
+    if (ext > 0.0f && isfinite(ext)) {
+        int e;
+        float m = frexpf(ext, &e);
+        side = ldexpf(1.0f, m == 0.5f ? e - 1 : e);
+    }
+    cubes[s] = make_float4(lo[0], lo[1], lo[2], side);
+}
+void launch_cube_finalize(const uint32_t* minmax, float4* cubes, uint32_t nsys, hipStream_t s) {
+    hipLaunchKernelGGL(k_cube_finalize, dim3((nsys + 63) / 64), dim3(64), 0, s, minmax, cubes,
+                       nsys);
+}
+
+// ------------------------------------------------------------------------------- Morton
+// Spread the low 21 bits of v to every third bit (bit i -> bit 3i).
+__device__ inline uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x001F00000000FFFFull;
+    x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+// float -> u32: NaN and negatives to 0, saturating, truncating (the oracle's definition).
+__device__ inline uint32_t cvt_u32_sat(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
+// MortonCode.metal:26-52 (generateMortonCodes) + Octree.metal:39-51 (fillIndices), fused:
+// key per tree source, written to the per-particle code array and the sort input, plus iota.
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ pos, uint32_t n,
+                                                const uint32_t* __restrict__ sys_of,
+                                                const float4* __restrict__ cubes,
+                                                astro_params p, uint64_t* __restrict__ codes,
+                                                uint64_t* __restrict__ codes2,
+                                                uint32_t* __restrict__ idx) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x = pos[i];
+    uint32_t q[3];
+    if (p.quantizer == ASTRO_QUANT_OFFSET_WRAP) {
+        q[0] = cvt_u32_sat(x.x + p.quant_offset);
+        q[1] = cvt_u32_sat(x.y + p.quant_offset);
+        q[2] = cvt_u32_sat(x.z + p.quant_offset);
+    } else {
+        float4 c = cubes[sys_of ? sys_of[i] : 0u];
+        int e;
+        frexpf(c.w, &e);
+        const float scale = ldexpf(1.0f, 21 - (e - 1));
+        const float v[3] = {x.x, x.y, x.z}, lo[3] = {c.x, c.y, c.z};
+        for (int a = 0; a < 3; a++) {
+            uint32_t t = cvt_u32_sat((v[a] - lo[a]) * scale);
+            q[a] = t > 0x1FFFFFu ? 0x1FFFFFu : t;
+        }
+    }
+    uint64_t key = (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2)) >>
+                   (63 - 3 * p.bits_per_axis);
+    if (codes) codes[i] = key;
+    if (codes2) codes2[i] = key;
+    if (idx) idx[i] = i;
+}
+void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const float4* cubes,
+                   const astro_params& p, uint64_t* codes, uint64_t* codes2, uint32_t* idx,
+                   hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, s, pos, n, sys_of, cubes,
+                       p, codes, codes2, idx);
+}
+
+// --------------------------------------------------------------------------- aggregation
+// Octree.metal:84-94 / 157-172: mass += m; com = fma(x, m, com); COM = com / mass.
+struct Agg32 {
+    float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    __device__ void add(float x, float y, float z, float w) {
+        m = m + w;
+        cx = __builtin_fmaf(x, w, cx);
+        cy = __builtin_fmaf(y, w, cy);
+        cz = __builtin_fmaf(z, w, cz);
+    }
+    __device__ void fin(float* o) const {
+        o[3] = m;
+        o[0] = m > 0.0f ? cx / m : 0.0f;
+        o[1] = m > 0.0f ? cy / m : 0.0f;
+        o[2] = m > 0.0f ? cz / m : 0.0f;
+    }
+};
+struct Agg64 {
+    double m = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    __device__ void add(float x, float y, float z, float w) {
+        double dw = w;
+        m += dw;
+        cx = __builtin_fma((double)x, dw, cx);
+        cy = __builtin_fma((double)y, dw, cy);
+        cz = __builtin_fma((double)z, dw, cz);
+    }
+    __device__ void fin(float* o) const {
+        o[3] = (float)m;
+        o[0] = m > 0.0 ? (float)(cx / m) : 0.0f;
+        o[1] = m > 0.0 ? (float)(cy / m) : 0.0f;
+        o[2] = m > 0.0 ? (float)(cz / m) : 0.0f;
+    }
+};
+
+__device__ inline void write_onode(OctreeNode* o, uint64_t code, const float* cm,
+                                   uint32_t first, uint32_t nchild, uint32_t layer) {
+    // emittedColor / emittedColorCenter stay zero: the physics never reads them.
+    float4* q = reinterpret_cast<float4*>(o);
+    q[0] = make_float4(__uint_as_float((uint32_t)code), __uint_as_float((uint32_t)(code >> 32)),
+                       0.0f, 0.0f);
+    q[1] = make_float4(cm[0], cm[1], cm[2], 0.0f);
+    q[2] = make_float4(cm[3], 0.0f, 0.0f, 0.0f);
+    q[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t ch[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) ch[c] = (uint32_t)c < nchild ? first + c : ASTRO_INVALID_CHILD;
+    q[5] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), __uint_as_float(ch[2]),
+                       __uint_as_float(ch[3]));
+    q[6] = make_float4(__uint_as_float(ch[4]), __uint_as_float(ch[5]), __uint_as_float(ch[6]),
+                       __uint_as_float(ch[7]));
+    q[7] = make_float4(__uint_as_float(layer), 0.0f, 0.0f, 0.0f);
+}
+
+__device__ inline uint32_t head_level(uint64_t a, uint64_t b, uint32_t bits) {
+    uint64_t x = a ^ b;  // nonzero for distinct keys
+    uint32_t msb = 63u - (uint32_t)__clzll((long long)x);
+    uint32_t h = msb / 3u;
+    return h < bits ? h : bits;
+}
+
+// ------------------------------------------------------------------------------ leaf layer
+// Octree.metal:282-318 + 53-117 fused: heads of equal-key runs in the sorted keys, their
+// leaf index by tile scan, and one leaf per head aggregated serially in sorted order.
+__device__ inline bool is_leaf_head(const TreeArgs& a, uint32_t j) {
+    if (j == 0) return true;
+    if (a.segmented && a.sys_of[j] != a.sys_of[j - 1]) return true;
+    return a.sorted_keys[j] != a.sorted_keys[j - 1];
+}
+
+template <typename AGG>
+__device__ void emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
+    uint32_t end = j + 1;
+    while (end < a.nt && !is_leaf_head(a, end)) end++;
+    // Octree.metal:71: the reference ends the last leaf's run at numUnique (= u + 1).
+    if (a.quirk_last_leaf && end == a.nt) end = u + 1;
+    AGG g;
+    for (uint32_t i = j; i < end; i++) {
+        float4 x = a.pos[a.sorted_idx[i]];
+        g.add(x.x, x.y, x.z, x.w);
+    }
+    float cm[4];
+    g.fin(cm);
+    const uint64_t key = a.sorted_keys[j];
+    uint32_t h = a.bits;
+    if (j > 0 && !(a.segmented && a.sys_of[j] != a.sys_of[j - 1]))
+        h = head_level(key, a.sorted_keys[j - 1], a.bits);
+    const uint64_t n = a.off[0] + u;
+    TNode t;
+    t.x = cm[0];
+    t.y = cm[1];
+    t.z = cm[2];
+    t.m = cm[3];
+    t.first = ASTRO_INVALID_CHILD;
+    t.info = tn_info(0, 0, h);
+    t.head = j;
+    t.pad = 0;
+    a.tnodes[n] = t;
+    if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_leaf_layer(TreeArgs a, uint32_t* status,
+                                                    uint32_t* counter) {
+    __shared__ uint32_t s_tile, s_prefix, s_tmp[8];
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base64 = (uint64_t)tile * kScanTile;
+    if (base64 >= a.nt) return;
+    const uint32_t j0 = (uint32_t)base64 + threadIdx.x * 8u;
+    uint32_t hmask = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) {
+        uint32_t j = j0 + i;
+        if (j < a.nt && is_leaf_head(a, j)) {
+            hmask |= 1u << i;
+            cnt++;
+        }
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<256>(cnt, s_tmp, &total);
+    if (threadIdx.x == 0) {
+        uint32_t pre = lookback_exclusive(status, tile, total, a.err);
+        s_prefix = pre;
+        if (base64 + kScanTile >= a.nt) a.counts[0] = pre + total;  // last tile
+    }
+    __syncthreads();
+    uint32_t u = s_prefix + excl;  // leaf index of the next head
+    const bool fp64 = a.fp64_from == 0;
+    for (uint32_t i = 0; i < 8; i++) {
+        uint32_t j = j0 + i;
+        if (j >= a.nt) break;
+        if (hmask & (1u << i)) {
+            if (fp64) emit_leaf<Agg64>(a, j, u);
+            else emit_leaf<Agg32>(a, j, u);
+            u++;
+        }
+        a.leaf_of_sorted[j] = u - 1;
+    }
+}
+
+void launch_leaf_layer(const TreeArgs& a, uint32_t* status, hipStream_t s) {
+    if (!a.nt) return;
+    uint32_t tiles = scan_tiles(a.nt);
+    hipLaunchKernelGGL(k_leaf_layer, dim3(tiles), dim3(256), 0, s, a, status, status + tiles);
+}
+
+// -------------------------------------------------------------------------- internal layer
+// Octree.metal:282-334 + 120-202 fused for layer l: child c (of layer l-1) starts a parent
+// when its head level is >= l; the parent aggregates children c, c+1, ... up to the next
+// head, in order, and records them as a contiguous (first, count) run.
+template <typename AGG>
+__device__ void emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint32_t p,
+                            uint32_t nchild_layer) {
+    const TNode* ch = a.tnodes + a.off[l - 1];
+    AGG g;
+    uint32_t k = 0;
+    uint32_t head = 0, level = 0;
+    while (c + k < nchild_layer) {
+        const TNode t = ch[c + k];
+        if (k > 0 && tn_level(t.info) >= l) break;
+        if (k == 0) {
+            head = t.head;
+            level = tn_level(t.info);
+        }
+        g.add(t.x, t.y, t.z, t.m);
+        k++;
+        if (k == 8) break;  // a parent holds at most 8 children (3 key bits per layer)
+    }
+    float cm[4];
+    g.fin(cm);
+    const uint64_t n = a.off[l] + p;
+    TNode t;
+    t.x = cm[0];
+    t.y = cm[1];
+    t.z = cm[2];
+    t.m = cm[3];
+    t.first = (uint32_t)(a.off[l - 1] + c);
+    t.info = tn_info(l, k, level);
+    t.head = head;
+    t.pad = 0;
+    a.tnodes[n] = t;
+    if (a.onodes)
+        write_onode(&a.onodes[n], a.sorted_keys[head] >> (3 * l), cm, t.first, k, l);
+}
+
+__global__ __launch_bounds__(256) void k_internal_layer(TreeArgs a, uint32_t l,
+                                                        uint32_t* status, uint32_t* counter) {
+    __shared__ uint32_t s_tile, s_prefix, s_tmp[8];
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t nchild_layer = a.counts[l - 1];
+    const uint32_t tile = s_tile;
+    const uint64_t base64 = (uint64_t)tile * kScanTile;
+    if (base64 >= nchild_layer) return;
+    const TNode* ch = a.tnodes + a.off[l - 1];
+    const uint32_t c0 = (uint32_t)base64 + threadIdx.x * 8u;
+    uint32_t hmask = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) {
+        uint32_t c = c0 + i;
+        if (c < nchild_layer && tn_level(ch[c].info) >= l) {
+            hmask |= 1u << i;
+            cnt++;
+        }
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<256>(cnt, s_tmp, &total);
+    if (threadIdx.x == 0) {
+        uint32_t pre = lookback_exclusive(status, tile, total, a.err);
+        s_prefix = pre;
+        if (base64 + kScanTile >= nchild_layer) {
+            a.counts[l] = pre + total;
+            if (pre + total > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
+        }
+    }
+    __syncthreads();
+    uint32_t p = s_prefix + excl;
+    const bool fp64 = l >= a.fp64_from;
+    for (uint32_t i = 0; i < 8; i++) {
+        if (!(hmask & (1u << i))) continue;
+        if (p < a.cap[l]) {
+            if (fp64) emit_parent<Agg64>(a, l, c0 + i, p, nchild_layer);
+            else emit_parent<Agg32>(a, l, c0 + i, p, nchild_layer);
+        }
+        p++;
+    }
+}
+
+void launch_internal_layer(const TreeArgs& a, uint32_t layer, uint32_t* status, hipStream_t s) {
+    uint64_t child_cap = a.cap[layer - 1];
+    if (!child_cap) return;
+    uint32_t tiles = scan_tiles((uint32_t)child_cap);
+    hipLaunchKernelGGL(k_internal_layer, dim3(tiles), dim3(256), 0, s, a, layer, status,
+                       status + tiles);
+}
+
+// Empty every OctreeNode slot that held a node in the previous build but not in this one
+// (Octree.metal:253-269 applied to the stale tail of each layer only).
+__global__ void k_clear_stale(TreeArgs a, const uint32_t* counts_prev) {
+    const uint32_t l = blockIdx.y;
+    const uint32_t lo = a.counts[l], hi = counts_prev[l];
+    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+         i += gridDim.x * blockDim.x) {
+        float cm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        write_onode(&a.onodes[a.off[l] + i], ASTRO_INVALID_MORTON_CODE, cm, 0, 0, 0);
+    }
+}
+void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s) {
+    if (!a.onodes) return;
+    hipLaunchKernelGGL(k_clear_stale, dim3(64, a.layers), dim3(256), 0, s, a, counts_prev);
+}
+
+// --------------------------------------------------- reference-shaped stage kernels
+// Octree.metal:282-303 (markUniques)
+__global__ void k_mark_uniques(const uint64_t* sorted, uint32_t* flags, uint32_t count,
+                               uint32_t aggregate) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    uint64_t k = sorted[g];
+    if (k == ASTRO_INVALID_MORTON_CODE) {
+        flags[g] = 0;
+        return;
+    }
+    bool u;
+    if (g == 0) u = true;
+    else if (aggregate == 1) u = (k >> 3) != (sorted[g - 1] >> 3);
+    else u = k != sorted[g - 1];
+    flags[g] = u ? 1u : 0u;
+}
+void launch_mark_uniques(const uint64_t* sorted, uint32_t* flags, uint32_t count,
+                         uint32_t aggregate, hipStream_t s) {
+    if (count)
+        hipLaunchKernelGGL(k_mark_uniques, dim3((count + 255) / 256), dim3(256), 0, s, sorted,
+                           flags, count, aggregate);
+}
+// Octree.metal:306-318 (scatterUniques)
+__global__ void k_scatter_uniques(const uint32_t* flags, const uint32_t* scan, uint32_t* uniq,
+                                  uint32_t count) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    if (flags[g]) uniq[scan[g]] = g;
+}
+void launch_scatter_uniques(const uint32_t* flags, const uint32_t* scan, uint32_t* uniq,
+                            uint32_t count, hipStream_t s) {
+    if (count)
+        hipLaunchKernelGGL(k_scatter_uniques, dim3((count + 255) / 256), dim3(256), 0, s, flags,
+                           scan, uniq, count);
+}
+// Octree.metal:321-334 (copyLastScanToParentCount)
+__global__ void k_copy_last_scan(const uint32_t* scan, const uint32_t* flags, uint32_t* parent,
+                                 uint32_t* child, uint32_t count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        uint32_t last = count - 1;
+        child[0] = parent[0];
+        parent[0] = scan[last] + flags[last];
+    }
+}
+void launch_copy_last_scan(const uint32_t* scan, const uint32_t* flags, uint32_t* parent,
+                           uint32_t* child, uint32_t count, hipStream_t s) {
+    if (count)
+        hipLaunchKernelGGL(k_copy_last_scan, dim3(1), dim3(64), 0, s, scan, flags, parent, child,
+                           count);
+}
+
+// Octree.metal:53-117 (aggregateLeafNodes): one thread per unique code.
+template <typename AGG>
+__device__ void agg_leaf_one(const uint64_t* sorted, const uint32_t* sorted_idx,
+                             const uint32_t* uniq, const float4* pos, uint32_t num_unique,
+                             OctreeNode* nodes, uint64_t* unsorted, uint32_t* unsorted_idx,
+                             uint32_t g, uint32_t quirk, uint32_t nt) {
+    uint32_t start = uniq[g];
+    uint32_t end = g + 1 < num_unique ? uniq[g + 1] : (quirk ? num_unique : nt);
+    AGG a;
+    for (uint32_t i = start; i < end; i++) {
+        float4 x = pos[sorted_idx[i]];
+        a.add(x.x, x.y, x.z, x.w);
+    }
+    float cm[4];
+    a.fin(cm);
+    uint64_t code = sorted[start];
+    write_onode(&nodes[g], code, cm, 0, 0, 0);
+    unsorted[g] = code;
+    unsorted_idx[g] = g;
+}
+__global__ void k_aggregate_leaf(const uint64_t* sorted, const uint32_t* sorted_idx,
+                                 const uint32_t* uniq, const float4* pos, const uint32_t* parent,
+                                 OctreeNode* nodes, uint64_t* unsorted, uint32_t* unsorted_idx,
+                                 uint32_t quirk, uint32_t nt, uint32_t fp64) {
+    uint32_t num_unique = parent[0];
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= num_unique) return;
+    if (fp64)
+        agg_leaf_one<Agg64>(sorted, sorted_idx, uniq, pos, num_unique, nodes, unsorted,
+                            unsorted_idx, g, quirk, nt);
+    else
+        agg_leaf_one<Agg32>(sorted, sorted_idx, uniq, pos, num_unique, nodes, unsorted,
+                            unsorted_idx, g, quirk, nt);
+}
+void launch_aggregate_leaf(const uint64_t* sorted, const uint32_t* sorted_idx,
+                           const uint32_t* uniq, const float4* pos, const uint32_t* parent,
+                           OctreeNode* nodes, uint64_t* unsorted, uint32_t* unsorted_idx,
+                           uint32_t grid_count, uint32_t quirk, uint32_t nt, uint32_t fp64,
+                           hipStream_t s) {
+    if (grid_count)
+        hipLaunchKernelGGL(k_aggregate_leaf, dim3((grid_count + 63) / 64), dim3(64), 0, s,
+                           sorted, sorted_idx, uniq, pos, parent, nodes, unsorted, unsorted_idx,
+                           quirk, nt, fp64);
+}
+
+// Octree.metal:120-202 (aggregateNodes): one thread per parent at `layer`.
+template <typename AGG>
+__device__ void agg_node_one(OctreeNode* nodes, const uint64_t* sorted,
+                             const uint32_t* sorted_idx, const uint32_t* uniq,
+                             uint32_t num_unique, uint32_t in_off, uint32_t out_off,
+                             uint32_t layer, uint64_t* unsorted, uint32_t* unsorted_idx,
+                             uint32_t child_count, uint32_t g) {
+    uint32_t start = uniq[g];
+    uint32_t end = g + 1 < num_unique ? uniq[g + 1] : child_count;
+    AGG a;
+    uint32_t ch[8];
+    uint32_t nc = 0;
+    for (int c = 0; c < 8; c++) ch[c] = ASTRO_INVALID_CHILD;
+    for (uint32_t i = start; i < end; i++) {
+        uint32_t ni = sorted_idx[i] + in_off;
+        const OctreeNode* c = &nodes[ni];
+        a.add(c->centerOfMass[0], c->centerOfMass[1], c->centerOfMass[2], c->totalMass);
+        if (nc < 8) ch[nc++] = ni;
+    }
+    float cm[4];
+    a.fin(cm);
+    uint64_t code = sorted[start] >> 3;
+    OctreeNode* o = &nodes[g + out_off];
+    write_onode(o, code, cm, 0, 0, layer);
+    for (int c = 0; c < 8; c++) o->children[c] = ch[c];
+    unsorted[g] = code;
+    unsorted_idx[g] = g;
+}
+__global__ void k_aggregate_nodes(OctreeNode* nodes, const uint64_t* sorted,
+                                  const uint32_t* sorted_idx, const uint32_t* uniq,
+                                  const uint32_t* parent, uint32_t in_off, uint32_t out_off,
+                                  uint32_t layer, uint64_t* unsorted, uint32_t* unsorted_idx,
+                                  const uint32_t* child, uint32_t fp64) {
+    uint32_t num_unique = parent[0];
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= num_unique) return;
+    uint32_t child_count = child[0];
+    if (fp64)
+        agg_node_one<Agg64>(nodes, sorted, sorted_idx, uniq, num_unique, in_off, out_off, layer,
+                            unsorted, unsorted_idx, child_count, g);
+    else
+        agg_node_one<Agg32>(nodes, sorted, sorted_idx, uniq, num_unique, in_off, out_off, layer,
+                            unsorted, unsorted_idx, child_count, g);
+}
+void launch_aggregate_nodes(OctreeNode* nodes, const uint64_t* sorted, const uint32_t* sorted_idx,
+                            const uint32_t* uniq, const uint32_t* parent, uint32_t in_off,
+                            uint32_t out_off, uint32_t layer, uint64_t* unsorted,
+                            uint32_t* unsorted_idx, const uint32_t* child, uint32_t node_count,
+                            uint32_t fp64, hipStream_t s) {
+    if (node_count)
+        hipLaunchKernelGGL(k_aggregate_nodes, dim3((node_count + 63) / 64), dim3(64), 0, s, nodes,
+                           sorted, sorted_idx, uniq, parent, in_off, out_off, layer, unsorted,
+                           unsorted_idx, child, fp64);
+}
+
+// Octree.metal:253-269 (clearOctreeNodeBuffer), guarded.
+__global__ void k_clear_onodes(OctreeNode* nodes, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float cm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        write_onode(&nodes[i], ASTRO_INVALID_MORTON_CODE, cm, 0, 0, 0);
+    }
+}
+void launch_clear_onodes(OctreeNode* nodes, uint64_t count, hipStream_t s) {
+    if (!count) return;
+    uint64_t b = (count + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_clear_onodes, dim3((uint32_t)b), dim3(256), 0, s, nodes, count);
+}
+
+}  // namespace astro

@@ -1,0 +1,138 @@
+// launch.hpp -- host-side launchers of the libastro kernels (defined in the k_*.hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/astro/astro.h"
+#include "common.hpp"
+
+namespace astro {
+
+// ---- k_sort.hip ---------------------------------------------------------------------------
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortItems = 8;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;  // keys per tile
+constexpr uint32_t kScanTile = 2048;                       // values per scan tile
+
+inline uint32_t sort_tiles(uint32_t n) { return (n + kSortTile - 1) / kSortTile; }
+inline uint32_t scan_tiles(uint32_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
+void launch_iota(uint32_t* buf, uint32_t start, uint32_t end, hipStream_t s);
+void launch_copy_u64_count(const uint64_t* in, uint64_t* out, const uint32_t* d_count,
+                           uint32_t cap, hipStream_t s);
+void launch_copy_u32_count(const uint32_t* in, uint32_t* out, const uint32_t* d_count,
+                           uint32_t cap, hipStream_t s);
+void launch_copy_u32(const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s);
+
+// Exclusive scan of len values; status holds scan_tiles(len) words + 1 counter word, zeroed.
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t len, uint32_t* status,
+                 uint32_t* err, hipStream_t s);
+
+// 1-bit split pass (RadixSort.metal:89-131 semantics).
+void launch_split_prep(const uint64_t* kin, uint32_t* e, uint32_t bit, const uint32_t* d_count,
+                       uint32_t cap, hipStream_t s);
+void launch_split_scatter(const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
+                          uint32_t* iout, uint32_t bit, const uint32_t* e, const uint32_t* f,
+                          const uint32_t* d_count, uint32_t cap, hipStream_t s);
+
+// LSD radix sort building blocks.  A "digit pass" p sorts by (key >> shift[p]) & 0xFF, or,
+// for a segment pass, by (seg_of[index] >> seg_shift) & 0xFF.
+struct RadixPassDesc {
+    uint32_t shift;      // key bit offset of the digit
+    uint32_t segment;    // 1: digit from seg_of[index] instead of the key
+};
+constexpr uint32_t kMaxRadixPasses = 10;
+struct RadixPlan {
+    uint32_t npasses;
+    RadixPassDesc pass[kMaxRadixPasses];
+};
+// hist: npasses x 256 words, zeroed.
+void launch_radix_hist(const uint64_t* keys, const uint32_t* d_count, uint32_t cap,
+                       const uint32_t* seg_of, const RadixPlan& plan, uint32_t* hist,
+                       hipStream_t s);
+// status: sort_tiles(cap) x 256 words + 1 tile counter, zeroed.
+void launch_radix_pass(const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
+                       uint32_t* iout, const uint32_t* d_count, uint32_t cap,
+                       const uint32_t* hist, RadixPassDesc desc, const uint32_t* seg_of,
+                       uint32_t* status, uint32_t* err, hipStream_t s);
+
+// ---- k_tree.hip ---------------------------------------------------------------------------
+struct TreeArgs {
+    const float4* pos;          // particle positions + mass
+    const uint32_t* sys_of;     // system of each particle / sorted position
+    const uint64_t* sorted_keys;
+    const uint32_t* sorted_idx;
+    TNode* tnodes;
+    OctreeNode* onodes;         // may be null (no emission)
+    uint32_t* counts;           // [layers] valid nodes per layer
+    uint32_t* leaf_of_sorted;   // [nt]
+    uint32_t* err;
+    uint64_t off[kMaxLayers];
+    uint64_t cap[kMaxLayers];
+    uint32_t layers;            // B + 1
+    uint32_t bits;              // B
+    uint32_t nt;                // sources
+    uint32_t fp64_from;         // first layer aggregated in FP64
+    uint32_t quirk_last_leaf;
+    uint32_t segmented;         // 1: heads also at system starts
+};
+
+void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s);
+void launch_bbox(const float4* pos, uint32_t n, const uint32_t* sys_of, uint32_t* minmax,
+                 hipStream_t s);
+void launch_cube_finalize(const uint32_t* minmax, float4* cubes, uint32_t nsys, hipStream_t s);
+void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const float4* cubes,
+                   const astro_params& p, uint64_t* codes, uint64_t* codes2, uint32_t* idx,
+                   hipStream_t s);
+// status: scan_tiles(cap) words + 1 counter per layer region, zeroed.
+void launch_leaf_layer(const TreeArgs& a, uint32_t* status, hipStream_t s);
+void launch_internal_layer(const TreeArgs& a, uint32_t layer, uint32_t* status, hipStream_t s);
+void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s);
+
+// Reference-shaped stage kernels (Octree.metal live kernels).
+void launch_mark_uniques(const uint64_t* sorted, uint32_t* flags, uint32_t count,
+                         uint32_t aggregate, hipStream_t s);
+void launch_scatter_uniques(const uint32_t* flags, const uint32_t* scan, uint32_t* uniq,
+                            uint32_t count, hipStream_t s);
+void launch_copy_last_scan(const uint32_t* scan, const uint32_t* flags, uint32_t* parent,
+                           uint32_t* child, uint32_t count, hipStream_t s);
+void launch_aggregate_leaf(const uint64_t* sorted, const uint32_t* sorted_idx,
+                           const uint32_t* uniq, const float4* pos, const uint32_t* parent,
+                           OctreeNode* nodes, uint64_t* unsorted, uint32_t* unsorted_idx,
+                           uint32_t grid_count, uint32_t quirk, uint32_t nt, uint32_t fp64,
+                           hipStream_t s);
+void launch_aggregate_nodes(OctreeNode* nodes, const uint64_t* sorted, const uint32_t* sorted_idx,
+                            const uint32_t* uniq, const uint32_t* parent, uint32_t in_off,
+                            uint32_t out_off, uint32_t layer, uint64_t* unsorted,
+                            uint32_t* unsorted_idx, const uint32_t* child, uint32_t node_count,
+                            uint32_t fp64, hipStream_t s);
+void launch_clear_onodes(OctreeNode* nodes, uint64_t count, hipStream_t s);
+
+// ---- k_force.hip --------------------------------------------------------------------------
+struct ForceArgs {
+    const TNode* tnodes;
+    const OctreeNode* onodes;
+    const float4* pos;
+    const uint32_t* type;
+    const uint32_t* sorted_idx;
+    const uint32_t* leaf_of_sorted;
+    const uint32_t* sys_of;
+    const float4* cubes;
+    const uint64_t* codes;      // per-particle keys (reference-layout traversal self test)
+    float4* force;
+    uint32_t* err;
+    unsigned long long* stats;  // 4 counters, or null
+    uint64_t off0;              // leaf layer offset
+    uint64_t root_off;          // top layer offset (root of system s at root_off + s)
+    uint32_t ns, nt;
+    astro_params p;
+};
+void launch_force_compact(const ForceArgs& a, hipStream_t s);
+void launch_force_reference(const ForceArgs& a, hipStream_t s);
+void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,
+                   uint32_t n, float dt, const astro_params& p, hipStream_t s);
+
+}  // namespace astro

@@ -1,0 +1,175 @@
+/*
+ * astro.h -- C-ABI of libastro, the MI355X (gfx950) Barnes-Hut step.
+ *
+ * Conventions
+ *   - every function returns an int status: ASTRO_OK (0) or a negative ASTRO_ERR_*;
+ *     astro_last_error() gives the message of the last failure on that context;
+ *   - no exceptions cross this boundary; no torch or HIP types appear in it;
+ *   - pointers named d_* are device pointers (from astro_get_buffer or the caller's own
+ *     device allocations); all others are host pointers;
+ *   - every call only enqueues work on the context's stream unless it says otherwise;
+ *     variable counts stay in device memory, as in the reference (Renderer.swift:518-519,
+ *     RadixSort.metal:117,140, Octree.metal:65,136,143);
+ *   - a context is used from one thread at a time (the reference drives one command queue
+ *     from one thread, Renderer.swift:117); separate contexts are independent.
+ */
+#ifndef ASTRO_ASTRO_H
+#define ASTRO_ASTRO_H
+
+#include "bridge.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASTRO_OK 0
+#define ASTRO_ERR_ARG (-1)      /* invalid argument                                    */
+#define ASTRO_ERR_HIP (-2)      /* a HIP runtime call failed                           */
+#define ASTRO_ERR_CAPACITY (-3) /* size beyond what the context was created for        */
+#define ASTRO_ERR_STATE (-4)    /* call out of order (e.g. no particles uploaded)      */
+#define ASTRO_ERR_DEVICE (-5)   /* a kernel raised its error word (see astro_check)    */
+#define ASTRO_ERR_NOMEM (-6)    /* device allocation failed / HBM budget exceeded      */
+
+typedef struct astro_ctx astro_ctx;
+
+/* Sizing of a context: the equivalent of Renderer.makeBuffers (Renderer.swift:336-391) plus
+ * the sorter temporaries (RadixSort.swift:40-67), allocated once. */
+typedef struct {
+    uint32_t max_particles;     /* N_s: particles integrated (force targets)              */
+    uint32_t max_tree;          /* N_t: tree sources (the first N_t particles); 0 = N_s   */
+    uint32_t bits_per_axis;     /* largest B that will be used (tree capacity)           */
+    uint32_t max_systems;       /* independent systems per batched step (>= 1)           */
+    uint32_t emit_octree_nodes; /* 1: keep the 128-B OctreeNode array up to date         */
+    int32_t device;             /* HIP device ordinal                                    */
+} astro_config;
+
+/* Buffers reachable through astro_get_buffer (names follow Renderer.swift:56-70). */
+enum {
+    ASTRO_BUF_POSITION_MASS = 0,   /* float4 {x,y,z,m} per particle                       */
+    ASTRO_BUF_VELOCITY_RADIUS = 1, /* float4 {vx,vy,vz,radius}                            */
+    ASTRO_BUF_TYPE = 2,            /* uint32 type per particle                            */
+    ASTRO_BUF_FORCE = 3,           /* float4 {fx,fy,fz,0}                                 */
+    ASTRO_BUF_MORTON_CODES = 4,    /* uint64 per tree source (unsorted, per particle)     */
+    ASTRO_BUF_UNSORTED_CODES = 5,  /* uint64                                              */
+    ASTRO_BUF_UNSORTED_INDICES = 6,/* uint32                                              */
+    ASTRO_BUF_SORTED_CODES = 7,    /* uint64                                              */
+    ASTRO_BUF_SORTED_INDICES = 8,  /* uint32                                              */
+    ASTRO_BUF_UNIQUE_INDICES = 9,  /* uint32                                              */
+    ASTRO_BUF_PARENT_COUNT = 10,   /* uint32[1]                                           */
+    ASTRO_BUF_CHILD_COUNT = 11,    /* uint32[1]                                           */
+    ASTRO_BUF_FLAGS = 12,          /* uint32                                              */
+    ASTRO_BUF_SCAN = 13,           /* uint32                                              */
+    ASTRO_BUF_OCTREE_NODES = 14,   /* OctreeNode[sum of layer capacities]                 */
+    ASTRO_BUF_TREE_NODES = 15,     /* compact traversal nodes, 32 B each                  */
+    ASTRO_BUF_LAYER_COUNTS = 16,   /* uint32[layers]: valid nodes per layer, last build   */
+    ASTRO_BUF_LEAF_OF_SORTED = 17, /* uint32 leaf index of each sorted source             */
+    ASTRO_BUF_CUBES = 18,          /* float4 {lo.x, lo.y, lo.z, side} per system          */
+    ASTRO_BUF_COUNT
+};
+
+/* Traversal statistics, summed over targets (for the work figures of a step). */
+typedef struct {
+    uint64_t visits;
+    uint64_t leaf_terms;
+    uint64_t monopole_terms;
+    uint64_t opened;
+} astro_stats;
+
+/* ---- lifetime and parameters ---------------------------------------------------------- */
+int astro_create(const astro_config* cfg, astro_ctx** out);
+int astro_destroy(astro_ctx* ctx);
+const char* astro_last_error(const astro_ctx* ctx);
+int astro_device_count(int* count);
+void astro_params_compat(astro_params* p);                    /* the reference's constants */
+void astro_params_physics(astro_params* p, uint32_t bits_per_axis);
+int astro_set_params(astro_ctx* ctx, const astro_params* p);
+int astro_get_params(const astro_ctx* ctx, astro_params* p);
+/* 1: the fused build also writes the 128-B OctreeNode array (default: config value). */
+int astro_set_emit_octree_nodes(astro_ctx* ctx, int on);
+/* 1: astro_step replays one captured hipGraph per step (default). */
+int astro_set_use_graph(astro_ctx* ctx, int on);
+
+/* ---- state transfer (reference byte layouts; replaces makeSpheres' memcpy,
+ *      Renderer.swift:490-492) -------------------------------------------------------- */
+int astro_upload(astro_ctx* ctx, const PositionMass* pos, const VelocityRadius* vel,
+                 const ColorType* col, uint32_t n_particles, uint32_t n_tree);
+int astro_download(astro_ctx* ctx, PositionMass* pos, VelocityRadius* vel, ColorType* col);
+int astro_download_forces(astro_ctx* ctx, float* force4); /* float3 in 16-B slots */
+int astro_get_buffer(astro_ctx* ctx, int which, void** d_ptr, uint64_t* bytes);
+int astro_memcpy_d2h(astro_ctx* ctx, void* host, const void* d_src, uint64_t bytes);
+int astro_memcpy_h2d(astro_ctx* ctx, void* d_dst, const void* host, uint64_t bytes);
+int astro_layer_table(const astro_ctx* ctx, uint32_t* layers, uint64_t* cap, uint64_t* off);
+int astro_sync(astro_ctx* ctx);
+int astro_check(astro_ctx* ctx); /* syncs; ASTRO_ERR_DEVICE if a kernel raised an error */
+
+/* ---- stage calls: the Renderer / RadixSort dispatch surface --------------------------- */
+/* Renderer.swift:556-568 -> MortonCode.metal:26-52 (writes morton and unsorted codes) */
+int astro_generate_morton_codes(astro_ctx* ctx, uint32_t count);
+/* Renderer.swift:394-409 -> Octree.metal:39-51 */
+int astro_fill_indices(astro_ctx* ctx, uint32_t* d_buf, uint32_t start, uint32_t end);
+/* Renderer.swift:411-427 -> Octree.metal:239-251: u32 <- 0 (elem 4), u64 <- INVALID (8) */
+int astro_clear_buffer(astro_ctx* ctx, void* d_buf, uint32_t count, uint32_t elem_bytes);
+/* Renderer.swift:429-439 -> Octree.metal:253-269 */
+int astro_clear_octree_node_buffer(astro_ctx* ctx, OctreeNode* d_nodes, uint32_t count);
+/* RadixSort.swift:98-185: stable ascending sort of (u64 key, u32 index) pairs; the number
+ * of pairs is read from d_count (<= estimated_len) on the device. */
+int astro_radix_sort(astro_ctx* ctx, const uint64_t* d_keys_in, const uint32_t* d_idx_in,
+                     uint64_t* d_keys_out, uint32_t* d_idx_out, uint32_t estimated_len,
+                     const uint32_t* d_count);
+/* RadixSort.swift:225-279: exclusive scan of len u32 values */
+int astro_scan(astro_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32_t len);
+/* RadixSort.swift:312-349: one stable 1-bit split pass */
+int astro_split_pass(astro_ctx* ctx, const uint64_t* d_keys_in, const uint32_t* d_idx_in,
+                     uint64_t* d_keys_out, uint32_t* d_idx_out, uint32_t bit,
+                     uint32_t estimated_len, const uint32_t* d_count);
+/* Renderer.swift:570-625 */
+int astro_count_unique_morton_codes(astro_ctx* ctx, int aggregate, uint32_t count);
+/* Renderer.swift:654-672 -> Octree.metal:53-117 */
+int astro_aggregate_leaf_nodes(astro_ctx* ctx, uint32_t count);
+/* Renderer.swift:674-712 -> Octree.metal:120-202 */
+int astro_aggregate_internal_nodes(astro_ctx* ctx, uint32_t layer, uint32_t input_offset,
+                                   uint32_t output_offset, uint32_t node_count);
+/* Renderer.swift:714-755 */
+int astro_build_internal_nodes(astro_ctx* ctx, uint32_t count);
+/* Renderer.swift:851-869 -> BarnesHut.metal:22-101, over the OctreeNode array */
+int astro_barnes_hut(astro_ctx* ctx);
+/* Renderer.swift:885-899 -> BarnesHut.metal:169-196 */
+int astro_update_spheres(astro_ctx* ctx, float dt);
+
+/* ---- the fused MI355X path ------------------------------------------------------------ */
+/* Renderer.swift:511-534 in one pass per layer: bbox, Morton, LSD sort, layered build. */
+int astro_build_tree(astro_ctx* ctx);
+/* Renderer.swift:536-538 over the compact tree: forces for every particle. */
+int astro_compute_forces(astro_ctx* ctx);
+/* Renderer.swift:495-542 with explicit dt: build + forces + update, nsteps times. */
+int astro_step(astro_ctx* ctx, float dt, uint32_t nsteps);
+/* Same as astro_step with hipEvents around each stage of each step; stage_ms receives the
+ * per-stage sums (ASTRO_STAGE_COUNT floats). */
+enum {
+    ASTRO_STAGE_BBOX_MORTON = 0,
+    ASTRO_STAGE_SORT = 1,
+    ASTRO_STAGE_BUILD = 2,
+    ASTRO_STAGE_FORCE = 3,
+    ASTRO_STAGE_UPDATE = 4,
+    ASTRO_STAGE_COUNT = 5
+};
+int astro_step_timed(astro_ctx* ctx, float dt, uint32_t nsteps, float* stage_ms);
+/* Work figures of the current tree and positions (one extra counting traversal). */
+int astro_traversal_stats(astro_ctx* ctx, astro_stats* out);
+
+/* ---- seeded initial conditions (host only; replace Renderer.makeSpheres,
+ *      Renderer.swift:441-493, whose draws are unseeded) -------------------------------- */
+int astro_ic_plummer(uint32_t n, uint64_t seed, PositionMass* pos, VelocityRadius* vel,
+                     ColorType* col);
+int astro_ic_two_disks(uint32_t n_per_disk, uint64_t seed, float separation, float pericentre,
+                       PositionMass* pos, VelocityRadius* vel, ColorType* col);
+int astro_ic_plummer_background(uint32_t n, uint64_t seed, float bg_fraction, float cube_side,
+                                float slope, float m_lo, float m_hi, PositionMass* pos,
+                                VelocityRadius* vel, ColorType* col);
+int astro_ic_batch_sizes(uint32_t n_systems, uint32_t n_min, uint32_t n_max, uint64_t seed,
+                         uint32_t* sizes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASTRO_ASTRO_H */
