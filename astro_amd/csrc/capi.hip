@@ -12,83 +12,14 @@
 #include <string>
 #include <vector>
 
-#include "../../include/astro/astro.h"
-#include "launch.hpp"
+#include "ctx.hpp"
 
 using namespace astro;
 
-struct astro_ctx {
-    astro_config cfg{};
-    astro_params params{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-
-    uint32_t ns = 0, nt = 0, nsys = 1;
-    std::vector<uint32_t> sys_off;     // nsys + 1 particle offsets
-    std::vector<float> colors;         // host copy of ColorType.color (render-only data)
-
-    // particle state (SoA float4: 16-B rows instead of the 32-B reference structs)
-    float4 *pos = nullptr, *vel = nullptr, *force = nullptr;
-    uint32_t* type = nullptr;
-    uint32_t* sys_of = nullptr;
-    // keys and sort
-    uint64_t *codes = nullptr, *unsorted_codes = nullptr, *sorted_codes = nullptr;
-    uint64_t *tmp_keys_a = nullptr, *tmp_keys_b = nullptr;
-    uint32_t *unsorted_idx = nullptr, *sorted_idx = nullptr, *tmp_idx_a = nullptr,
-             *tmp_idx_b = nullptr;
-    uint32_t *unique_idx = nullptr, *flags = nullptr, *scan = nullptr, *ebuf = nullptr,
-             *fbuf = nullptr;
-    uint32_t *parent_count = nullptr, *child_count = nullptr;
-    uint32_t* dev_consts = nullptr;    // [0] = ns, [1] = nt
-    // tree
-    TNode* tnodes = nullptr;
-    OctreeNode* onodes = nullptr;
-    uint64_t node_capacity = 0;
-    uint32_t *layer_counts = nullptr, *layer_counts_prev = nullptr, *leaf_of_sorted = nullptr;
-    float4* cubes = nullptr;
-    uint32_t* minmax = nullptr;
-    uint32_t* err_word = nullptr;
-    unsigned long long* stats = nullptr;
-    uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
-    uint64_t scratch_words = 0;
-    bool emit = true;
-    bool onodes_dirty = true;          // stage calls wrote onodes outside the fused build
-
-    // layer table of the current particles
-    uint32_t layers = 0;
-    uint64_t cap[kMaxLayers] = {}, off[kMaxLayers] = {}, total = 0;
-
-    // graph of one fused step
-    hipGraphExec_t graph = nullptr;
-    float graph_dt = 0.0f;
-    bool graph_valid = false;
-    bool use_graph = true;
-};
-
-// ------------------------------------------------------------------------------- helpers
-static int fail(astro_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-#define HIPCHK(c, call)                                                                   \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail((c), ASTRO_ERR_HIP,                                               \
-                        std::string(#call) + ": " + hipGetErrorString(e_));               \
-    } while (0)
-#define LAUNCHCHK(c)                                                                      \
-    do {                                                                                  \
-        hipError_t e_ = hipGetLastError();                                                \
-        if (e_ != hipSuccess)                                                             \
-            return fail((c), ASTRO_ERR_HIP, std::string("kernel launch: ") +              \
-                                                hipGetErrorString(e_));                   \
-    } while (0)
-
 template <typename T>
-static hipError_t dalloc(T** p, uint64_t count) {
+static hipError_t dalloc(uint64_t& tally, T** p, uint64_t count) {
     if (count == 0) count = 1;
+    tally += count * sizeof(T);
     return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
 }
 
@@ -135,6 +66,7 @@ static int refresh_layers(astro_ctx* c) {
     if (c->total > c->node_capacity)
         return fail(c, ASTRO_ERR_CAPACITY, "tree needs more nodes than the context holds");
     invalidate_graph(c);
+    c->onodes_dirty = true;  // stale-tail clearing assumes an unchanged layer table
     return ASTRO_OK;
 }
 
@@ -194,7 +126,8 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
                     c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
-                    c->leaf_of_sorted, c->cubes, c->minmax, c->err_word, c->stats, c->scratch};
+                    c->leaf_of_sorted, c->cubes, c->minmax, c->err_word, c->stats, c->scratch,
+                    c->d_sys_off, c->phi, c->diag_partial};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -226,8 +159,8 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         uint64_t t = 0;
         for (uint32_t l = 0; l <= bits; l++) {
             uint32_t b = 3 * bits - 3 * l;
-            uint64_t possible = b >= 60 ? (1ull << 60) : (1ull << b);
-            possible = std::min<uint64_t>(possible * c->cfg.max_systems, 1ull << 60);
+            // S * 8^(B-l), saturated (S < 2^32, so 2^b * S fits when b < 32)
+            uint64_t possible = b >= 32 ? UINT64_MAX : (1ull << b) * c->cfg.max_systems;
             uint64_t cl = std::min<uint64_t>(NT, possible);
             t += cl;
             if (bits == cfg->bits_per_axis) cap_max[l] = cl;
@@ -250,43 +183,46 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     if (bad(hipSetDevice(c->device))) return ASTRO_ERR_HIP;
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
     hipError_t e = hipSuccess;
-    e = e ? e : dalloc(&c->pos, NP);
-    e = e ? e : dalloc(&c->vel, NP);
-    e = e ? e : dalloc(&c->force, NP);
-    e = e ? e : dalloc(&c->type, NP);
-    e = e ? e : dalloc(&c->sys_of, NP);
-    e = e ? e : dalloc(&c->codes, NP);
-    e = e ? e : dalloc(&c->unsorted_codes, NP);
-    e = e ? e : dalloc(&c->sorted_codes, NP);
-    e = e ? e : dalloc(&c->tmp_keys_a, NP);
-    e = e ? e : dalloc(&c->tmp_keys_b, NP);
-    e = e ? e : dalloc(&c->unsorted_idx, NP);
-    e = e ? e : dalloc(&c->sorted_idx, NP);
-    e = e ? e : dalloc(&c->tmp_idx_a, NP);
-    e = e ? e : dalloc(&c->tmp_idx_b, NP);
-    e = e ? e : dalloc(&c->unique_idx, NP);
-    e = e ? e : dalloc(&c->flags, NP);
-    e = e ? e : dalloc(&c->scan, NP);
-    e = e ? e : dalloc(&c->ebuf, NP);
-    e = e ? e : dalloc(&c->fbuf, NP);
-    e = e ? e : dalloc(&c->parent_count, 1);
-    e = e ? e : dalloc(&c->child_count, 1);
-    e = e ? e : dalloc(&c->dev_consts, 4);
-    e = e ? e : dalloc(&c->tnodes, cap_bound);
-    e = e ? e : dalloc(&c->onodes, cap_bound);
-    e = e ? e : dalloc(&c->layer_counts, kMaxLayers);
-    e = e ? e : dalloc(&c->layer_counts_prev, kMaxLayers);
-    e = e ? e : dalloc(&c->leaf_of_sorted, NT);
-    e = e ? e : dalloc(&c->cubes, c->cfg.max_systems);
-    e = e ? e : dalloc(&c->minmax, 6ull * c->cfg.max_systems);
-    e = e ? e : dalloc(&c->err_word, 1);
-    e = e ? e : dalloc(&c->stats, 4);
+    e = e ? e : dalloc(c->device_bytes, &c->pos, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->vel, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->force, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->type, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->sys_of, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->codes, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->unsorted_codes, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->sorted_codes, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->tmp_keys_a, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->tmp_keys_b, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->unsorted_idx, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->sorted_idx, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->tmp_idx_a, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->tmp_idx_b, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->unique_idx, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->flags, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->scan, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->ebuf, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->fbuf, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->parent_count, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->child_count, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->dev_consts, 4);
+    e = e ? e : dalloc(c->device_bytes, &c->tnodes, cap_bound);
+    e = e ? e : dalloc(c->device_bytes, &c->onodes, cap_bound);
+    e = e ? e : dalloc(c->device_bytes, &c->layer_counts, kMaxLayers);
+    e = e ? e : dalloc(c->device_bytes, &c->layer_counts_prev, kMaxLayers);
+    e = e ? e : dalloc(c->device_bytes, &c->leaf_of_sorted, NT);
+    e = e ? e : dalloc(c->device_bytes, &c->cubes, c->cfg.max_systems);
+    e = e ? e : dalloc(c->device_bytes, &c->minmax, 6ull * c->cfg.max_systems);
+    e = e ? e : dalloc(c->device_bytes, &c->err_word, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->stats, 4);
+    e = e ? e : dalloc(c->device_bytes, &c->d_sys_off, c->cfg.max_systems + 1ull);
+    e = e ? e : dalloc(c->device_bytes, &c->phi, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->diag_partial, (uint64_t)c->cfg.max_systems * kDiagChunks * 9);
     if (e != hipSuccess) {
         astro_destroy(c);
         return ASTRO_ERR_NOMEM;
     }
     c->scratch_words = scratch_need(NP, cfg->bits_per_axis, cap_max, cfg->bits_per_axis + 1);
-    if (bad(dalloc(&c->scratch, c->scratch_words))) return ASTRO_ERR_NOMEM;
+    if (bad(dalloc(c->device_bytes, &c->scratch, c->scratch_words))) return ASTRO_ERR_NOMEM;
     hipStream_t s = c->stream;
     launch_fill_u32(c->err_word, 0, 1, s);
     launch_fill_u32(c->layer_counts, 0, kMaxLayers, s);
@@ -349,12 +285,8 @@ static int set_counts(astro_ctx* c, uint32_t ns, uint32_t nt) {
     return ASTRO_OK;
 }
 
-extern "C" int astro_upload(astro_ctx* c, const PositionMass* pos, const VelocityRadius* vel,
-                            const ColorType* col, uint32_t n, uint32_t n_tree) {
-    if (!c || !pos) return ASTRO_ERR_ARG;
-    if (n > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "too many particles");
-    if (n_tree == 0 || n_tree > n) n_tree = n;
-    if (n_tree > c->cfg.max_tree) return fail(c, ASTRO_ERR_CAPACITY, "too many tree sources");
+static int upload_particles(astro_ctx* c, const PositionMass* pos, const VelocityRadius* vel,
+                            const ColorType* col, uint32_t n) {
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<float4> p4(n), v4(n);
     std::vector<uint32_t> ty(n, 0u);
@@ -380,12 +312,62 @@ extern "C" int astro_upload(astro_ctx* c, const PositionMass* pos, const Velocit
         HIPCHK(c, hipMemcpyAsync(c->type, ty.data(), n * sizeof(uint32_t),
                                  hipMemcpyHostToDevice, c->stream));
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ASTRO_OK;
+}
+
+extern "C" int astro_upload(astro_ctx* c, const PositionMass* pos, const VelocityRadius* vel,
+                            const ColorType* col, uint32_t n, uint32_t n_tree) {
+    if (!c || !pos) return ASTRO_ERR_ARG;
+    if (n > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "too many particles");
+    if (n_tree == 0 || n_tree > n) n_tree = n;
+    if (n_tree > c->cfg.max_tree) return fail(c, ASTRO_ERR_CAPACITY, "too many tree sources");
+    if (int rc = upload_particles(c, pos, vel, col, n)) return rc;
     launch_fill_u32(c->sys_of, 0, c->cfg.max_particles, c->stream);
     c->nsys = 1;
     c->sys_off = {0u, n};
+    HIPCHK(c, hipMemcpyAsync(c->d_sys_off, c->sys_off.data(), 2 * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, c->stream));
     int rc = set_counts(c, n, n_tree);
     if (rc) return rc;
     return refresh_layers(c);
+}
+
+// Batched layout: system s owns particles [offsets[s], offsets[s+1]); every particle is a
+// tree source; each system gets its own bounding cube, key range, tree and root.
+extern "C" int astro_upload_systems(astro_ctx* c, const PositionMass* pos,
+                                    const VelocityRadius* vel, const ColorType* col,
+                                    const uint32_t* offsets, uint32_t n_systems) {
+    if (!c || !pos || !offsets || n_systems == 0) return ASTRO_ERR_ARG;
+    if (n_systems > c->cfg.max_systems)
+        return fail(c, ASTRO_ERR_CAPACITY, "more systems than the context was created for");
+    if (offsets[0] != 0) return fail(c, ASTRO_ERR_ARG, "offsets[0] must be 0");
+    for (uint32_t s = 0; s < n_systems; s++)
+        if (offsets[s + 1] <= offsets[s])
+            return fail(c, ASTRO_ERR_ARG, "every system needs at least one particle");
+    const uint32_t n = offsets[n_systems];
+    if (n > c->cfg.max_particles || n > c->cfg.max_tree)
+        return fail(c, ASTRO_ERR_CAPACITY, "too many particles");
+    if (int rc = upload_particles(c, pos, vel, col, n)) return rc;
+    c->nsys = n_systems;
+    c->sys_off.assign(offsets, offsets + n_systems + 1);
+    std::vector<uint32_t> so(n);
+    for (uint32_t s = 0; s < n_systems; s++)
+        for (uint32_t i = offsets[s]; i < offsets[s + 1]; i++) so[i] = s;
+    if (n)
+        HIPCHK(c, hipMemcpyAsync(c->sys_of, so.data(), n * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_sys_off, c->sys_off.data(), (n_systems + 1) * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, c->stream));
+    int rc = set_counts(c, n, n);
+    if (rc) return rc;
+    return refresh_layers(c);
+}
+
+extern "C" int astro_system_count(const astro_ctx* c, uint32_t* n_systems) {
+    if (!c || !n_systems) return ASTRO_ERR_ARG;
+    *n_systems = c->nsys;
+    return ASTRO_OK;
 }
 
 extern "C" int astro_download(astro_ctx* c, PositionMass* pos, VelocityRadius* vel,
@@ -513,6 +495,7 @@ extern "C" int astro_check(astro_ctx* c) {
         if (w & ERR_SPIN) m += " look-back wait bound exceeded;";
         if (w & ERR_STACK) m += " traversal stack overflow;";
         if (w & ERR_CAPACITY) m += " layer capacity exceeded;";
+        if (w & ERR_NODE_INDEX) m += " node index outside the tree;";
         return fail(c, ASTRO_ERR_DEVICE, m);
     }
     return ASTRO_OK;
@@ -755,6 +738,9 @@ static ForceArgs force_args(astro_ctx* c) {
     a.stats = nullptr;
     a.off0 = c->off[0];
     a.root_off = c->off[c->layers - 1];
+    a.node_total = c->total;
+    a.phi = nullptr;
+    a.block = c->force_block;
     a.ns = c->ns;
     a.nt = c->nt;
     a.p = c->params;
@@ -977,3 +963,90 @@ extern "C" int astro_traversal_stats(astro_ctx* c, astro_stats* out) {
     out->opened = h[3];
     return ASTRO_OK;
 }
+
+extern "C" int astro_device_bytes(const astro_ctx* c, uint64_t* bytes) {
+    if (!c || !bytes) return ASTRO_ERR_ARG;
+    *bytes = c->device_bytes;
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_force_block(astro_ctx* c, uint32_t block) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (!force_block_supported(block)) return fail(c, ASTRO_ERR_ARG, "block must be 64/128/256/512");
+    c->force_block = block;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+// Times the force traversal over the current tree for every supported block size (median of
+// `reps` launches each) and keeps the fastest.  Builds the tree first.
+extern "C" int astro_autotune(astro_ctx* c, uint32_t reps, uint32_t* chosen, float* best_ms) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    if (reps == 0) reps = 3;
+    if (int rc = enqueue_build(c)) return rc;
+    const uint32_t cand[4] = {64, 128, 256, 512};
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    uint32_t best = c->force_block;
+    float best_t = 1e30f;
+    for (uint32_t b : cand) {
+        c->force_block = b;
+        if (int rc = enqueue_forces(c, nullptr)) return rc;  // warm
+        std::vector<float> ts;
+        for (uint32_t r = 0; r < reps; r++) {
+            hipEventRecord(e0, c->stream);
+            enqueue_forces(c, nullptr);
+            hipEventRecord(e1, c->stream);
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            ts.push_back(ms);
+        }
+        std::sort(ts.begin(), ts.end());
+        const float med = ts[ts.size() / 2];
+        if (med < best_t) {
+            best_t = med;
+            best = b;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->force_block = best;
+    invalidate_graph(c);
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+    return ASTRO_OK;
+}
+
+// FP64 diagnostics per system: {mass, kinetic, potential, px, py, pz, mx, my, mz} (9 doubles
+// per system).  The potential comes from one extra traversal of the current tree (the same
+// interactions as the forces, monopole -G M / sqrt(r^2 + eps^2)), so call it after a build.
+extern "C" int astro_diagnostics(astro_ctx* c, double* out) {
+    if (!c || !out) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    if (int rc = enqueue_build(c)) return rc;
+    ForceArgs a = force_args(c);
+    a.phi = c->phi;
+    launch_force_compact(a, c->stream);
+    launch_diag(c->pos, c->vel, c->phi, c->d_sys_off, c->nsys, kDiagChunks, c->diag_partial,
+                c->stream);
+    LAUNCHCHK(c);
+    std::vector<double> part((size_t)c->nsys * kDiagChunks * 9);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->diag_partial, part.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t s = 0; s < c->nsys; s++)
+        for (int k = 0; k < 9; k++) {
+            double acc = 0.0;
+            for (uint32_t ch = 0; ch < kDiagChunks; ch++) acc += part[((size_t)s * kDiagChunks + ch) * 9 + k];
+            out[(size_t)s * 9 + k] = acc;
+        }
+    return ASTRO_OK;
+}
+
+// ------------------------------------------------------------- shared with serve.hip
+int astro_refresh_layers(astro_ctx* c) { return refresh_layers(c); }
+int astro_set_counts(astro_ctx* c, uint32_t ns, uint32_t nt) { return set_counts(c, ns, nt); }
+int astro_enqueue_step(astro_ctx* c, float dt) { return enqueue_step(c, dt); }

@@ -32,6 +32,7 @@ enum : uint32_t {
     ERR_SPIN = 1u,       // a look-back wait exceeded its bound
     ERR_STACK = 2u,      // traversal stack overflow
     ERR_CAPACITY = 4u,   // a layer had more nodes than its capacity
+    ERR_NODE_INDEX = 8u, // a traversal met a node index outside the tree
 };
 
 constexpr uint32_t kMaxLayers = 24;   // B <= 21 -> 22 layers

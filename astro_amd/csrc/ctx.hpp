@@ -1,0 +1,93 @@
+// ctx.hpp -- the libastro context (private to the library's host code).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/astro/astro.h"
+#include "launch.hpp"
+
+constexpr uint32_t kDiagChunks = 64;
+
+struct astro_ctx {
+    using TNode = astro::TNode;
+    astro_config cfg{};
+    astro_params params{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    uint32_t ns = 0, nt = 0, nsys = 1;
+    std::vector<uint32_t> sys_off;     // nsys + 1 particle offsets
+    std::vector<float> colors;         // host copy of ColorType.color (render-only data)
+
+    // particle state (SoA float4: 16-B rows instead of the 32-B reference structs)
+    float4 *pos = nullptr, *vel = nullptr, *force = nullptr;
+    uint32_t* type = nullptr;
+    uint32_t* sys_of = nullptr;
+    // keys and sort
+    uint64_t *codes = nullptr, *unsorted_codes = nullptr, *sorted_codes = nullptr;
+    uint64_t *tmp_keys_a = nullptr, *tmp_keys_b = nullptr;
+    uint32_t *unsorted_idx = nullptr, *sorted_idx = nullptr, *tmp_idx_a = nullptr,
+             *tmp_idx_b = nullptr;
+    uint32_t *unique_idx = nullptr, *flags = nullptr, *scan = nullptr, *ebuf = nullptr,
+             *fbuf = nullptr;
+    uint32_t *parent_count = nullptr, *child_count = nullptr;
+    uint32_t* dev_consts = nullptr;    // [0] = ns, [1] = nt
+    // tree
+    TNode* tnodes = nullptr;
+    OctreeNode* onodes = nullptr;
+    uint64_t node_capacity = 0;
+    uint32_t *layer_counts = nullptr, *layer_counts_prev = nullptr, *leaf_of_sorted = nullptr;
+    float4* cubes = nullptr;
+    uint32_t* minmax = nullptr;
+    uint32_t* err_word = nullptr;
+    unsigned long long* stats = nullptr;
+    uint32_t* d_sys_off = nullptr;     // [max_systems + 1] system offsets on the device
+    double* phi = nullptr;             // [max_particles] FP64 potential (diagnostics)
+    double* diag_partial = nullptr;    // [max_systems * kDiagChunks * 9]
+    uint32_t force_block = 256;        // traversal block size (autotuned)
+    uint64_t device_bytes = 0;         // bytes allocated on the device at creation
+    uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
+    uint64_t scratch_words = 0;
+    bool emit = true;
+    bool onodes_dirty = true;          // stage calls wrote onodes outside the fused build
+
+    // layer table of the current particles
+    uint32_t layers = 0;
+    uint64_t cap[astro::kMaxLayers] = {}, off[astro::kMaxLayers] = {}, total = 0;
+
+    // graph of one fused step
+    hipGraphExec_t graph = nullptr;
+    float graph_dt = 0.0f;
+    bool graph_valid = false;
+    bool use_graph = true;
+};
+
+// ------------------------------------------------------------------------------- helpers
+inline int fail(astro_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define HIPCHK(c, call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), ASTRO_ERR_HIP,                                               \
+                        std::string(#call) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+#define LAUNCHCHK(c)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), ASTRO_ERR_HIP, std::string("kernel launch: ") +              \
+                                                hipGetErrorString(e_));                   \
+    } while (0)
+
+
+// host-side pieces shared by capi.hip and serve.hip
+int astro_refresh_layers(astro_ctx* c);
+int astro_set_counts(astro_ctx* c, uint32_t ns, uint32_t nt);
+int astro_enqueue_step(astro_ctx* c, float dt);

@@ -14,8 +14,6 @@
 
 namespace astro {
 
-constexpr uint32_t kForceThreads = 256;
-constexpr uint32_t kStackDepth = 24;
 
 struct Acc3 {
     float x = 0.0f, y = 0.0f, z = 0.0f;
@@ -55,16 +53,17 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
     return v;
 }
 
-template <bool KAHAN, bool STATS>
-__global__ __launch_bounds__(kForceThreads) void k_force_compact(ForceArgs a) {
-    __shared__ uint32_t s_stack[kStackDepth * kForceThreads];
+// MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64 potential per target.
+template <int NT, bool KAHAN, int MODE>
+__global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t depth) {
+    extern __shared__ uint32_t s_stack[];  // [depth][NT]
 
     // XCD-aware mapping: hardware places block b on XCD b % 8; give XCD x the x-th
     // contiguous eighth of the target tiles.
     const uint32_t nb = gridDim.x, b = blockIdx.x, xcd = b & 7u, q = nb >> 3, r = nb & 7u;
     const uint32_t tile = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
     const uint32_t tid = threadIdx.x;
-    const uint32_t j = tile * kForceThreads + tid;
+    const uint32_t j = tile * NT + tid;
 
     unsigned long long visits = 0, leafs = 0, monos = 0, opens = 0;
     if (j < a.ns) {
@@ -81,59 +80,76 @@ __global__ __launch_bounds__(kForceThreads) void k_force_compact(ForceArgs a) {
         const float cut = a.p.leaf_cutoff_r2, theta = a.p.theta, G = a.p.G;
 
         Acc3 F;
+        double phi = 0.0;
+        // The loop leaves only at its head: an overflow or a bad index empties the stack
+        // instead of breaking out of a divergent branch.
         uint32_t top_first = (uint32_t)(a.root_off + sys), top_cnt = 1, sp = 0;
         while (true) {
             if (top_cnt == 0) {
                 if (sp == 0) break;
                 sp--;
-                const uint32_t e = s_stack[sp * kForceThreads + tid];
+                const uint32_t e = s_stack[sp * NT + tid];
                 top_first = e >> 3;
                 top_cnt = (e & 7u) + 1u;
             }
             const uint32_t n = top_first + top_cnt - 1u;
             top_cnt--;
+            if (n >= a.node_total) {
+                atomicOr(a.err, ERR_NODE_INDEX);
+                top_cnt = 0;
+                sp = 0;
+                continue;
+            }
             const float4* np = reinterpret_cast<const float4*>(a.tnodes + n);
             const float4 cm = np[0];
-            const uint4 li = reinterpret_cast<const uint4*>(np)[1];
-            if (STATS) visits++;
+            const uint2 li = reinterpret_cast<const uint2*>(np)[2];
+            if (MODE == 1) visits++;
             const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
             float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
             r2 = fmaxf(r2, r2f);
-            const float rr = fmaxf(__fsqrt_rn(r2), rf);
+            const float rr = fmaxf(sqrtf(r2), rf);
             const uint32_t layer = tn_layer(li.y);
-            bool interact;
+            bool interact, open = false;
             if (layer == 0u) {
                 interact = (n != self) && (r2 > cut);
-                if (STATS) leafs += interact;
+                if (MODE == 1) leafs += interact;
             } else {
                 const float side = ldexpf(side0, (int)layer);
-                interact = __fdiv_rn(side, rr) < theta;
-                if (interact) {
-                    if (STATS) monos++;
+                interact = (side / rr) < theta;
+                open = !interact;
+                if (MODE == 1) {
+                    monos += interact;
+                    opens += open;
+                }
+            }
+            if (interact) {
+                const float gm = G * cm.w;
+                const float s = (gm / (r2 * rr));
+                accumulate<KAHAN>(F, s, dx, dy, dz);
+                if (MODE == 2) phi -= (double)gm / (double)rr;
+            }
+            if (open) {
+                const uint32_t entry = (top_first << 3) | (top_cnt - 1u);
+                const bool push = top_cnt > 0u;
+                if (push && sp >= depth) {
+                    atomicOr(a.err, ERR_STACK);
+                    top_cnt = 0;
+                    sp = 0;
                 } else {
-                    if (STATS) opens++;
-                    if (top_cnt > 0u) {
-                        if (sp >= kStackDepth) {
-                            atomicOr(a.err, ERR_STACK);
-                            break;
-                        }
-                        s_stack[sp * kForceThreads + tid] = (top_first << 3) | (top_cnt - 1u);
+                    if (push) {
+                        s_stack[sp * NT + tid] = entry;
                         sp++;
                     }
                     top_first = li.x;
                     top_cnt = tn_nchild(li.y);
                 }
             }
-            if (interact) {
-                const float gm = G * cm.w;
-                const float s = __fdiv_rn(gm, r2 * rr);
-                accumulate<KAHAN>(F, s, dx, dy, dz);
-            }
         }
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
+        if (MODE == 2) a.phi[pidx] = phi;
     }
-    if (STATS) {
+    if (MODE == 1) {
         visits = wave_sum_u64(visits);
         leafs = wave_sum_u64(leafs);
         monos = wave_sum_u64(monos);
@@ -147,16 +163,36 @@ __global__ __launch_bounds__(kForceThreads) void k_force_compact(ForceArgs a) {
     }
 }
 
+template <int NT>
+static void launch_force_nt(const ForceArgs& a, int mode, hipStream_t s) {
+    const uint32_t blocks = (a.ns + NT - 1) / NT;
+    const uint32_t depth = a.p.bits_per_axis + 2u;  // open levels <= layers - 1
+    const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
+    const bool k = a.p.kahan != 0;
+#define ASTRO_FORCE_LAUNCH(KH, MD) \
+    hipLaunchKernelGGL((k_force_compact<NT, KH, MD>), dim3(blocks), dim3(NT), lds, s, a, depth)
+    if (k) {
+        if (mode == 0) ASTRO_FORCE_LAUNCH(true, 0);
+        else if (mode == 1) ASTRO_FORCE_LAUNCH(true, 1);
+        else ASTRO_FORCE_LAUNCH(true, 2);
+    } else {
+        if (mode == 0) ASTRO_FORCE_LAUNCH(false, 0);
+        else if (mode == 1) ASTRO_FORCE_LAUNCH(false, 1);
+        else ASTRO_FORCE_LAUNCH(false, 2);
+    }
+#undef ASTRO_FORCE_LAUNCH
+}
+
+bool force_block_supported(uint32_t nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 512; }
+
 void launch_force_compact(const ForceArgs& a, hipStream_t s) {
     if (!a.ns) return;
-    const uint32_t blocks = (a.ns + kForceThreads - 1) / kForceThreads;
-    const bool st = a.stats != nullptr;
-    if (a.p.kahan) {
-        if (st) hipLaunchKernelGGL((k_force_compact<true, true>), dim3(blocks), dim3(kForceThreads), 0, s, a);
-        else hipLaunchKernelGGL((k_force_compact<true, false>), dim3(blocks), dim3(kForceThreads), 0, s, a);
-    } else {
-        if (st) hipLaunchKernelGGL((k_force_compact<false, true>), dim3(blocks), dim3(kForceThreads), 0, s, a);
-        else hipLaunchKernelGGL((k_force_compact<false, false>), dim3(blocks), dim3(kForceThreads), 0, s, a);
+    const int mode = a.phi ? 2 : (a.stats ? 1 : 0);
+    switch (a.block) {
+        case 64: launch_force_nt<64>(a, mode, s); break;
+        case 128: launch_force_nt<128>(a, mode, s); break;
+        case 512: launch_force_nt<512>(a, mode, s); break;
+        default: launch_force_nt<256>(a, mode, s); break;
     }
 }
 
@@ -181,6 +217,11 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
     while (sp > 0) {
         sp--;
         const uint32_t ni = s_stack[sp * 64u + tid];
+        if (ni >= a.node_total) {  // empty the stack; the loop leaves at its head
+            atomicOr(a.err, ERR_NODE_INDEX);
+            sp = 0;
+            continue;
+        }
         const OctreeNode* nd = &a.onodes[ni];
         if (nd->mortonCode == ASTRO_INVALID_MORTON_CODE) continue;
         const float dx = nd->centerOfMass[0] - p.x;
@@ -188,7 +229,7 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
         const float dz = nd->centerOfMass[2] - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + a.p.eps2;
         r2 = fmaxf(r2, a.p.r2_floor);
-        const float rr = fmaxf(__fsqrt_rn(r2), a.p.r_floor);
+        const float rr = fmaxf(sqrtf(r2), a.p.r_floor);
         const uint32_t layer = nd->layer;
         bool interact;
         if (layer == 0u) {
@@ -196,7 +237,7 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
             interact = !self && r2 > a.p.leaf_cutoff_r2;
         } else {
             const float side = ldexpf(side0, (int)layer);
-            interact = __fdiv_rn(side, rr) < a.p.theta;
+            interact = (side / rr) < a.p.theta;
             if (!interact) {
                 for (int c = 0; c < 8; c++) {
                     const uint32_t ch = nd->children[c];
@@ -207,7 +248,7 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
             }
         }
         if (interact) {
-            const float s = __fdiv_rn(a.p.G * nd->totalMass, r2 * rr);
+            const float s = ((a.p.G * nd->totalMass) / (r2 * rr));
             if (a.p.kahan) accumulate<true>(F, s, dx, dy, dz);
             else accumulate<false>(F, s, dx, dy, dz);
         }
@@ -256,6 +297,53 @@ void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t
     if (!n) return;
     hipLaunchKernelGGL(k_update, dim3((n + 255) / 256), dim3(256), 0, s, pos, vel, force, type, n,
                        dt, p);
+}
+
+// FP64 diagnostics: per (system, chunk) sums of {m, kinetic, potential (1/2 m phi), m v,
+// m x}; partial[(s * chunks + chunk) * 9 + k].  Summed on the host in a fixed order.
+__global__ __launch_bounds__(256) void k_diag(const float4* __restrict__ pos,
+                                              const float4* __restrict__ vel,
+                                              const double* __restrict__ phi,
+                                              const uint32_t* __restrict__ sys_off,
+                                              uint32_t chunks, double* __restrict__ partial) {
+    __shared__ double s_red[9][4];
+    const uint32_t sys = blockIdx.y, chunk = blockIdx.x;
+    const uint32_t b = sys_off[sys], e = sys_off[sys + 1];
+    const uint32_t len = e - b, per = (len + chunks - 1) / chunks;
+    const uint32_t lo = b + min(len, chunk * per), hi = b + min(len, (chunk + 1) * per);
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+        const float4 x = pos[i], v = vel[i];
+        const double m = x.w;
+        acc[0] += m;
+        acc[1] += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+        acc[2] += phi ? 0.5 * m * phi[i] : 0.0;
+        acc[3] += m * v.x;
+        acc[4] += m * v.y;
+        acc[5] += m * v.z;
+        acc[6] += m * x.x;
+        acc[7] += m * x.y;
+        acc[8] += m * x.z;
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 9; k++) {
+        double v = acc[k];
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (lane == 0) s_red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int k = threadIdx.x;
+        partial[((size_t)sys * chunks + chunk) * 9 + k] =
+            ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
+    }
+}
+
+void launch_diag(const float4* pos, const float4* vel, const double* phi,
+                 const uint32_t* d_sys_off, uint32_t nsys, uint32_t chunks, double* partial,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_diag, dim3(chunks, nsys), dim3(256), 0, s, pos, vel, phi, d_sys_off,
+                       chunks, partial);
 }
 
 }  // namespace astro

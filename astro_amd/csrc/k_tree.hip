@@ -429,6 +429,23 @@ void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s)
     hipLaunchKernelGGL(k_clear_stale, dim3(64, a.layers), dim3(256), 0, s, a, counts_prev);
 }
 
+// System of every position from the offsets (binary search over nsys + 1 offsets).
+__global__ void k_fill_sys_of(uint32_t* sys_of, const uint32_t* off, uint32_t nsys, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = nsys;  // off[lo] <= i < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    sys_of[i] = lo;
+}
+void launch_fill_sys_of(uint32_t* sys_of, const uint32_t* d_off, uint32_t nsys, uint32_t n,
+                        hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_sys_of, dim3((n + 255) / 256), dim3(256), 0, s, sys_of, d_off, nsys, n);
+}
+
 // --------------------------------------------------- reference-shaped stage kernels
 // Octree.metal:282-303 (markUniques)
 __global__ void k_mark_uniques(const uint64_t* sorted, uint32_t* flags, uint32_t count,

@@ -90,6 +90,8 @@ void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const 
 // status: scan_tiles(cap) words + 1 counter per layer region, zeroed.
 void launch_leaf_layer(const TreeArgs& a, uint32_t* status, hipStream_t s);
 void launch_internal_layer(const TreeArgs& a, uint32_t layer, uint32_t* status, hipStream_t s);
+void launch_fill_sys_of(uint32_t* sys_of, const uint32_t* d_off, uint32_t nsys, uint32_t n,
+                        hipStream_t s);
 void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s);
 
 // Reference-shaped stage kernels (Octree.metal live kernels).
@@ -125,13 +127,20 @@ struct ForceArgs {
     float4* force;
     uint32_t* err;
     unsigned long long* stats;  // 4 counters, or null
+    double* phi;                // per-particle FP64 potential, or null
     uint64_t off0;              // leaf layer offset
     uint64_t root_off;          // top layer offset (root of system s at root_off + s)
+    uint64_t node_total;        // nodes in the current layer table (index bound)
     uint32_t ns, nt;
+    uint32_t block;             // traversal block size (64, 128, 256 or 512)
     astro_params p;
 };
+bool force_block_supported(uint32_t nt);
 void launch_force_compact(const ForceArgs& a, hipStream_t s);
 void launch_force_reference(const ForceArgs& a, hipStream_t s);
+void launch_diag(const float4* pos, const float4* vel, const double* phi,
+                 const uint32_t* d_sys_off, uint32_t nsys, uint32_t chunks, double* partial,
+                 hipStream_t s);
 void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,
                    uint32_t n, float dt, const astro_params& p, hipStream_t s);
 

@@ -93,6 +93,12 @@ int astro_set_use_graph(astro_ctx* ctx, int on);
  *      Renderer.swift:490-492) -------------------------------------------------------- */
 int astro_upload(astro_ctx* ctx, const PositionMass* pos, const VelocityRadius* vel,
                  const ColorType* col, uint32_t n_particles, uint32_t n_tree);
+/* Batched layout (no reference counterpart: the reference steps one system per frame):
+ * system s owns particles [offsets[s], offsets[s+1]), all of them tree sources; each system
+ * gets its own bounding cube, keys, tree and root, so results do not depend on the batch. */
+int astro_upload_systems(astro_ctx* ctx, const PositionMass* pos, const VelocityRadius* vel,
+                         const ColorType* col, const uint32_t* offsets, uint32_t n_systems);
+int astro_system_count(const astro_ctx* ctx, uint32_t* n_systems);
 int astro_download(astro_ctx* ctx, PositionMass* pos, VelocityRadius* vel, ColorType* col);
 int astro_download_forces(astro_ctx* ctx, float* force4); /* float3 in 16-B slots */
 int astro_get_buffer(astro_ctx* ctx, int which, void** d_ptr, uint64_t* bytes);
@@ -156,6 +162,51 @@ enum {
 int astro_step_timed(astro_ctx* ctx, float dt, uint32_t nsteps, float* stage_ms);
 /* Work figures of the current tree and positions (one extra counting traversal). */
 int astro_traversal_stats(astro_ctx* ctx, astro_stats* out);
+/* Traversal block size (64, 128, 256 or 512; default 256). */
+int astro_set_force_block(astro_ctx* ctx, uint32_t block);
+/* Builds the tree, times the traversal at every block size (median of reps launches) and
+ * keeps the fastest; returns it and its time. */
+int astro_autotune(astro_ctx* ctx, uint32_t reps, uint32_t* chosen, float* best_ms);
+/* Builds the tree and returns, per system, 9 FP64 sums: mass, kinetic energy, potential
+ * energy (1/2 sum m phi, phi from the same monopole interactions as the forces), momentum
+ * x/y/z and mass-weighted position x/y/z. */
+int astro_diagnostics(astro_ctx* ctx, double* out);
+/* Device memory the context allocated at creation. */
+int astro_device_bytes(const astro_ctx* ctx, uint64_t* bytes);
+
+/* ---- continuous-batching server (no reference counterpart: the reference renders one
+ *      system).  Requests wait FIFO; at each tick the server retires finished systems,
+ *      admits queued ones while the batch fits its HBM budget and capacities, and runs one
+ *      batched step.  A request's result does not depend on what it was batched with. ---- */
+typedef struct astro_server astro_server;
+typedef struct {
+    uint32_t max_particles;  /* particles the batch holds at once                         */
+    uint32_t max_systems;    /* systems the batch holds at once                           */
+    uint64_t hbm_budget;     /* device bytes admitted requests may occupy (0: no limit)   */
+    int32_t device;
+} astro_server_config;
+typedef struct {
+    uint64_t id;
+    uint32_t n, steps;
+    double t_submit, t_admit, t_done; /* seconds on the server's monotonic clock */
+} astro_completion;
+typedef struct {
+    uint32_t admitted, completed, active, queued, particles, active_after;
+    double step_ms;
+} astro_tick_info;
+int astro_server_create(const astro_server_config* cfg, const astro_params* p,
+                        astro_server** out);
+int astro_server_destroy(astro_server* s);
+const char* astro_server_last_error(const astro_server* s);
+int astro_server_device_bytes(const astro_server* s, uint64_t* bytes, uint64_t* per_particle);
+int astro_server_submit(astro_server* s, uint64_t id, const PositionMass* pos,
+                        const VelocityRadius* vel, const ColorType* col, uint32_t n,
+                        uint32_t steps);
+int astro_server_tick(astro_server* s, float dt, astro_tick_info* info);
+int astro_server_poll(astro_server* s, astro_completion* out, uint32_t max, uint32_t* n_out);
+int astro_server_fetch(astro_server* s, uint64_t id, PositionMass* pos, VelocityRadius* vel);
+int astro_server_counters(const astro_server* s, uint64_t* ticks, uint64_t* particle_steps,
+                          uint32_t* queued, uint32_t* active);
 
 /* ---- seeded initial conditions (host only; replace Renderer.makeSpheres,
  *      Renderer.swift:441-493, whose draws are unseeded) -------------------------------- */

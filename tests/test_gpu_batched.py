@@ -1,0 +1,189 @@
+"""Batched systems, the continuous-batching server, FP64 diagnostics and the autotuner on the
+GPU.  A system's result must not depend on what it is batched with: each system has its own
+cube, keys, tree and root, and no reduction crosses systems, so results are compared bit for
+bit against the same system run alone and against the per-system oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def make_batch(astro, sizes, seed0=100):
+    parts = [astro.ic_plummer(int(n), seed0 + k) for k, n in enumerate(sizes)]
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    col = np.concatenate([p[2] for p in parts])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    return parts, pos, vel, col, off
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batched_equals_alone_and_oracle(astro, orc, graph):
+    say("batched vs alone, graph", graph)
+    sizes = [1024, 3000, 1500, 7777, 2048, 1]
+    parts, pos, vel, col, off = make_batch(astro, sizes)
+    P = astro.params_physics(21)
+    steps = 3
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=8) as ctx:
+        ctx.set_params(P)
+        ctx.set_use_graph(graph)
+        ctx.upload_systems(pos, vel, col, off)
+        ctx.step(1e-3, steps)
+        ctx.check()
+        pb, vb, _ = ctx.download()
+    Po = orc.params_physics(21)
+    with astro.Context(max(sizes), bits_per_axis=21) as one:
+        one.set_params(P)
+        for k, (p, v, c) in enumerate(parts):
+            one.upload(p, v, c)
+            one.step(1e-3, steps)
+            one.check()
+            pa, va, _ = one.download()
+            sl = slice(off[k], off[k + 1])
+            assert pb[sl].tobytes() == pa.tobytes(), f"system {k} differs from alone"
+            assert vb[sl].tobytes() == va.tobytes()
+            po, vo = p, v
+            for _ in range(steps):
+                po, vo, _, _ = orc.step(po, vo, c, Po, 1e-3, threads=8)
+            assert np.array_equal(pa["position"], po["position"]), f"system {k} vs oracle"
+
+
+def test_batched_256_systems_properties(astro):
+    say("256 systems")
+    sizes = astro.ic_batch_sizes(256, 1024, 4096, 5)
+    parts, pos, vel, col, off = make_batch(astro, sizes, seed0=1000)
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=256) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload_systems(pos, vel, col, off)
+        ctx.build_tree()
+        ctx.check()
+        counts = ctx.layer_counts()
+        assert counts[-1] == 256  # one root per system
+        assert counts[0] <= len(pos)
+        keys = ctx.read("SORTED_CODES", np.uint64, len(pos))
+        perm = ctx.read("SORTED_INDICES", np.uint32, len(pos))
+        for k in range(0, 256, 17):
+            a, b = off[k], off[k + 1]
+            assert np.all((perm[a:b] >= a) & (perm[a:b] < b))  # segments stay in place
+            assert np.all(keys[a + 1:b] >= keys[a:b - 1])
+
+
+def run_server(astro, P, reqs, max_particles, max_systems, budget=0, dt=1e-3):
+    """reqs: list of (id, tick_of_arrival, pos, vel, col, steps).  Returns results, records."""
+    srv = astro.Server(P, max_particles, max_systems, budget)
+    pending = sorted(reqs, key=lambda r: r[1])
+    results, records, tick = {}, [], 0
+    while pending or srv.counters()["active"] or srv.counters()["queued"]:
+        while pending and pending[0][1] <= tick:
+            rid, _, p, v, c, steps = pending.pop(0)
+            srv.submit(rid, p, v, c, steps)
+        srv.tick(dt)
+        for rec in srv.poll():
+            records.append(rec)
+            results[rec["id"]] = srv.fetch(rec["id"], rec["n"])
+        tick += 1
+        assert tick < 10000
+    srv.close()
+    return results, records
+
+
+def test_server_alone_vs_mix(astro):
+    say("server alone vs mix")
+    P = astro.params_physics(21)
+    a = astro.ic_plummer(3000, 7)
+    alone, _ = run_server(astro, P, [(1, 0, *a, 6)], 100000, 16)
+    others = [(10 + k, k % 4, *astro.ic_plummer(1000 + 700 * k, 50 + k), 2 + 3 * k)
+              for k in range(6)]
+    mix, recs = run_server(astro, P, [(1, 2, *a, 6)] + others, 100000, 16)
+    assert alone[1][0].tobytes() == mix[1][0].tobytes()
+    assert alone[1][1].tobytes() == mix[1][1].tobytes()
+    assert len(recs) == 7 and all(r["t_done"] >= r["t_admit"] >= r["t_submit"] for r in recs)
+    # and the plain single-system path agrees
+    with astro.Context(3000, bits_per_axis=21) as ctx:
+        ctx.set_params(P)
+        ctx.upload(*a)
+        ctx.step(1e-3, 6)
+        assert ctx.download()[0].tobytes() == alone[1][0].tobytes()
+
+
+def test_server_admission_respects_budget(astro):
+    say("server budget")
+    P = astro.params_physics(21)
+    srv = astro.Server(P, 50000, 8)
+    total, per = srv.device_bytes()
+    srv.close()
+    budget = per * 6000  # room for one 5000-particle request at a time
+    r1, r2 = astro.ic_plummer(5000, 1), astro.ic_plummer(5000, 2)
+    _, recs = run_server(astro, P, [(1, 0, *r1, 3), (2, 0, *r2, 3)], 50000, 8, budget)
+    d = {r["id"]: r for r in recs}
+    assert d[2]["t_admit"] >= d[1]["t_done"]  # the second waited for memory
+    srv = astro.Server(P, 50000, 8, per * 1000)
+    with pytest.raises(astro.AstroError):
+        srv.submit(3, *astro.ic_plummer(5000, 3), 1)  # can never fit
+    srv.close()
+
+
+def test_diagnostics_fp64(astro, orc):
+    say("diagnostics")
+    pos, vel, col = astro.ic_plummer(8192, 9)
+    with astro.Context(8192, bits_per_axis=21) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload(pos, vel, col)
+        d = ctx.diagnostics()[0]
+        ctx.check()
+    e = orc.energy_momentum(pos, vel)
+    assert abs(d[0] - pos["mass"].astype(np.float64).sum()) < 1e-9
+    assert abs(d[1] - e[0]) < 1e-9 * abs(e[0]) + 1e-12
+    assert np.allclose(d[3:6], e[1:4], atol=1e-9)
+    assert np.allclose(d[6:9] / d[0], e[4:7], atol=1e-9)
+    # potential energy against the FP64 direct sum on all pairs (softened)
+    x = pos["position"].astype(np.float64)
+    m = pos["mass"].astype(np.float64)
+    W = 0.0
+    for i0 in range(0, 8192, 1024):
+        dx = x[i0:i0 + 1024, None, :] - x[None, :, :]
+        r = np.sqrt((dx ** 2).sum(-1) + float(np.float32(0.01)))
+        w = m[i0:i0 + 1024, None] * m[None, :] / r
+        idx = np.arange(i0, min(i0 + 1024, 8192))
+        w[idx - i0, idx] = 0.0
+        W -= 0.5 * w.sum()
+    assert abs(d[2] - W) < 5e-3 * abs(W)
+
+
+def test_autotune_keeps_results(astro):
+    say("autotune")
+    pos, vel, col = astro.ic_plummer(30000, 4)
+    with astro.Context(30000, bits_per_axis=21) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload(pos, vel, col)
+        ctx.build_tree()
+        ctx.compute_forces()
+        f0 = ctx.forces()
+        b, ms = ctx.autotune(3)
+        assert b in (64, 128, 256, 512) and ms > 0
+        for blk in (64, 128, 512):
+            ctx.set_force_block(blk)
+            ctx.compute_forces()
+            assert ctx.forces().tobytes() == f0.tobytes()
+        ctx.check()
+
+
+def test_energy_drift_small(astro):
+    say("energy drift")
+    pos, vel, col = astro.ic_plummer(16384, 21)
+    with astro.Context(16384, bits_per_axis=21) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload(pos, vel, col)
+        d0 = ctx.diagnostics()[0]
+        ctx.step(1e-3, 20)
+        d1 = ctx.diagnostics()[0]
+    E0, E1 = d0[1] + d0[2], d1[1] + d1[2]
+    assert abs(E1 - E0) < 1e-3 * abs(E0)
+    assert np.all(np.abs(d1[3:6] - d0[3:6]) < 1e-4)

@@ -1,0 +1,295 @@
+"""GPU parity: libastro's HIP path against the CPU oracle on the same seeded inputs.
+
+Integer work (keys, permutation, topology) must be bit-exact; forces and positions must be
+within 1e-5 relative (norm-wise, per particle) and, because kernel and oracle perform the same
+IEEE operations in the same order, are also checked for bit equality.  Each test prints its
+case before touching the GPU."""
+import sys
+
+import numpy as np
+import pytest
+
+from astro_amd.structs import COLOR_TYPE, INVALID_MORTON_CODE, OCTREE_NODE, POSITION_MASS
+
+pytestmark = pytest.mark.gpu
+FORCE_RTOL = 1e-5
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)[:, :3]
+    b = np.asarray(b, np.float64)[:, :3]
+    den = np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+    return np.linalg.norm(a - b, axis=1) / den
+
+
+@pytest.fixture(scope="module")
+def small_ctx(astro):
+    c = astro.Context(70000, bits_per_axis=21, max_systems=1, emit_octree_nodes=True)
+    yield c
+    c.close()
+
+
+def plummer(astro, n, seed):
+    return astro.ic_plummer(n, seed)
+
+
+CASES = [
+    ("compat_205_of_2000", "compat", 2000, 205),
+    ("physics_B7_16384", 7, 16384, None),
+    ("physics_B21_16384", 21, 16384, None),
+    ("physics_B21_3001", 21, 3001, None),
+    ("physics_B6_4000", 6, 4000, None),
+]
+
+
+def setup_case(astro, orc, ctx, case):
+    name, prof, n, nt = case
+    if prof == "compat":
+        rng = np.random.default_rng(5)
+        pos = np.zeros(n, POSITION_MASS)
+        pos["position"] = rng.uniform(-2e6, 2e6, (n, 3)).astype(np.float32)
+        pos["mass"] = np.where(np.arange(n) < 5, 1e4, np.where(np.arange(n) < nt, 1e3, 1.0))
+        col = np.zeros(n, COLOR_TYPE)
+        col["type"] = np.where(np.arange(n) < 5, 0, np.where(np.arange(n) < nt, 1, 2))
+        vel = np.zeros(n, astro.VELOCITY_RADIUS)
+        vel["velocity"] = rng.uniform(-2e4, 2e4, (n, 3)).astype(np.float32)
+        P, Po = astro.params_compat(), orc.params_compat()
+    else:
+        pos, vel, col = plummer(astro, n, 42)
+        P, Po = astro.params_physics(prof), orc.params_physics(prof)
+    ctx.set_params(P)
+    ctx.upload(pos, vel, col, n_tree=nt or 0)
+    return pos, vel, col, Po, (nt or n)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fused_build_bitexact(astro, orc, small_ctx, case):
+    say("fused build", case[0])
+    ctx = small_ctx
+    pos, vel, col, Po, nt = setup_case(astro, orc, ctx, case)
+    ctx.set_emit_octree_nodes(True)
+    ctx.build_tree()
+    ctx.check()
+    t = orc.build_tree(pos, Po, n_tree=nt)
+    assert np.array_equal(ctx.read("MORTON_CODES", np.uint64, nt), t["codes"])
+    assert np.array_equal(ctx.read("SORTED_CODES", np.uint64, nt), t["sorted_codes"])
+    assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, nt), t["sorted_idx"])
+    if Po.quantizer == 1:
+        assert np.array_equal(ctx.read("CUBES", np.float32, 4), t["cube"])
+    assert list(ctx.layer_counts()) == t["count"]
+    got = ctx.octree_nodes()
+    assert len(got) == len(t["nodes"])
+    bad = np.nonzero(got.view(np.uint8).reshape(-1, 128) != t["nodes"].view(np.uint8).reshape(-1, 128))[0]
+    assert len(bad) == 0, f"{len(np.unique(bad))} nodes differ, first {np.unique(bad)[:5]}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_forces_bitexact(astro, orc, small_ctx, case):
+    say("forces", case[0])
+    ctx = small_ctx
+    pos, vel, col, Po, nt = setup_case(astro, orc, ctx, case)
+    ctx.build_tree()
+    ctx.compute_forces()
+    ctx.check()
+    t = orc.build_tree(pos, Po, n_tree=nt)
+    want, _ = orc.barnes_hut(t, pos, col, Po, n_tree=nt, threads=8)
+    got = ctx.forces()
+    e = rel_err(got, want)
+    assert e.max() <= FORCE_RTOL, f"max rel err {e.max():.3e}"
+    assert np.array_equal(got[:, :3], want[:, :3]), f"not bit-exact, max rel {e.max():.3e}"
+
+
+@pytest.mark.parametrize("kahan", [0, 1])
+def test_forces_kahan(astro, orc, small_ctx, kahan):
+    say("forces kahan", kahan)
+    ctx = small_ctx
+    pos, vel, col = plummer(astro, 16384, 7)
+    P, Po = astro.params_physics(21), orc.params_physics(21)
+    P.kahan = Po.kahan = kahan
+    ctx.set_params(P)
+    ctx.upload(pos, vel, col)
+    ctx.build_tree()
+    ctx.compute_forces()
+    t = orc.build_tree(pos, Po)
+    want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
+    assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
+
+
+def run_stage_sequence(ctx, nt, layers_cap_off):
+    """Renderer.swift:511-534 through the stage calls."""
+    cap, off = layers_cap_off
+    total = int(off[-1] + cap[-1])
+    ctx.clear_buffer("SORTED_CODES", nt, 8)
+    ctx.clear_buffer("SORTED_INDICES", nt, 4)
+    ctx.clear_octree_node_buffer(total)
+    ctx.generate_morton_codes(nt)
+    ctx.fill_indices("UNSORTED_INDICES", 0, nt)
+    ctx.write("PARENT_COUNT", np.array([nt], np.uint32))
+    ctx.radix_sort("UNSORTED_CODES", "UNSORTED_INDICES", "SORTED_CODES", "SORTED_INDICES", nt,
+                   "PARENT_COUNT")
+    ctx.count_unique_morton_codes(False, nt)
+    ctx.aggregate_leaf_nodes(nt)
+    ctx.build_internal_nodes(nt)
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_stage_path_matches_oracle(astro, orc, small_ctx, case):
+    say("stage path", case[0])
+    ctx = small_ctx
+    pos, vel, col, Po, nt = setup_case(astro, orc, ctx, case)
+    run_stage_sequence(ctx, nt, ctx.layer_table())
+    ctx.barnes_hut()
+    ctx.check()
+    t = orc.build_tree(pos, Po, n_tree=nt)
+    got = ctx.octree_nodes()
+    assert got.tobytes() == t["nodes"].tobytes()
+    want, _ = orc.barnes_hut(t, pos, col, Po, n_tree=nt, threads=8)
+    assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
+    # and the integrator stage (BarnesHut.metal:169-196)
+    ctx.update_spheres(np.float32(1 / 30))
+    p2, v2, _ = ctx.download()
+    wp, wv = orc.update_spheres(pos, vel, want, col, np.float32(1 / 30), Po)
+    assert np.array_equal(p2["position"], wp["position"])
+    assert np.array_equal(v2["velocity"], wv["velocity"])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ten_steps_bitexact(astro, orc, small_ctx, graph):
+    # 16,384 Plummer, B=7 (21-bit keys), theta 0.7, eps^2 0.01, dt 1e-3, 10 steps
+    say("10 steps, graph", graph)
+    ctx = small_ctx
+    pos, vel, col = plummer(astro, 16384, 42)
+    P, Po = astro.params_physics(7), orc.params_physics(7)
+    ctx.set_params(P)
+    ctx.set_use_graph(graph)
+    ctx.upload(pos, vel, col)
+    ctx.step(1e-3, 10)
+    ctx.check()
+    p2, v2, _ = ctx.download()
+    p, v = pos, vel
+    for _ in range(10):
+        p, v, f, _ = orc.step(p, v, col, Po, 1e-3, threads=8)
+    ctx.set_use_graph(True)
+    assert np.array_equal(p2["position"], p["position"])
+    assert np.array_equal(v2["velocity"], v["velocity"])
+
+
+def test_graph_replays_match_eager(astro, small_ctx):
+    say("graph replays vs eager")
+    ctx = small_ctx
+    pos, vel, col = plummer(astro, 20000, 3)
+    ctx.set_params(astro.params_physics(21))
+    outs = []
+    for graph in (False, True):
+        ctx.set_use_graph(graph)
+        ctx.upload(pos, vel, col)
+        for _ in range(4):  # replays with host synchronisation between them
+            ctx.step(1e-3, 1)
+            ctx.sync()
+        outs.append(ctx.download()[0].tobytes())
+    ctx.set_use_graph(True)
+    assert outs[0] == outs[1]
+
+
+@pytest.mark.parametrize("n", [1, 5, 2047, 2048, 2049, 300000])
+def test_scan(astro, small_ctx, n):
+    say("scan", n)
+    rng = np.random.default_rng(n)
+    ctx = astro.Context(max(n, 16), bits_per_axis=7) if n > 70000 else small_ctx
+    a = rng.integers(0, 5, n).astype(np.uint32)
+    ctx.write("FLAGS", a)
+    ctx.scan("FLAGS", "SCAN", n)
+    ctx.check()
+    got = ctx.read("SCAN", np.uint32, n)
+    assert np.array_equal(got, np.concatenate([[0], np.cumsum(a)[:-1]]).astype(np.uint32))
+
+
+@pytest.mark.parametrize("bit", [0, 5, 63])
+def test_split_pass(astro, orc, small_ctx, bit):
+    say("split pass", bit)
+    ctx = small_ctx
+    rng = np.random.default_rng(bit)
+    n, count = 5000, 4321
+    k = rng.integers(0, 1 << 63, n, dtype=np.uint64) | (rng.integers(0, 2, n).astype(np.uint64) << np.uint64(63))
+    idx = np.arange(n, dtype=np.uint32)
+    ctx.write("UNSORTED_CODES", k)
+    ctx.write("UNSORTED_INDICES", idx)
+    ctx.write("PARENT_COUNT", np.array([count], np.uint32))
+    ctx.split_pass("UNSORTED_CODES", "UNSORTED_INDICES", "SORTED_CODES", "SORTED_INDICES", bit, n,
+                   "PARENT_COUNT")
+    ctx.check()
+    ko, io = orc.split_pass(k[:count], idx[:count], bit)
+    assert np.array_equal(ctx.read("SORTED_CODES", np.uint64, count), ko)
+    assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), io)
+
+
+@pytest.mark.parametrize("kind", ["random64", "invalid_tail", "equal", "few", "small"])
+def test_radix_sort_generic(astro, small_ctx, kind):
+    say("radix sort", kind)
+    ctx = small_ctx
+    rng = np.random.default_rng(11)
+    n = 60000 if kind != "small" else 37
+    count = n - 123 if kind != "small" else n
+    if kind == "random64":
+        k = rng.integers(0, 1 << 63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n).astype(np.uint64)
+    elif kind == "invalid_tail":
+        k = rng.integers(0, 1 << 40, n, dtype=np.uint64)
+        k[rng.integers(0, n, 3000)] = INVALID_MORTON_CODE
+    elif kind == "equal":
+        k = np.full(n, 77, np.uint64)
+    elif kind == "few":
+        k = rng.integers(0, 3, n).astype(np.uint64) << np.uint64(61)
+    else:
+        k = rng.integers(0, 10, n).astype(np.uint64)
+    idx = rng.permutation(n).astype(np.uint32)
+    ctx.write("UNSORTED_CODES", k)
+    ctx.write("UNSORTED_INDICES", idx)
+    ctx.write("PARENT_COUNT", np.array([count], np.uint32))
+    ctx.radix_sort("UNSORTED_CODES", "UNSORTED_INDICES", "SORTED_CODES", "SORTED_INDICES", n,
+                   "PARENT_COUNT")
+    ctx.check()
+    o = np.argsort(k[:count], kind="stable")
+    assert np.array_equal(ctx.read("SORTED_CODES", np.uint64, count), k[:count][o])
+    assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), idx[:count][o])
+
+
+def test_two_disks_500k(astro, orc):
+    # the 500,000-particle two-disk system at 63-bit keys: whole tree bit-exact, forces on a
+    # 4,096-target subset bit-exact, sortedness and permutation at full size
+    say("two disks 500k")
+    pos, vel, col = astro.ic_two_disks(250000, 2024)
+    n = len(pos)
+    with astro.Context(n, bits_per_axis=21, emit_octree_nodes=True) as ctx:
+        P, Po = astro.params_physics(21), orc.params_physics(21)
+        ctx.set_params(P)
+        ctx.upload(pos, vel, col)
+        ctx.build_tree()
+        ctx.compute_forces()
+        ctx.check()
+        keys = ctx.read("SORTED_CODES", np.uint64, n)
+        perm = ctx.read("SORTED_INDICES", np.uint32, n)
+        assert np.all(keys[1:] >= keys[:-1])
+        assert np.array_equal(np.sort(perm), np.arange(n, dtype=np.uint32))
+        t = orc.build_tree(pos, Po)
+        assert np.array_equal(perm, t["sorted_idx"])
+        assert list(ctx.layer_counts()) == t["count"]
+        assert ctx.octree_nodes().tobytes() == t["nodes"].tobytes()
+        f = ctx.forces()
+        tg = np.random.default_rng(1).choice(n, 4096, replace=False)
+        tg.sort()
+        ok = 0
+        for a in range(0, 4096, 512):  # oracle on contiguous runs of targets
+            for i in tg[a:a + 512]:
+                w, _ = orc.barnes_hut(t, pos, col, Po, targets=(int(i), int(i) + 1))
+                ok += np.array_equal(f[i, :3], w[i, :3])
+        assert ok == 4096
+
+
+def test_astro_check_reports_clean(astro, small_ctx):
+    say("error word clean")
+    small_ctx.check()
