@@ -41,14 +41,12 @@ static void compute_layers(const std::vector<uint32_t>& sizes, uint32_t bits, ui
     *total = o;
 }
 
-static uint64_t scratch_need(uint32_t nmax, uint32_t bits, const uint64_t* cap, uint32_t layers) {
-    uint64_t w = (uint64_t)kMaxRadixPasses * 256;                              // histograms
-    w += (uint64_t)kMaxRadixPasses * ((uint64_t)sort_tiles(nmax) * 256 + 1);   // pass status
-    w += scan_tiles(nmax) + 1;                                                 // leaf layer
-    for (uint32_t l = 1; l < layers; l++) w += scan_tiles((uint32_t)cap[l - 1]) + 1;
-    w += scan_tiles(nmax) + 1;                                                 // astro_scan
-    (void)bits;
-    return w;
+static uint64_t scratch_need(uint32_t nmax, const uint64_t* cap, uint32_t layers) {
+    uint64_t sort = radix_work_words(nmax);                          // one digit pass
+    uint64_t lay = scan_tiles(nmax);                                 // leaf layer
+    for (uint32_t l = 1; l < layers; l++) lay += scan_tiles((uint32_t)cap[l - 1]);
+    uint64_t scan = scan_tiles(nmax) + 1;                            // astro_scan
+    return std::max(std::max(sort, lay), scan);
 }
 
 static void invalidate_graph(astro_ctx* c) {
@@ -221,7 +219,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         astro_destroy(c);
         return ASTRO_ERR_NOMEM;
     }
-    c->scratch_words = scratch_need(NP, cfg->bits_per_axis, cap_max, cfg->bits_per_axis + 1);
+    c->scratch_words = scratch_need(NP, cap_max, cfg->bits_per_axis + 1);
     if (bad(dalloc(c->device_bytes, &c->scratch, c->scratch_words))) return ASTRO_ERR_NOMEM;
     hipStream_t s = c->stream;
     launch_fill_u32(c->err_word, 0, 1, s);
@@ -509,21 +507,16 @@ static int radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_t* iin, ui
                       const RadixPlan& plan, const uint32_t* seg_of) {
     hipStream_t s = c->stream;
     if (cap == 0 || plan.npasses == 0) return ASTRO_OK;
-    const uint64_t per_pass = (uint64_t)sort_tiles(cap) * 256 + 1;
-    const uint64_t need = (uint64_t)plan.npasses * 256 + plan.npasses * per_pass;
-    if (need > c->scratch_words) return fail(c, ASTRO_ERR_CAPACITY, "sort scratch too small");
-    uint32_t* hist = c->scratch;
-    uint32_t* status = c->scratch + (uint64_t)plan.npasses * 256;
-    launch_fill_u32(c->scratch, 0, need, s);
-    launch_radix_hist(kin, d_count, cap, seg_of, plan, hist, s);
+    if (radix_work_words(cap) > c->scratch_words)
+        return fail(c, ASTRO_ERR_CAPACITY, "sort scratch too small");
     const uint64_t* ksrc = kin;
     const uint32_t* isrc = iin;
     for (uint32_t p = 0; p < plan.npasses; p++) {
         const bool last = p + 1 == plan.npasses;
         uint64_t* kdst = last ? kout : (p % 2 == 0 ? c->tmp_keys_a : c->tmp_keys_b);
         uint32_t* idst = last ? iout : (p % 2 == 0 ? c->tmp_idx_a : c->tmp_idx_b);
-        launch_radix_pass(ksrc, isrc, kdst, idst, d_count, cap, hist + p * 256, plan.pass[p],
-                          seg_of, status + p * per_pass, c->err_word, s);
+        launch_radix_pass(ksrc, isrc, kdst, idst, d_count, cap, plan.pass[p], seg_of,
+                          c->scratch, s);
         ksrc = kdst;
         isrc = idst;
     }
@@ -562,20 +555,12 @@ extern "C" int astro_radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_
     return radix_sort(c, kin, iin, kout, iout, estimated_len, d_count, plan, nullptr);
 }
 
-static uint32_t* scan_scratch(astro_ctx* c, uint32_t len) {
-    const uint64_t need = scan_tiles(len) + 1;
-    uint32_t* p = c->scratch + (c->scratch_words - need);
-    launch_fill_u32(p, 0, need, c->stream);
-    return p;
-}
-
 extern "C" int astro_scan(astro_ctx* c, const uint32_t* in, uint32_t* out, uint32_t len) {
     if (!c || (!in && len) || (!out && len)) return ASTRO_ERR_ARG;
     if (len == 0) return ASTRO_OK;
-    if (scan_tiles(len) + 1 > c->scratch_words)
+    if (scan_tiles(len) > c->scratch_words)
         return fail(c, ASTRO_ERR_CAPACITY, "scan longer than the context supports");
-    uint32_t* st = scan_scratch(c, len);
-    launch_scan(in, out, len, st, c->err_word, c->stream);
+    launch_scan(in, out, len, c->scratch, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -744,6 +729,7 @@ static ForceArgs force_args(astro_ctx* c) {
     a.ns = c->ns;
     a.nt = c->nt;
     a.p = c->params;
+    set_accept_threshold(a);  // needs a.p
     return a;
 }
 
@@ -806,26 +792,21 @@ static int enqueue_sort(astro_ctx* c) {
 static int enqueue_layers(astro_ctx* c) {
     hipStream_t s = c->stream;
     TreeArgs a = tree_args(c);
-    // scratch after the sort's region: one status block per layer
-    const RadixPlan plan = key_plan(3 * c->params.bits_per_axis, c->nsys);
-    uint64_t base = (uint64_t)plan.npasses * 256 +
-                    (uint64_t)plan.npasses * ((uint64_t)sort_tiles(c->nt) * 256 + 1);
-    uint64_t need = scan_tiles(c->nt) + 1;
-    for (uint32_t l = 1; l < c->layers; l++) need += scan_tiles((uint32_t)c->cap[l - 1]) + 1;
-    if (base + need > c->scratch_words)
-        return fail(c, ASTRO_ERR_CAPACITY, "layer scratch too small");
-    uint32_t* st = c->scratch + base;
-    launch_fill_u32(st, 0, need, s);
+    // scratch: one tile-count block per layer (all layers' blocks kept apart)
+    uint64_t need = scan_tiles(c->nt);
+    for (uint32_t l = 1; l < c->layers; l++) need += scan_tiles((uint32_t)c->cap[l - 1]);
+    if (need > c->scratch_words) return fail(c, ASTRO_ERR_CAPACITY, "layer scratch too small");
     if (a.onodes && c->onodes_dirty) {
         launch_clear_onodes(c->onodes, c->node_capacity, s);
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
         c->onodes_dirty = false;
     }
-    launch_leaf_layer(a, st, s);
-    st += scan_tiles(c->nt) + 1;
+    uint32_t* st = c->scratch;
+    launch_layer(a, 0, st, s);
+    st += scan_tiles(c->nt);
     for (uint32_t l = 1; l < c->layers; l++) {
-        launch_internal_layer(a, l, st, s);
-        st += scan_tiles((uint32_t)c->cap[l - 1]) + 1;
+        launch_layer(a, l, st, s);
+        st += scan_tiles((uint32_t)c->cap[l - 1]);
     }
     if (a.onodes) {
         launch_clear_stale(a, c->layer_counts_prev, s);

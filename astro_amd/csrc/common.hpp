@@ -29,26 +29,13 @@ __host__ __device__ inline uint32_t tn_level(uint32_t info) { return (info >> 9)
 
 // Error word bits raised by kernels (read back by astro_check).
 enum : uint32_t {
-    ERR_SPIN = 1u,       // a look-back wait exceeded its bound
+    ERR_SPIN = 1u,       // reserved (no kernel waits on another block)
     ERR_STACK = 2u,      // traversal stack overflow
     ERR_CAPACITY = 4u,   // a layer had more nodes than its capacity
     ERR_NODE_INDEX = 8u, // a traversal met a node index outside the tree
 };
 
 constexpr uint32_t kMaxLayers = 24;   // B <= 21 -> 22 layers
-constexpr uint32_t kSpinLimit = 1u << 22;
-
-// Decoupled look-back status word: 2 flag bits over a 30-bit count.
-constexpr uint32_t kFlagAgg = 1u << 30;
-constexpr uint32_t kFlagInc = 2u << 30;
-constexpr uint32_t kValMask = (1u << 30) - 1u;
-
-__device__ inline uint32_t ld_acquire(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_release(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Popcount of the lanes below this one in a 64-lane mask.
 __device__ inline uint32_t lanes_below(uint64_t mask) {
@@ -90,40 +77,6 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint3
     *total = tmp[NW];
     __syncthreads();
     return r;
-}
-
-// Decoupled look-back for one value per tile (Merrill & Garland, "Single-pass Parallel Prefix
-// Scan with Decoupled Look-back", 2016).  Called by one thread.  Publishes the tile's
-// aggregate, walks back to the nearest inclusive prefix, publishes its own inclusive prefix
-// and returns the exclusive prefix of this tile.  Waits are bounded: on timeout the error
-// word is raised and the partial sum returned, so every wave still exits.
-__device__ inline uint32_t lookback_exclusive(uint32_t* status, uint32_t tile, uint32_t agg,
-                                              uint32_t* err) {
-    if (tile == 0) {
-        st_release(&status[0], kFlagInc | agg);
-        return 0;
-    }
-    st_release(&status[tile], kFlagAgg | agg);
-    uint32_t prefix = 0;
-    int64_t t = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    while (t >= 0) {
-        uint32_t s = ld_acquire(&status[t]);
-        uint32_t f = s & ~kValMask;
-        if (f == 0u) {
-            if (++spins > kSpinLimit) {
-                atomicOr(err, ERR_SPIN);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        prefix += s & kValMask;
-        if (f == kFlagInc) break;
-        t--;
-    }
-    st_release(&status[tile], kFlagInc | (prefix + agg));
-    return prefix;
 }
 
 // Float <-> order-preserving unsigned key, for atomic min/max of floats.

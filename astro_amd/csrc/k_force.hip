@@ -10,6 +10,9 @@
 //
 // k_force_reference: the reference kernel shape (BarnesHut.metal:22-101) over the 128-B
 // OctreeNode array with children[8], for callers that built the tree with the stage calls.
+#include <cmath>
+#include <cstring>
+
 #include "launch.hpp"
 
 namespace astro {
@@ -40,6 +43,37 @@ __device__ inline void accumulate(Acc3& F, float s, float dx, float dy, float dz
         F.y = __builtin_fmaf(s, dy, F.y);
         F.z = __builtin_fmaf(s, dz, F.z);
     }
+}
+
+// The opening test of the reference, RN(side / r) < theta (BarnesHut.metal:75), without the
+// division.  RN is monotone, so RN(x) < theta exactly when x lies below the midpoint m between
+// theta and its float predecessor a, or on m when the tie rounds to a (a's last mantissa bit
+// even).  m has at most 26 significant bits and r 24, so thr*r is exact in double and the
+// comparison with the power-of-two side is exact: the decision equals the divided one for
+// every input, which keeps the traversal bit-identical to the oracle.
+void set_accept_threshold(ForceArgs& a) {
+    const float th = a.p.theta;
+    if (!(th > 0.0f)) {  // nothing is ever accepted
+        a.acc_thr = 0.0;
+        a.acc_tie = 0;
+        return;
+    }
+    if (std::isinf(th)) {  // everything finite is accepted
+        a.acc_thr = INFINITY;
+        a.acc_tie = 0;
+        return;
+    }
+    const float pred = std::nextafter(th, 0.0f);
+    uint32_t bits;
+    std::memcpy(&bits, &pred, sizeof(bits));
+    a.acc_thr = ((double)pred + (double)th) * 0.5;
+    a.acc_tie = (bits & 1u) == 0u ? 1u : 0u;
+}
+
+__device__ inline bool accept_open_angle(float side, float r, double thr, uint32_t tie) {
+    const double P = thr * (double)r;
+    const double S = (double)side;
+    return S < P || (tie && S == P);
 }
 
 __device__ inline float clampf(float v, float c) {
@@ -77,7 +111,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
         if (a.p.side_rule == ASTRO_SIDE_SHIFTED) side0 = 2097152.0f;  // 2^21
         else side0 = ldexpf(a.cubes[sys].w, -(int)a.p.bits_per_axis);
         const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
-        const float cut = a.p.leaf_cutoff_r2, theta = a.p.theta, G = a.p.G;
+        const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
 
         Acc3 F;
         double phi = 0.0;
@@ -115,7 +149,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
                 if (MODE == 1) leafs += interact;
             } else {
                 const float side = ldexpf(side0, (int)layer);
-                interact = (side / rr) < theta;
+                interact = accept_open_angle(side, rr, a.acc_thr, a.acc_tie);
                 open = !interact;
                 if (MODE == 1) {
                     monos += interact;

@@ -2,11 +2,11 @@
 //
 // The reference sorts with 64 one-bit split passes, each a prep + 1-3 scan dispatches + a
 // scatter (RadixSort.swift:98-185, RadixSort.metal:30-131): ~3.8 KB of traffic per key.  Here
-// a sort is one histogram pass over the keys followed by one pass per 8-bit digit.  A digit
-// pass ranks keys inside a wavefront with ballots (no atomics), turns per-wave counts into
-// block offsets in LDS, gets the tile's global offset per digit by decoupled look-back, stages
-// the tile in LDS in digit order and writes it out coalesced.  Every pass is stable, so the
-// result is the stable ascending argsort -- the same permutation the split sort produces.
+// a sort is one pass per 8-bit digit, each three launches: per-tile digit counts, a per-digit
+// scan over the tiles, and a rank-and-scatter kernel that ranks keys inside a wavefront with
+// ballots (no atomics), turns per-wave counts into block offsets in LDS, stages the tile in LDS
+// in digit order and writes it out coalesced.  No block ever waits on another.  Every pass is
+// stable, so the result is the stable ascending argsort -- the split sort's permutation.
 #include "launch.hpp"
 
 namespace astro {
@@ -77,44 +77,51 @@ void launch_copy_u32(const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t 
 }
 
 // ------------------------------------------------------------------------- exclusive scan
-// One pass: tile = 256 threads x 8 consecutive values; block scan in LDS; tile prefix by
-// decoupled look-back; tile ids handed out in launch order by an atomic counter so that a
-// tile only ever waits on tiles that are already running.
-__global__ __launch_bounds__(256) void k_scan(const uint32_t* __restrict__ in,
-                                              uint32_t* __restrict__ out, uint32_t len,
-                                              uint32_t* status, uint32_t* counter,
-                                              uint32_t* err) {
-    __shared__ uint32_t s_tile, s_prefix, s_tmp[8];
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+// Two launches over tiles of 256 threads x 8 values: per-tile sums, then each tile adds up the
+// sums before it (no block waits on another), scans itself in LDS and writes.
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ in, uint32_t len,
+                                                   uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_tmp[8];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * 8u;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+        if (base + i < len) sum += in[base + i];
+    uint32_t total;
+    block_exclusive_scan<256>(sum, s_tmp, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_emit(const uint32_t* __restrict__ in,
+                                                   uint32_t* __restrict__ out, uint32_t len,
+                                                   const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_tmp[8];
+    const uint32_t tile = blockIdx.x;
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < tile; t += 256) part += sums[t];
+    uint32_t prefix;
+    block_exclusive_scan<256>(part, s_tmp, &prefix);
     const uint64_t base = (uint64_t)tile * kScanTile + threadIdx.x * 8u;
     uint32_t v[8], sum = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        uint64_t e = base + i;
-        v[i] = e < len ? in[e] : 0u;
+        v[i] = base + i < len ? in[base + i] : 0u;
         sum += v[i];
     }
     uint32_t total;
-    uint32_t excl = block_exclusive_scan<256>(sum, s_tmp, &total);
-    if (threadIdx.x == 0) s_prefix = lookback_exclusive(status, tile, total, err);
-    __syncthreads();
-    uint32_t run = s_prefix + excl;
+    uint32_t run = prefix + block_exclusive_scan<256>(sum, s_tmp, &total);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        uint64_t e = base + i;
-        if (e < len) out[e] = run;
+        if (base + i < len) out[base + i] = run;
         run += v[i];
     }
 }
 
-void launch_scan(const uint32_t* in, uint32_t* out, uint32_t len, uint32_t* status,
-                 uint32_t* err, hipStream_t s) {
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t len, uint32_t* sums, hipStream_t s) {
     if (!len) return;
-    uint32_t tiles = scan_tiles(len);
-    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(256), 0, s, in, out, len, status,
-                       status + tiles, err);
+    const uint32_t tiles = scan_tiles(len);
+    hipLaunchKernelGGL(k_scan_sums, dim3(tiles), dim3(256), 0, s, in, len, sums);
+    hipLaunchKernelGGL(k_scan_emit, dim3(tiles), dim3(256), 0, s, in, out, len, sums);
 }
 
 // ------------------------------------------------------------------------ 1-bit split pass
@@ -159,46 +166,69 @@ __device__ inline uint32_t radix_digit(uint64_t key, uint32_t idx, RadixPassDesc
     return (uint32_t)(key >> d.shift) & 0xFFu;
 }
 
-// Histograms of every digit pass in one read of the keys.
-__global__ __launch_bounds__(256) void k_radix_hist(const uint64_t* __restrict__ keys,
-                                                    const uint32_t* d_count,
-                                                    const uint32_t* seg_of, RadixPlan plan,
-                                                    uint32_t* hist) {
-    __shared__ uint32_t s_h[kMaxRadixPasses][256];
-    for (uint32_t k = threadIdx.x; k < kMaxRadixPasses * 256; k += 256) (&s_h[0][0])[k] = 0;
+// Per-tile digit counts of one pass: counts[d * tiles + tile].  Tiles past the live count
+// write zeros, so the digit scan can run over the whole (host-sized) grid.
+__global__ __launch_bounds__(256) void k_radix_count(const uint64_t* __restrict__ kin,
+                                                     const uint32_t* __restrict__ iin,
+                                                     const uint32_t* d_count, RadixPassDesc desc,
+                                                     const uint32_t* __restrict__ seg_of,
+                                                     uint32_t* __restrict__ counts,
+                                                     uint32_t tiles) {
+    __shared__ uint32_t s_h[256];
+    const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+    s_h[tid] = 0;
     __syncthreads();
-    const uint32_t n = *d_count;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        uint64_t key = keys[i];
-        for (uint32_t p = 0; p < plan.npasses; p++) {
-            // the segment digit multiset does not depend on the order, so position i serves
-            uint32_t d = radix_digit(key, i, plan.pass[p], seg_of);
-            atomicAdd(&s_h[p][d], 1u);
+    const uint32_t count = *d_count;
+    const uint64_t base = (uint64_t)tile * kSortTile;
+    for (uint32_t k = tid; k < kSortTile; k += kSortThreads) {
+        const uint64_t e = base + k;
+        if (e < count) {
+            const uint32_t id = desc.segment ? iin[e] : 0u;
+            atomicAdd(&s_h[radix_digit(desc.segment ? 0ull : kin[e], id, desc, seg_of)], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t p = 0; p < plan.npasses; p++) {
-        uint32_t c = s_h[p][threadIdx.x];
-        if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
+    counts[(uint64_t)tid * tiles + tile] = s_h[tid];
+}
+
+// Exclusive scan of each digit's row of tile counts (one block per digit) and the digit
+// totals.
+__global__ __launch_bounds__(256) void k_radix_digit_scan(const uint32_t* __restrict__ counts,
+                                                          uint32_t tiles,
+                                                          uint32_t* __restrict__ excl,
+                                                          uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_tmp[8];
+    const uint32_t d = blockIdx.x;
+    const uint32_t* row = counts + (uint64_t)d * tiles;
+    uint32_t* out = excl + (uint64_t)d * tiles;
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < tiles; t0 += 256 * 4) {
+        uint32_t v[4], sum = 0;
+        const uint32_t tb = t0 + threadIdx.x * 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = tb + i < tiles ? row[tb + i] : 0u;
+            sum += v[i];
+        }
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan<256>(sum, s_tmp, &tot);
+        uint32_t run = carry + ex;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (tb + i < tiles) out[tb + i] = run;
+            run += v[i];
+        }
+        carry += tot;
     }
+    if (threadIdx.x == 0) totals[d] = carry;
 }
 
-void launch_radix_hist(const uint64_t* keys, const uint32_t* d_count, uint32_t cap,
-                       const uint32_t* seg_of, const RadixPlan& plan, uint32_t* hist,
-                       hipStream_t s) {
-    if (!cap || !plan.npasses) return;
-    uint32_t blocks = (cap + 256 * 16 - 1) / (256 * 16);
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_radix_hist, dim3(blocks), dim3(256), 0, s, keys, d_count, seg_of, plan,
-                       hist);
-}
-
-// One stable 8-bit digit pass.
-__global__ __launch_bounds__(256) void k_radix_pass(
+// One stable 8-bit digit pass: rank inside the tile, then scatter through LDS.
+__global__ __launch_bounds__(256) void k_radix_scatter(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ iin,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ iout, const uint32_t* d_count,
-    const uint32_t* __restrict__ hist, RadixPassDesc desc, const uint32_t* __restrict__ seg_of,
-    uint32_t* status, uint32_t* counter, uint32_t* err) {
+    RadixPassDesc desc, const uint32_t* __restrict__ seg_of, const uint32_t* __restrict__ excl,
+    const uint32_t* __restrict__ totals, uint32_t tiles) {
     constexpr int NW = kSortThreads / 64;
     __shared__ uint32_t s_whist[NW][256];   // per-wave digit counts, then wave offsets
     __shared__ uint32_t s_boff[256];        // block-local start of each digit
@@ -207,16 +237,14 @@ __global__ __launch_bounds__(256) void k_radix_pass(
     __shared__ uint32_t s_idx[kSortTile];
     __shared__ uint8_t s_dig[kSortTile];
     __shared__ uint32_t s_tmp[8];
-    __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s_tile = atomicAdd(counter, 1u);
-    for (uint32_t k = tid; k < NW * 256; k += kSortThreads) (&s_whist[0][0])[k] = 0;
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const uint32_t count = *d_count;
     const uint64_t base = (uint64_t)tile * kSortTile;
     if (base >= count) return;  // uniform over the block
+    for (uint32_t k = tid; k < NW * 256; k += kSortThreads) (&s_whist[0][0])[k] = 0;
+    __syncthreads();
 
     // Load: wave w owns elements [base + w*64*ITEMS, ...); item i of lane l is element
     // base + w*64*ITEMS + i*64 + l, so (wave, item, lane) order is input order.
@@ -225,7 +253,7 @@ __global__ __launch_bounds__(256) void k_radix_pass(
     const uint64_t wbase = base + (uint64_t)wave * 64 * kSortItems;
 #pragma unroll
     for (uint32_t i = 0; i < kSortItems; i++) {
-        uint64_t e = wbase + i * 64 + lane;
+        const uint64_t e = wbase + i * 64 + lane;
         if (e < count) {
             key[i] = kin[e];
             id[i] = iin[e];
@@ -265,41 +293,15 @@ __global__ __launch_bounds__(256) void k_radix_pass(
         uint32_t tot = 0;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
-            uint32_t c = s_whist[w][d];
+            const uint32_t c = s_whist[w][d];
             s_whist[w][d] = tot;
             tot += c;
         }
         uint32_t dummy;
         const uint32_t boff = block_exclusive_scan<kSortThreads>(tot, s_tmp, &dummy);
-        const uint32_t hex = block_exclusive_scan<kSortThreads>(hist[d], s_tmp, &dummy);
+        const uint32_t dbase = block_exclusive_scan<kSortThreads>(totals[d], s_tmp, &dummy);
         s_boff[d] = boff;
-        // look-back along this digit's column of the status table
-        uint32_t* col = status + d;
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            st_release(&col[0], kFlagInc | tot);
-        } else {
-            st_release(&col[(uint64_t)tile * 256], kFlagAgg | tot);
-            int64_t t = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            while (t >= 0) {
-                uint32_t sv = ld_acquire(&col[(uint64_t)t * 256]);
-                uint32_t f = sv & ~kValMask;
-                if (f == 0u) {
-                    if (++spins > kSpinLimit) {
-                        atomicOr(err, ERR_SPIN);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                prefix += sv & kValMask;
-                if (f == kFlagInc) break;
-                t--;
-            }
-            st_release(&col[(uint64_t)tile * 256], kFlagInc | (prefix + tot));
-        }
-        s_goff[d] = hex + prefix;
+        s_goff[d] = dbase + excl[(uint64_t)d * tiles + tile];
     }
     __syncthreads();
 
@@ -326,12 +328,19 @@ __global__ __launch_bounds__(256) void k_radix_pass(
 
 void launch_radix_pass(const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
                        uint32_t* iout, const uint32_t* d_count, uint32_t cap,
-                       const uint32_t* hist, RadixPassDesc desc, const uint32_t* seg_of,
-                       uint32_t* status, uint32_t* err, hipStream_t s) {
+                       RadixPassDesc desc, const uint32_t* seg_of, uint32_t* work,
+                       hipStream_t s) {
     if (!cap) return;
-    uint32_t tiles = sort_tiles(cap);
-    hipLaunchKernelGGL(k_radix_pass, dim3(tiles), dim3(kSortThreads), 0, s, kin, iin, kout, iout,
-                       d_count, hist, desc, seg_of, status, status + (uint64_t)tiles * 256, err);
+    const uint32_t tiles = sort_tiles(cap);
+    uint32_t* counts = work;
+    uint32_t* excl = work + (uint64_t)tiles * 256;
+    uint32_t* totals = excl + (uint64_t)tiles * 256;
+    hipLaunchKernelGGL(k_radix_count, dim3(tiles), dim3(kSortThreads), 0, s, kin, iin, d_count,
+                       desc, seg_of, counts, tiles);
+    hipLaunchKernelGGL(k_radix_digit_scan, dim3(256), dim3(256), 0, s, counts, tiles, excl,
+                       totals);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, kin, iin, kout,
+                       iout, d_count, desc, seg_of, excl, totals, tiles);
 }
 
 }  // namespace astro

@@ -2,10 +2,10 @@
 //
 // The reference builds each layer with clears, an identity re-sort, mark/scan/scatter of
 // segment heads, a count copy and an aggregation dispatch (Renderer.swift:570-755).  Here a
-// layer is one kernel: head flags (a node starts a parent when its key differs from its
-// predecessor above bit 3l), a tile scan with decoupled look-back for the parent index, and
-// the head thread aggregating its (at most 8) contiguous children in the reference's serial
-// order.  Aggregation order and arithmetic match the oracle exactly.
+// layer is a pair of kernels: head flags (a node starts a parent when its key differs from its
+// predecessor above bit 3l) counted per tile, then a tile scan for the parent index and the
+// head thread aggregating its (at most 8) contiguous children in the reference's serial order.
+// Aggregation order and arithmetic match the oracle exactly.
 #include "launch.hpp"
 
 namespace astro {
@@ -277,52 +277,6 @@ __device__ void emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void k_leaf_layer(TreeArgs a, uint32_t* status,
-                                                    uint32_t* counter) {
-    __shared__ uint32_t s_tile, s_prefix, s_tmp[8];
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t base64 = (uint64_t)tile * kScanTile;
-    if (base64 >= a.nt) return;
-    const uint32_t j0 = (uint32_t)base64 + threadIdx.x * 8u;
-    uint32_t hmask = 0, cnt = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < 8; i++) {
-        uint32_t j = j0 + i;
-        if (j < a.nt && is_leaf_head(a, j)) {
-            hmask |= 1u << i;
-            cnt++;
-        }
-    }
-    uint32_t total;
-    uint32_t excl = block_exclusive_scan<256>(cnt, s_tmp, &total);
-    if (threadIdx.x == 0) {
-        uint32_t pre = lookback_exclusive(status, tile, total, a.err);
-        s_prefix = pre;
-        if (base64 + kScanTile >= a.nt) a.counts[0] = pre + total;  // last tile
-    }
-    __syncthreads();
-    uint32_t u = s_prefix + excl;  // leaf index of the next head
-    const bool fp64 = a.fp64_from == 0;
-    for (uint32_t i = 0; i < 8; i++) {
-        uint32_t j = j0 + i;
-        if (j >= a.nt) break;
-        if (hmask & (1u << i)) {
-            if (fp64) emit_leaf<Agg64>(a, j, u);
-            else emit_leaf<Agg32>(a, j, u);
-            u++;
-        }
-        a.leaf_of_sorted[j] = u - 1;
-    }
-}
-
-void launch_leaf_layer(const TreeArgs& a, uint32_t* status, hipStream_t s) {
-    if (!a.nt) return;
-    uint32_t tiles = scan_tiles(a.nt);
-    hipLaunchKernelGGL(k_leaf_layer, dim3(tiles), dim3(256), 0, s, a, status, status + tiles);
-}
-
 // -------------------------------------------------------------------------- internal layer
 // Octree.metal:282-334 + 120-202 fused for layer l: child c (of layer l-1) starts a parent
 // when its head level is >= l; the parent aggregates children c, c+1, ... up to the next
@@ -362,55 +316,98 @@ __device__ void emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint32_t 
         write_onode(&a.onodes[n], a.sorted_keys[head] >> (3 * l), cm, t.first, k, l);
 }
 
-__global__ __launch_bounds__(256) void k_internal_layer(TreeArgs a, uint32_t l,
-                                                        uint32_t* status, uint32_t* counter) {
-    __shared__ uint32_t s_tile, s_prefix, s_tmp[8];
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t nchild_layer = a.counts[l - 1];
-    const uint32_t tile = s_tile;
-    const uint64_t base64 = (uint64_t)tile * kScanTile;
-    if (base64 >= nchild_layer) return;
-    const TNode* ch = a.tnodes + a.off[l - 1];
-    const uint32_t c0 = (uint32_t)base64 + threadIdx.x * 8u;
+// ------------------------------------------------------------------- layer kernel pair
+// Layer l is built by two launches over tiles of 2048 candidates (sorted particles for the
+// leaf layer, nodes of layer l-1 otherwise): k_layer_count stores each tile's number of heads;
+// k_layer_emit adds up the counts of the tiles before its own (no block waits on another),
+// scans its tile in LDS and writes one node per head.
+__device__ inline uint32_t layer_candidates(const TreeArgs& a, uint32_t l) {
+    return l == 0 ? a.nt : a.counts[l - 1];
+}
+__device__ inline bool layer_head(const TreeArgs& a, uint32_t l, uint32_t c) {
+    if (l == 0) return is_leaf_head(a, c);
+    return tn_level(a.tnodes[a.off[l - 1] + c].info) >= l;
+}
+
+__global__ __launch_bounds__(256) void k_layer_count(TreeArgs a, uint32_t l,
+                                                     uint32_t* tile_cnt) {
+    __shared__ uint32_t s_tmp[8];
+    const uint32_t n = layer_candidates(a, l);
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    if (base >= n) return;
+    const uint32_t c0 = (uint32_t)base + threadIdx.x * 8u;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) {
+        const uint32_t c = c0 + i;
+        if (c < n && layer_head(a, l, c)) cnt++;
+    }
+    uint32_t total;
+    block_exclusive_scan<256>(cnt, s_tmp, &total);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_layer_emit(TreeArgs a, uint32_t l,
+                                                    const uint32_t* __restrict__ tile_cnt) {
+    __shared__ uint32_t s_tmp[8];
+    const uint32_t n = layer_candidates(a, l);
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = (uint64_t)tile * kScanTile;
+    if (base >= n) return;
+    // heads in the tiles before this one
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < tile; t += 256) part += tile_cnt[t];
+    uint32_t prefix;
+    block_exclusive_scan<256>(part, s_tmp, &prefix);
+    const uint32_t c0 = (uint32_t)base + threadIdx.x * 8u;
     uint32_t hmask = 0, cnt = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 8; i++) {
-        uint32_t c = c0 + i;
-        if (c < nchild_layer && tn_level(ch[c].info) >= l) {
+        const uint32_t c = c0 + i;
+        if (c < n && layer_head(a, l, c)) {
             hmask |= 1u << i;
             cnt++;
         }
     }
     uint32_t total;
-    uint32_t excl = block_exclusive_scan<256>(cnt, s_tmp, &total);
-    if (threadIdx.x == 0) {
-        uint32_t pre = lookback_exclusive(status, tile, total, a.err);
-        s_prefix = pre;
-        if (base64 + kScanTile >= nchild_layer) {
-            a.counts[l] = pre + total;
-            if (pre + total > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
-        }
+    const uint32_t excl = block_exclusive_scan<256>(cnt, s_tmp, &total);
+    if (threadIdx.x == 0 && base + kScanTile >= n) {
+        a.counts[l] = prefix + total;  // the last tile knows the layer's size
+        if (prefix + total > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
     }
-    __syncthreads();
-    uint32_t p = s_prefix + excl;
-    const bool fp64 = l >= a.fp64_from;
-    for (uint32_t i = 0; i < 8; i++) {
-        if (!(hmask & (1u << i))) continue;
-        if (p < a.cap[l]) {
-            if (fp64) emit_parent<Agg64>(a, l, c0 + i, p, nchild_layer);
-            else emit_parent<Agg32>(a, l, c0 + i, p, nchild_layer);
+    uint32_t p = prefix + excl;  // index of this thread's first head
+    if (l == 0) {
+        const bool fp64 = a.fp64_from == 0;
+        for (uint32_t i = 0; i < 8; i++) {
+            const uint32_t j = c0 + i;
+            if (j >= n) break;
+            if (hmask & (1u << i)) {
+                if (fp64) emit_leaf<Agg64>(a, j, p);
+                else emit_leaf<Agg32>(a, j, p);
+                p++;
+            }
+            a.leaf_of_sorted[j] = p - 1;
         }
-        p++;
+    } else {
+        const bool fp64 = l >= a.fp64_from;
+        for (uint32_t i = 0; i < 8; i++) {
+            if (!(hmask & (1u << i))) continue;
+            if (p < a.cap[l]) {
+                if (fp64) emit_parent<Agg64>(a, l, c0 + i, p, n);
+                else emit_parent<Agg32>(a, l, c0 + i, p, n);
+            }
+            p++;
+        }
     }
 }
 
-void launch_internal_layer(const TreeArgs& a, uint32_t layer, uint32_t* status, hipStream_t s) {
-    uint64_t child_cap = a.cap[layer - 1];
-    if (!child_cap) return;
-    uint32_t tiles = scan_tiles((uint32_t)child_cap);
-    hipLaunchKernelGGL(k_internal_layer, dim3(tiles), dim3(256), 0, s, a, layer, status,
-                       status + tiles);
+// tile_cnt: scan_tiles(cap of the candidates) words; no clearing needed.
+void launch_layer(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, hipStream_t s) {
+    const uint64_t cand_cap = layer == 0 ? a.nt : a.cap[layer - 1];
+    if (!cand_cap) return;
+    const uint32_t tiles = scan_tiles((uint32_t)cand_cap);
+    hipLaunchKernelGGL(k_layer_count, dim3(tiles), dim3(256), 0, s, a, layer, tile_cnt);
+    hipLaunchKernelGGL(k_layer_emit, dim3(tiles), dim3(256), 0, s, a, layer, tile_cnt);
 }
 
 // Empty every OctreeNode slot that held a node in the previous build but not in this one

@@ -27,9 +27,8 @@ void launch_copy_u32_count(const uint32_t* in, uint32_t* out, const uint32_t* d_
                            uint32_t cap, hipStream_t s);
 void launch_copy_u32(const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s);
 
-// Exclusive scan of len values; status holds scan_tiles(len) words + 1 counter word, zeroed.
-void launch_scan(const uint32_t* in, uint32_t* out, uint32_t len, uint32_t* status,
-                 uint32_t* err, hipStream_t s);
+// Exclusive scan of len values; sums holds scan_tiles(len) words (no clearing needed).
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t len, uint32_t* sums, hipStream_t s);
 
 // 1-bit split pass (RadixSort.metal:89-131 semantics).
 void launch_split_prep(const uint64_t* kin, uint32_t* e, uint32_t bit, const uint32_t* d_count,
@@ -49,15 +48,13 @@ struct RadixPlan {
     uint32_t npasses;
     RadixPassDesc pass[kMaxRadixPasses];
 };
-// hist: npasses x 256 words, zeroed.
-void launch_radix_hist(const uint64_t* keys, const uint32_t* d_count, uint32_t cap,
-                       const uint32_t* seg_of, const RadixPlan& plan, uint32_t* hist,
-                       hipStream_t s);
-// status: sort_tiles(cap) x 256 words + 1 tile counter, zeroed.
+// One stable digit pass = per-tile digit counts, per-digit scan over tiles, rank + scatter.
+// work: radix_work_words(cap) words (fully rewritten by each pass; no clearing needed).
+inline uint64_t radix_work_words(uint32_t cap) { return 2ull * sort_tiles(cap) * 256 + 256; }
 void launch_radix_pass(const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
                        uint32_t* iout, const uint32_t* d_count, uint32_t cap,
-                       const uint32_t* hist, RadixPassDesc desc, const uint32_t* seg_of,
-                       uint32_t* status, uint32_t* err, hipStream_t s);
+                       RadixPassDesc desc, const uint32_t* seg_of, uint32_t* work,
+                       hipStream_t s);
 
 // ---- k_tree.hip ---------------------------------------------------------------------------
 struct TreeArgs {
@@ -87,9 +84,9 @@ void launch_cube_finalize(const uint32_t* minmax, float4* cubes, uint32_t nsys, 
 void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const float4* cubes,
                    const astro_params& p, uint64_t* codes, uint64_t* codes2, uint32_t* idx,
                    hipStream_t s);
-// status: scan_tiles(cap) words + 1 counter per layer region, zeroed.
-void launch_leaf_layer(const TreeArgs& a, uint32_t* status, hipStream_t s);
-void launch_internal_layer(const TreeArgs& a, uint32_t layer, uint32_t* status, hipStream_t s);
+// Layer `layer` (0 = leaves) as a count/emit kernel pair; tile_cnt holds
+// scan_tiles(candidate capacity) words and needs no clearing.
+void launch_layer(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, hipStream_t s);
 void launch_fill_sys_of(uint32_t* sys_of, const uint32_t* d_off, uint32_t nsys, uint32_t n,
                         hipStream_t s);
 void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s);
@@ -133,8 +130,13 @@ struct ForceArgs {
     uint64_t node_total;        // nodes in the current layer table (index bound)
     uint32_t ns, nt;
     uint32_t block;             // traversal block size (64, 128, 256 or 512)
+    // accept(side, r) == (RN(side / r) < theta)  <=>  side < thr*r || (side == thr*r && tie)
+    double acc_thr;
+    uint32_t acc_tie;
     astro_params p;
 };
+// Fills acc_thr/acc_tie from p.theta (see accept_open_angle in k_force.hip).
+void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);
 void launch_force_compact(const ForceArgs& a, hipStream_t s);
 void launch_force_reference(const ForceArgs& a, hipStream_t s);
