@@ -43,8 +43,8 @@ static void compute_layers(const std::vector<uint32_t>& sizes, uint32_t bits, ui
 
 static uint64_t scratch_need(uint32_t nmax, const uint64_t* cap, uint32_t layers) {
     uint64_t sort = radix_work_words(nmax);                          // one digit pass
-    uint64_t lay = scan_tiles(nmax);                                 // leaf layer
-    for (uint32_t l = 1; l < layers; l++) lay += scan_tiles((uint32_t)cap[l - 1]);
+    uint64_t lay = layer_tiles(nmax) + layer_supers(nmax);           // leaf layer
+    for (uint32_t l = 1; l < layers; l++) lay += layer_tiles(cap[l - 1]) + layer_supers(cap[l - 1]);
     uint64_t scan = scan_tiles(nmax) + 1;                            // astro_scan
     return std::max(std::max(sort, lay), scan);
 }
@@ -125,7 +125,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
                     c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
                     c->leaf_of_sorted, c->cubes, c->minmax, c->err_word, c->stats, c->scratch,
-                    c->d_sys_off, c->phi, c->diag_partial};
+                    c->d_sys_off, c->phi, c->diag_partial, c->levels};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -204,6 +204,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->child_count, 1);
     e = e ? e : dalloc(c->device_bytes, &c->dev_consts, 4);
     e = e ? e : dalloc(c->device_bytes, &c->tnodes, cap_bound);
+    e = e ? e : dalloc(c->device_bytes, &c->levels, cap_bound);
     e = e ? e : dalloc(c->device_bytes, &c->onodes, cap_bound);
     e = e ? e : dalloc(c->device_bytes, &c->layer_counts, kMaxLayers);
     e = e ? e : dalloc(c->device_bytes, &c->layer_counts_prev, kMaxLayers);
@@ -758,6 +759,7 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.sorted_keys = c->sorted_codes;
     a.sorted_idx = c->sorted_idx;
     a.tnodes = c->tnodes;
+    a.levels = c->levels;
     a.onodes = c->emit ? c->onodes : nullptr;
     a.counts = c->layer_counts;
     a.leaf_of_sorted = c->leaf_of_sorted;
@@ -792,21 +794,37 @@ static int enqueue_sort(astro_ctx* c) {
 static int enqueue_layers(astro_ctx* c) {
     hipStream_t s = c->stream;
     TreeArgs a = tree_args(c);
-    // scratch: one tile-count block per layer (all layers' blocks kept apart)
-    uint64_t need = scan_tiles(c->nt);
-    for (uint32_t l = 1; l < c->layers; l++) need += scan_tiles((uint32_t)c->cap[l - 1]);
-    if (need > c->scratch_words) return fail(c, ASTRO_ERR_CAPACITY, "layer scratch too small");
+    // scratch: all layers' supertile sums first (one clear), then per-layer tile counts
+    uint64_t nsup = 0, ntile = 0;
+    for (uint32_t l = 0; l < c->layers; l++) {
+        const uint64_t cand = l == 0 ? c->nt : c->cap[l - 1];
+        nsup += layer_supers(cand);
+        ntile += layer_tiles(cand);
+    }
+    if (nsup + ntile > c->scratch_words)
+        return fail(c, ASTRO_ERR_CAPACITY, "layer scratch too small");
     if (a.onodes && c->onodes_dirty) {
         launch_clear_onodes(c->onodes, c->node_capacity, s);
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
         c->onodes_dirty = false;
     }
-    uint32_t* st = c->scratch;
-    launch_layer(a, 0, st, s);
-    st += scan_tiles(c->nt);
-    for (uint32_t l = 1; l < c->layers; l++) {
-        launch_layer(a, l, st, s);
-        st += scan_tiles((uint32_t)c->cap[l - 1]);
+    // supertile sums and the tile counts of layers 1.. are accumulated: clear them
+    launch_fill_u32(c->scratch, 0, nsup + ntile, s);
+    std::vector<uint32_t*> sup(c->layers), tc(c->layers);
+    uint32_t* p = c->scratch;
+    for (uint32_t l = 0; l < c->layers; l++) {
+        sup[l] = p;
+        p += layer_supers(l == 0 ? c->nt : c->cap[l - 1]);
+    }
+    for (uint32_t l = 0; l < c->layers; l++) {
+        tc[l] = p;
+        p += layer_tiles(l == 0 ? c->nt : c->cap[l - 1]);
+    }
+    launch_layer_count(a, tc[0], sup[0], s);
+    for (uint32_t l = 0; l < c->layers; l++) {
+        const bool top = l + 1 == c->layers;
+        launch_layer_emit(a, l, tc[l], sup[l], top ? nullptr : tc[l + 1],
+                          top ? nullptr : sup[l + 1], s);
     }
     if (a.onodes) {
         launch_clear_stale(a, c->layer_counts_prev, s);

@@ -38,6 +38,7 @@ struct astro_ctx {
     uint32_t* dev_consts = nullptr;    // [0] = ns, [1] = nt
     // tree
     TNode* tnodes = nullptr;
+    uint8_t* levels = nullptr;         // head level per node (parallel to tnodes)
     OctreeNode* onodes = nullptr;
     uint64_t node_capacity = 0;
     uint32_t *layer_counts = nullptr, *layer_counts_prev = nullptr, *leaf_of_sorted = nullptr;

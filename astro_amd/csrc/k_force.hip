@@ -45,35 +45,23 @@ __device__ inline void accumulate(Acc3& F, float s, float dx, float dy, float dz
     }
 }
 
-// The opening test of the reference, RN(side / r) < theta (BarnesHut.metal:75), without the
-// division.  RN is monotone, so RN(x) < theta exactly when x lies below the midpoint m between
-// theta and its float predecessor a, or on m when the tie rounds to a (a's last mantissa bit
-// even).  m has at most 26 significant bits and r 24, so thr*r is exact in double and the
-// comparison with the power-of-two side is exact: the decision equals the divided one for
-// every input, which keeps the traversal bit-identical to the oracle.
 void set_accept_threshold(ForceArgs& a) {
-    const float th = a.p.theta;
-    if (!(th > 0.0f)) {  // nothing is ever accepted
-        a.acc_thr = 0.0;
-        a.acc_tie = 0;
-        return;
-    }
-    if (std::isinf(th)) {  // everything finite is accepted
-        a.acc_thr = INFINITY;
-        a.acc_tie = 0;
-        return;
-    }
-    const float pred = std::nextafter(th, 0.0f);
-    uint32_t bits;
-    std::memcpy(&bits, &pred, sizeof(bits));
-    a.acc_thr = ((double)pred + (double)th) * 0.5;
-    a.acc_tie = (bits & 1u) == 0u ? 1u : 0u;
+    a.acc = make_accept_rule(a.p.theta);
+    a.fast_sqrt = a.p.r2_floor >= 0x1p-96f ? 1u : 0u;
 }
 
-__device__ inline bool accept_open_angle(float side, float r, double thr, uint32_t tie) {
-    const double P = thr * (double)r;
-    const double S = (double)side;
-    return S < P || (tie && S == P);
+// Correctly rounded sqrt for normal x >= 2^-96: the hardware estimate corrected by one ulp
+// with two fused residuals -- the same correction the compiler emits for sqrtf, without the
+// rescaling it adds for tiny inputs.
+__device__ inline float sqrt_rn_normal(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, x);
+    const float rp = __builtin_fmaf(-sp, s, x);
+    s = rm <= 0.0f ? sm : s;
+    s = rp > 0.0f ? sp : s;
+    return s;
 }
 
 __device__ inline float clampf(float v, float c) {
@@ -112,6 +100,9 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
         else side0 = ldexpf(a.cubes[sys].w, -(int)a.p.bits_per_axis);
         const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
         const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
+        const bool fast_sqrt = a.fast_sqrt != 0u;
+        // a node at layer l is accepted iff r >= rmin0 * 2^l (accept.hpp)
+        const float rmin0 = accept_rmin(side0, a.acc);
 
         Acc3 F;
         double phi = 0.0;
@@ -126,8 +117,8 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
                 top_first = e >> 3;
                 top_cnt = (e & 7u) + 1u;
             }
-            const uint32_t n = top_first + top_cnt - 1u;
             top_cnt--;
+            const uint32_t n = top_first + top_cnt;
             if (n >= a.node_total) {
                 atomicOr(a.err, ERR_NODE_INDEX);
                 top_cnt = 0;
@@ -137,33 +128,40 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
             const float4* np = reinterpret_cast<const float4*>(a.tnodes + n);
             const float4 cm = np[0];
             const uint2 li = reinterpret_cast<const uint2*>(np)[2];
-            if (MODE == 1) visits++;
             const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
             float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
             r2 = fmaxf(r2, r2f);
-            const float rr = fmaxf(sqrtf(r2), rf);
+            const float rr = fmaxf(fast_sqrt ? sqrt_rn_normal(r2) : sqrtf(r2), rf);
             const uint32_t layer = tn_layer(li.y);
-            bool interact, open = false;
-            if (layer == 0u) {
-                interact = (n != self) && (r2 > cut);
-                if (MODE == 1) leafs += interact;
+            const bool leaf = layer == 0u;
+            const bool acc = rr >= ldexpf(rmin0, (int)layer);
+            const bool interact = leaf ? ((n != self) && (r2 > cut)) : acc;
+            const bool open = !leaf && !acc;
+            // the term is computed for every lane and kept only where it interacts
+            const float gm = G * cm.w;
+            const float s = (gm / (r2 * rr));
+            if (KAHAN) {
+                Acc3 T = F;
+                accumulate<true>(T, s, dx, dy, dz);
+                F.x = interact ? T.x : F.x;
+                F.y = interact ? T.y : F.y;
+                F.z = interact ? T.z : F.z;
+                F.cx = interact ? T.cx : F.cx;
+                F.cy = interact ? T.cy : F.cy;
+                F.cz = interact ? T.cz : F.cz;
             } else {
-                const float side = ldexpf(side0, (int)layer);
-                interact = accept_open_angle(side, rr, a.acc_thr, a.acc_tie);
-                open = !interact;
-                if (MODE == 1) {
-                    monos += interact;
-                    opens += open;
-                }
+                F.x = interact ? __builtin_fmaf(s, dx, F.x) : F.x;
+                F.y = interact ? __builtin_fmaf(s, dy, F.y) : F.y;
+                F.z = interact ? __builtin_fmaf(s, dz, F.z) : F.z;
             }
-            if (interact) {
-                const float gm = G * cm.w;
-                const float s = (gm / (r2 * rr));
-                accumulate<KAHAN>(F, s, dx, dy, dz);
-                if (MODE == 2) phi -= (double)gm / (double)rr;
+            if (MODE == 1) {
+                visits++;
+                leafs += leaf && interact;
+                monos += !leaf && interact;
+                opens += open;
             }
+            if (MODE == 2 && interact) phi -= (double)gm / (double)rr;
             if (open) {
-                const uint32_t entry = (top_first << 3) | (top_cnt - 1u);
                 const bool push = top_cnt > 0u;
                 if (push && sp >= depth) {
                     atomicOr(a.err, ERR_STACK);
@@ -171,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
                     sp = 0;
                 } else {
                     if (push) {
-                        s_stack[sp * NT + tid] = entry;
+                        s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
                         sp++;
                     }
                     top_first = li.x;
@@ -271,7 +269,7 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
             interact = !self && r2 > a.p.leaf_cutoff_r2;
         } else {
             const float side = ldexpf(side0, (int)layer);
-            interact = (side / rr) < a.p.theta;
+            interact = accept_exact(side, rr, a.acc);
             if (!interact) {
                 for (int c = 0; c < 8; c++) {
                     const uint32_t ch = nd->children[c];

@@ -247,7 +247,7 @@ __device__ inline bool is_leaf_head(const TreeArgs& a, uint32_t j) {
 }
 
 template <typename AGG>
-__device__ void emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
+__device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     uint32_t end = j + 1;
     while (end < a.nt && !is_leaf_head(a, end)) end++;
     // Octree.metal:71: the reference ends the last leaf's run at numUnique (= u + 1).
@@ -274,7 +274,9 @@ __device__ void emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     t.head = j;
     t.pad = 0;
     a.tnodes[n] = t;
+    a.levels[n] = (uint8_t)h;
     if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0);
+    return h;
 }
 
 // -------------------------------------------------------------------------- internal layer
@@ -282,8 +284,8 @@ __device__ void emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
 // when its head level is >= l; the parent aggregates children c, c+1, ... up to the next
 // head, in order, and records them as a contiguous (first, count) run.
 template <typename AGG>
-__device__ void emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint32_t p,
-                            uint32_t nchild_layer) {
+__device__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint32_t p,
+                                uint32_t nchild_layer) {
     const TNode* ch = a.tnodes + a.off[l - 1];
     AGG g;
     uint32_t k = 0;
@@ -312,57 +314,97 @@ __device__ void emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint32_t 
     t.head = head;
     t.pad = 0;
     a.tnodes[n] = t;
+    a.levels[n] = (uint8_t)level;
     if (a.onodes)
         write_onode(&a.onodes[n], a.sorted_keys[head] >> (3 * l), cm, t.first, k, l);
+    return level;
 }
 
 // ------------------------------------------------------------------- layer kernel pair
-// Layer l is built by two launches over tiles of 2048 candidates (sorted particles for the
-// leaf layer, nodes of layer l-1 otherwise): k_layer_count stores each tile's number of heads;
-// k_layer_emit adds up the counts of the tiles before its own (no block waits on another),
-// scans its tile in LDS and writes one node per head.
+// Layer l is built by two launches over tiles of kLayerTile candidates (sorted particles for
+// the leaf layer, nodes of layer l-1 otherwise).  k_layer_count stores each tile's number of
+// heads and adds it to its supertile's sum (kSuper tiles); k_layer_emit takes its prefix from
+// the supertile sums and the tiles before it in its own supertile, scans its tile in LDS and
+// writes one node per head.  No block waits on another.  Head tests read the per-node level
+// bytes, not the 32-B nodes.
+constexpr uint32_t kLayerItems = 2;
+constexpr uint32_t kLayerTile = 256 * kLayerItems;
+constexpr uint32_t kSuper = 64;
+
 __device__ inline uint32_t layer_candidates(const TreeArgs& a, uint32_t l) {
     return l == 0 ? a.nt : a.counts[l - 1];
 }
 __device__ inline bool layer_head(const TreeArgs& a, uint32_t l, uint32_t c) {
     if (l == 0) return is_leaf_head(a, c);
-    return tn_level(a.tnodes[a.off[l - 1] + c].info) >= l;
+    return a.levels[a.off[l - 1] + c] >= l;
 }
 
 __global__ __launch_bounds__(256) void k_layer_count(TreeArgs a, uint32_t l,
-                                                     uint32_t* tile_cnt) {
+                                                     uint32_t* tile_cnt, uint32_t* super) {
     __shared__ uint32_t s_tmp[8];
     const uint32_t n = layer_candidates(a, l);
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    const uint64_t base = (uint64_t)blockIdx.x * kLayerTile;
     if (base >= n) return;
-    const uint32_t c0 = (uint32_t)base + threadIdx.x * 8u;
+    const uint32_t c0 = (uint32_t)base + threadIdx.x * kLayerItems;
     uint32_t cnt = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < 8; i++) {
+    for (uint32_t i = 0; i < kLayerItems; i++) {
         const uint32_t c = c0 + i;
         if (c < n && layer_head(a, l, c)) cnt++;
     }
     uint32_t total;
     block_exclusive_scan<256>(cnt, s_tmp, &total);
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+    if (threadIdx.x == 0) {
+        tile_cnt[blockIdx.x] = total;
+        if (total) atomicAdd(&super[blockIdx.x / kSuper], total);
+    }
 }
 
+// Adds the heads of layer l+1 among the parents this block writes (positions
+// [first, first + total), levels in lev[]) to the next layer's tile counts and supertile sums.
+// The positions span at most three tiles of kLayerTile.
+__device__ inline void count_next_heads(uint32_t l, uint32_t first, uint32_t total,
+                                        const uint32_t* pos, const uint32_t* lev, uint32_t k,
+                                        uint32_t* s_cnt, uint32_t* next_cnt,
+                                        uint32_t* next_super) {
+    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t t0 = first / kLayerTile;
+    for (uint32_t i = 0; i < k; i++)
+        if (lev[i] >= l + 1) atomicAdd(&s_cnt[pos[i] / kLayerTile - t0], 1u);
+    __syncthreads();
+    if (threadIdx.x < 3 && total) {
+        const uint32_t c = s_cnt[threadIdx.x];
+        const uint32_t t = t0 + threadIdx.x;
+        if (c) {
+            atomicAdd(&next_cnt[t], c);
+            atomicAdd(&next_super[t / kSuper], c);
+        }
+    }
+}
+
+// next_cnt / next_super: the next layer's tile counts and supertile sums (zeroed), or null
+// for the top layer.
 __global__ __launch_bounds__(256) void k_layer_emit(TreeArgs a, uint32_t l,
-                                                    const uint32_t* __restrict__ tile_cnt) {
+                                                    const uint32_t* __restrict__ tile_cnt,
+                                                    const uint32_t* __restrict__ super,
+                                                    uint32_t* next_cnt, uint32_t* next_super) {
     __shared__ uint32_t s_tmp[8];
+    __shared__ uint32_t s_cnt[3];
     const uint32_t n = layer_candidates(a, l);
     const uint32_t tile = blockIdx.x;
-    const uint64_t base = (uint64_t)tile * kScanTile;
+    const uint64_t base = (uint64_t)tile * kLayerTile;
     if (base >= n) return;
-    // heads in the tiles before this one
+    const uint32_t sup = tile / kSuper;
     uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < tile; t += 256) part += tile_cnt[t];
+    for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t];
+    if (threadIdx.x < tile - sup * kSuper) part += tile_cnt[sup * kSuper + threadIdx.x];
     uint32_t prefix;
     block_exclusive_scan<256>(part, s_tmp, &prefix);
-    const uint32_t c0 = (uint32_t)base + threadIdx.x * 8u;
+    const uint32_t c0 = (uint32_t)base + threadIdx.x * kLayerItems;
     uint32_t hmask = 0, cnt = 0;
 #pragma unroll
-    for (uint32_t i = 0; i < 8; i++) {
+    for (uint32_t i = 0; i < kLayerItems; i++) {
         const uint32_t c = c0 + i;
         if (c < n && layer_head(a, l, c)) {
             hmask |= 1u << i;
@@ -370,44 +412,62 @@ __global__ __launch_bounds__(256) void k_layer_emit(TreeArgs a, uint32_t l,
         }
     }
     uint32_t total;
-    const uint32_t excl = block_exclusive_scan<256>(cnt, s_tmp, &total);
-    if (threadIdx.x == 0 && base + kScanTile >= n) {
+    uint32_t p = prefix + block_exclusive_scan<256>(cnt, s_tmp, &total);
+    if (threadIdx.x == 0 && base + kLayerTile >= n) {
         a.counts[l] = prefix + total;  // the last tile knows the layer's size
         if (prefix + total > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
     }
-    uint32_t p = prefix + excl;  // index of this thread's first head
+    uint32_t pos[kLayerItems], lev[kLayerItems], k = 0;
     if (l == 0) {
         const bool fp64 = a.fp64_from == 0;
-        for (uint32_t i = 0; i < 8; i++) {
+        for (uint32_t i = 0; i < kLayerItems; i++) {
             const uint32_t j = c0 + i;
             if (j >= n) break;
             if (hmask & (1u << i)) {
-                if (fp64) emit_leaf<Agg64>(a, j, p);
-                else emit_leaf<Agg32>(a, j, p);
+                const uint32_t h = fp64 ? emit_leaf<Agg64>(a, j, p) : emit_leaf<Agg32>(a, j, p);
+                pos[k] = p;
+                lev[k++] = h;
                 p++;
             }
             a.leaf_of_sorted[j] = p - 1;
         }
     } else {
         const bool fp64 = l >= a.fp64_from;
-        for (uint32_t i = 0; i < 8; i++) {
+        for (uint32_t i = 0; i < kLayerItems; i++) {
             if (!(hmask & (1u << i))) continue;
             if (p < a.cap[l]) {
-                if (fp64) emit_parent<Agg64>(a, l, c0 + i, p, n);
-                else emit_parent<Agg32>(a, l, c0 + i, p, n);
+                const uint32_t h = fp64 ? emit_parent<Agg64>(a, l, c0 + i, p, n)
+                                        : emit_parent<Agg32>(a, l, c0 + i, p, n);
+                pos[k] = p;
+                lev[k++] = h;
             }
             p++;
         }
     }
+    if (next_cnt)
+        count_next_heads(l, prefix, total, pos, lev, k, s_cnt, next_cnt, next_super);
 }
 
-// tile_cnt: scan_tiles(cap of the candidates) words; no clearing needed.
-void launch_layer(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, hipStream_t s) {
+uint32_t layer_tiles(uint64_t candidates) {
+    return (uint32_t)((candidates + kLayerTile - 1) / kLayerTile);
+}
+uint32_t layer_supers(uint64_t candidates) {
+    return (layer_tiles(candidates) + kSuper - 1) / kSuper;
+}
+
+void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s) {
+    if (!a.nt) return;
+    hipLaunchKernelGGL(k_layer_count, dim3(layer_tiles(a.nt)), dim3(256), 0, s, a, 0u, tile_cnt,
+                       super);
+}
+
+void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_cnt,
+                       const uint32_t* super, uint32_t* next_cnt, uint32_t* next_super,
+                       hipStream_t s) {
     const uint64_t cand_cap = layer == 0 ? a.nt : a.cap[layer - 1];
     if (!cand_cap) return;
-    const uint32_t tiles = scan_tiles((uint32_t)cand_cap);
-    hipLaunchKernelGGL(k_layer_count, dim3(tiles), dim3(256), 0, s, a, layer, tile_cnt);
-    hipLaunchKernelGGL(k_layer_emit, dim3(tiles), dim3(256), 0, s, a, layer, tile_cnt);
+    hipLaunchKernelGGL(k_layer_emit, dim3(layer_tiles(cand_cap)), dim3(256), 0, s, a, layer,
+                       tile_cnt, super, next_cnt, next_super);
 }
 
 // Empty every OctreeNode slot that held a node in the previous build but not in this one

@@ -5,13 +5,14 @@
 #include <stdint.h>
 
 #include "../../include/astro/astro.h"
+#include "accept.hpp"
 #include "common.hpp"
 
 namespace astro {
 
 // ---- k_sort.hip ---------------------------------------------------------------------------
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortItems = 8;
+constexpr uint32_t kSortItems = 4;
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;  // keys per tile
 constexpr uint32_t kScanTile = 2048;                       // values per scan tile
 
@@ -63,6 +64,7 @@ struct TreeArgs {
     const uint64_t* sorted_keys;
     const uint32_t* sorted_idx;
     TNode* tnodes;
+    uint8_t* levels;            // head level of every node (parallel to tnodes)
     OctreeNode* onodes;         // may be null (no emission)
     uint32_t* counts;           // [layers] valid nodes per layer
     uint32_t* leaf_of_sorted;   // [nt]
@@ -84,9 +86,16 @@ void launch_cube_finalize(const uint32_t* minmax, float4* cubes, uint32_t nsys, 
 void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const float4* cubes,
                    const astro_params& p, uint64_t* codes, uint64_t* codes2, uint32_t* idx,
                    hipStream_t s);
-// Layer `layer` (0 = leaves) as a count/emit kernel pair; tile_cnt holds
-// scan_tiles(candidate capacity) words and needs no clearing.
-void launch_layer(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, hipStream_t s);
+// Layers: the leaf layer's heads are counted per tile by launch_layer_count; the emit of
+// layer l writes its nodes and adds the heads of layer l+1 to next_cnt / next_super (zeroed;
+// null for the top layer).  Per layer: layer_tiles(candidate capacity) tile counts and
+// layer_supers(...) supertile sums.
+uint32_t layer_tiles(uint64_t candidates);
+uint32_t layer_supers(uint64_t candidates);
+void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s);
+void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_cnt,
+                       const uint32_t* super, uint32_t* next_cnt, uint32_t* next_super,
+                       hipStream_t s);
 void launch_fill_sys_of(uint32_t* sys_of, const uint32_t* d_off, uint32_t nsys, uint32_t n,
                         hipStream_t s);
 void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s);
@@ -130,12 +139,11 @@ struct ForceArgs {
     uint64_t node_total;        // nodes in the current layer table (index bound)
     uint32_t ns, nt;
     uint32_t block;             // traversal block size (64, 128, 256 or 512)
-    // accept(side, r) == (RN(side / r) < theta)  <=>  side < thr*r || (side == thr*r && tie)
-    double acc_thr;
-    uint32_t acc_tie;
+    AcceptRule acc;             // opening test (accept.hpp), from p.theta
+    uint32_t fast_sqrt;         // 1: r2 >= r2_floor >= 2^-96, so no denormal scaling needed
     astro_params p;
 };
-// Fills acc_thr/acc_tie from p.theta (see accept_open_angle in k_force.hip).
+// Fills acc and fast_sqrt from p.
 void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);
 void launch_force_compact(const ForceArgs& a, hipStream_t s);

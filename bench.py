@@ -105,6 +105,17 @@ def main():
     wall_graph = time.perf_counter() - tg
     ctx.check()
 
+    # the same steps without writing the 128-B OctreeNode array (compact nodes only)
+    ctx.set_emit_octree_nodes(False)
+    ctx.step(a.dt, 2)
+    ctx.sync()
+    tn = time.perf_counter()
+    ctx.step(a.dt, a.steps)
+    ctx.sync()
+    wall_noemit = time.perf_counter() - tn
+    ctx.set_emit_octree_nodes(True)
+    ctx.check()
+
     K = a.steps
     value = n * K / wall
     force_ms = stage_ms["force"] / K
@@ -152,6 +163,8 @@ def main():
         "stages": stages,
         "tree_nodes": V,
         "graph_replay": {"value": n * K / wall_graph, "ms_per_step": wall_graph / K * 1e3},
+        "without_octree_nodes": {"value": n * K / wall_noemit,
+                                 "ms_per_step": wall_noemit / K * 1e3},
     }
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
