@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup + steps only, no JSON extras (for rocprofv3 runs)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="profile mode: also copy this many MiB device-to-device (a plain copy "
+                         "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -50,6 +53,24 @@ def stage_bytes(n, V):
         "build": n * (4 + 16 + 8 + 4) + V * (32 + 128 + 32),
         "update": n * (16 + 16 + 16 + 4 + 16 + 16),
     }
+
+
+def calibration_copy(nbytes):
+    """One cold device-to-device copy of nbytes (HIP's blit kernel), for counter scaling."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    big = ctypes.c_void_p()  # evict the L2 by touching another buffer
+    assert hip.hipMalloc(ctypes.byref(big), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(big, 2, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    for p in (src, dst, big):
+        hip.hipFree(p)
 
 
 def cpu_baseline(pos, vel, col, dt, steps, threads):
@@ -91,6 +112,8 @@ def main():
     wall = time.perf_counter() - t0
     ctx.check()
     if a.profile_only:
+        if a.calibrate_copy_mb:
+            calibration_copy(a.calibrate_copy_mb << 20)
         print(json.dumps({"wall_s": wall, "stage_ms": stage_ms}))
         return
 
