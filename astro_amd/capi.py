@@ -108,6 +108,7 @@ SIGNATURES = [
     ("astro_set_force_block", _I, [_P, _U32]),
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_F)]),
     ("astro_diagnostics", _I, [_P, _P]),
+    ("astro_direct_forces_fp64", _I, [_P, _P, _U32, _P]),
     ("astro_device_bytes", _I, [_P, ctypes.POINTER(_U64)]),
     ("astro_server_create", _I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(AstroParams),
                                  ctypes.POINTER(_P)]),
@@ -418,6 +419,13 @@ class Context:
         self._chk(self.lib.astro_system_count(self.h, ctypes.byref(ns)))
         out = np.zeros((ns.value, 9), np.float64)
         self._chk(self.lib.astro_diagnostics(self.h, ptr(out)))
+        return out
+
+    def direct_forces_fp64(self, targets):
+        """FP64 direct-sum accelerations (n_targets, 3) of the given particle indices."""
+        t = np.ascontiguousarray(targets, np.uint32)
+        out = np.zeros((len(t), 3), np.float64)
+        self._chk(self.lib.astro_direct_forces_fp64(self.h, ptr(t), len(t), ptr(out)))
         return out
 
     def device_bytes(self):

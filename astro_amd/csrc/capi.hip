@@ -89,6 +89,7 @@ extern "C" void astro_params_compat(astro_params* p) {
     p->quirk_last_leaf = 1;   // Octree.metal:71
     p->fp64_top_layers = 0;
     p->kahan = 0;
+    p->multipole = 0;
 }
 
 extern "C" void astro_params_physics(astro_params* p, uint32_t bits_per_axis) {
@@ -125,7 +126,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
                     c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
                     c->leaf_of_sorted, c->cubes, c->minmax, c->err_word, c->stats, c->scratch,
-                    c->d_sys_off, c->phi, c->diag_partial, c->levels};
+                    c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -247,8 +248,17 @@ extern "C" int astro_set_params(astro_ctx* c, const astro_params* p) {
         return fail(c, ASTRO_ERR_ARG, "bits_per_axis outside [1, configured maximum]");
     if (p->quantizer > ASTRO_QUANT_BBOX_POW2 || p->side_rule > ASTRO_SIDE_EXACT)
         return fail(c, ASTRO_ERR_ARG, "unknown quantizer or side rule");
+    if (p->multipole != 0 && p->multipole != 2)
+        return fail(c, ASTRO_ERR_ARG, "multipole must be 0 or 2");
     if (p->side_rule == ASTRO_SIDE_SHIFTED && p->bits_per_axis > 10)
         return fail(c, ASTRO_ERR_ARG, "the shifted side rule overflows above 10 layers");
+    if (p->multipole == 2 && !c->qmom) {  // second moments: 32 B per node, kept once made
+        hipSetDevice(c->device);
+        if (dalloc(c->device_bytes, &c->qmom, 2 * c->node_capacity) != hipSuccess) {
+            c->qmom = nullptr;
+            return fail(c, ASTRO_ERR_NOMEM, "no device memory for the second moments");
+        }
+    }
     c->params = *p;
     return refresh_layers(c);
 }
@@ -711,6 +721,7 @@ extern "C" int astro_build_internal_nodes(astro_ctx* c, uint32_t count) {
 static ForceArgs force_args(astro_ctx* c) {
     ForceArgs a{};
     a.tnodes = c->tnodes;
+    a.qmom = c->qmom;
     a.onodes = c->onodes;
     a.pos = c->pos;
     a.type = c->type;
@@ -738,6 +749,9 @@ extern "C" int astro_barnes_hut(astro_ctx* c) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
     if (c->nsys > 1) return fail(c, ASTRO_ERR_STATE, "stage traversal is single-system");
+    if (c->params.multipole != 0)
+        return fail(c, ASTRO_ERR_STATE,
+                    "the stage aggregation keeps no second moments: use astro_compute_forces");
     launch_force_reference(force_args(c), c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -760,6 +774,8 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.sorted_idx = c->sorted_idx;
     a.tnodes = c->tnodes;
     a.levels = c->levels;
+    a.qmom = c->qmom;
+    a.quad = c->params.multipole == 2 ? 1u : 0u;
     a.onodes = c->emit ? c->onodes : nullptr;
     a.counts = c->layer_counts;
     a.leaf_of_sorted = c->leaf_of_sorted;
@@ -1021,7 +1037,8 @@ extern "C" int astro_autotune(astro_ctx* c, uint32_t reps, uint32_t* chosen, flo
 
 // FP64 diagnostics per system: {mass, kinetic, potential, px, py, pz, mx, my, mz} (9 doubles
 // per system).  The potential comes from one extra traversal of the current tree (the same
-// interactions as the forces, monopole -G M / sqrt(r^2 + eps^2)), so call it after a build.
+// interactions as the forces: -G M / sqrt(r^2 + eps^2), plus the quadrupole term of accepted
+// nodes when the quadrupole option is on).
 extern "C" int astro_diagnostics(astro_ctx* c, double* out) {
     if (!c || !out) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
@@ -1041,6 +1058,51 @@ extern "C" int astro_diagnostics(astro_ctx* c, double* out) {
             double acc = 0.0;
             for (uint32_t ch = 0; ch < kDiagChunks; ch++) acc += part[((size_t)s * kDiagChunks + ch) * 9 + k];
             out[(size_t)s * 9 + k] = acc;
+        }
+    return ASTRO_OK;
+}
+
+// FP64 direct-sum accelerations of the given targets (particle indices) over the sources of
+// their systems, for accuracy audits of the tree forces.  Synchronous; allocates its own
+// temporaries.
+extern "C" int astro_direct_forces_fp64(astro_ctx* c, const uint32_t* targets, uint32_t n_targets,
+                                        double* out_xyz) {
+    if (!c || (n_targets && (!targets || !out_xyz))) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    if (!n_targets) return ASTRO_OK;
+    for (uint32_t t = 0; t < n_targets; t++)
+        if (targets[t] >= c->nt) return fail(c, ASTRO_ERR_ARG, "target beyond the tree particles");
+    constexpr uint32_t kChunks = 16;
+    uint32_t* d_t = nullptr;
+    double* d_part = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_t), (size_t)n_targets * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void**>(&d_part), (size_t)n_targets * kChunks * 3 * sizeof(double));
+    std::vector<double> part;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_t, targets, (size_t)n_targets * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess) {
+        launch_direct_fp64(c->pos, c->nsys > 1 ? c->sys_of : nullptr, c->d_sys_off, c->nt, d_t,
+                           n_targets, kChunks, (double)c->params.G, (double)c->params.eps2, d_part,
+                           c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        part.resize((size_t)n_targets * kChunks * 3);
+        e = hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d_t) hipFree(d_t);
+    if (d_part) hipFree(d_part);
+    if (e != hipSuccess)
+        return fail(c, ASTRO_ERR_HIP, std::string("direct sum: ") + hipGetErrorString(e));
+    for (uint32_t t = 0; t < n_targets; t++)
+        for (int k = 0; k < 3; k++) {
+            double acc = 0.0;
+            for (uint32_t ch = 0; ch < kChunks; ch++) acc += part[((size_t)t * kChunks + ch) * 3 + k];
+            out_xyz[(size_t)t * 3 + k] = acc;
         }
     return ASTRO_OK;
 }

@@ -40,6 +40,7 @@ struct astro_ctx {
     TNode* tnodes = nullptr;
     uint8_t* levels = nullptr;         // head level per node (parallel to tnodes)
     OctreeNode* onodes = nullptr;
+    float4* qmom = nullptr;            // 2 per node: second moments, allocated on first use
     uint64_t node_capacity = 0;
     uint32_t *layer_counts = nullptr, *layer_counts_prev = nullptr, *leaf_of_sorted = nullptr;
     float4* cubes = nullptr;

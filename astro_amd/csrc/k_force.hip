@@ -45,6 +45,58 @@ __device__ inline void accumulate(Acc3& F, float s, float dx, float dy, float dz
     }
 }
 
+// Quadrupole term of an accepted internal node with second moments S (xx, yy, zz, xy, xz, yz)
+// about its centre of mass, d = COM - target, D = r2 * r:
+//   a_Q = u * q,  u = 3G / (D r2),  q = w d - (S d + T/2 d),  w = 2.5 (d.S.d) / r2,  T = tr S.
+// The trace term is what Plummer softening adds to the expansion of  -G m / sqrt(|d|^2 + eps2).
+struct QuadTerm {
+    float u, q0, q1, q2;
+    double phi;  // -G (1.5 d.S.d - 0.5 T r2) / (D r2), for the potential diagnostics
+};
+__device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, float dz, float r2,
+                                     float D, float G) {
+    const float Sxx = s0.x, Syy = s0.y, Szz = s0.z, Sxy = s0.w, Sxz = s1.x, Syz = s1.y;
+    const float Sd0 = __builtin_fmaf(Sxz, dz, __builtin_fmaf(Sxy, dy, Sxx * dx));
+    const float Sd1 = __builtin_fmaf(Syz, dz, __builtin_fmaf(Syy, dy, Sxy * dx));
+    const float Sd2 = __builtin_fmaf(Szz, dz, __builtin_fmaf(Syz, dy, Sxz * dx));
+    const float dSd = __builtin_fmaf(dz, Sd2, __builtin_fmaf(dy, Sd1, dx * Sd0));
+    const float T = (Sxx + Syy) + Szz;
+    const float ht = 0.5f * T;
+    const float w = (2.5f * dSd) / r2;
+    QuadTerm t;
+    t.u = (3.0f * G) / (D * r2);
+    t.q0 = __builtin_fmaf(w, dx, -__builtin_fmaf(ht, dx, Sd0));
+    t.q1 = __builtin_fmaf(w, dy, -__builtin_fmaf(ht, dy, Sd1));
+    t.q2 = __builtin_fmaf(w, dz, -__builtin_fmaf(ht, dz, Sd2));
+    t.phi = -(double)G * (1.5 * (double)dSd - 0.5 * (double)T * (double)r2) /
+            ((double)D * (double)r2);
+    return t;
+}
+
+// F += s d (+ u q), with the oracle's order of roundings for both accumulation modes.
+template <bool KAHAN>
+__device__ inline void accumulate_quad(Acc3& F, float s, float dx, float dy, float dz,
+                                       const QuadTerm& t) {
+    if (KAHAN) {
+        float y = __builtin_fmaf(s, dx, __builtin_fmaf(t.u, t.q0, -F.cx));
+        float v = F.x + y;
+        F.cx = (v - F.x) - y;
+        F.x = v;
+        y = __builtin_fmaf(s, dy, __builtin_fmaf(t.u, t.q1, -F.cy));
+        v = F.y + y;
+        F.cy = (v - F.y) - y;
+        F.y = v;
+        y = __builtin_fmaf(s, dz, __builtin_fmaf(t.u, t.q2, -F.cz));
+        v = F.z + y;
+        F.cz = (v - F.z) - y;
+        F.z = v;
+    } else {
+        F.x = __builtin_fmaf(t.u, t.q0, __builtin_fmaf(s, dx, F.x));
+        F.y = __builtin_fmaf(t.u, t.q1, __builtin_fmaf(s, dy, F.y));
+        F.z = __builtin_fmaf(t.u, t.q2, __builtin_fmaf(s, dz, F.z));
+    }
+}
+
 void set_accept_threshold(ForceArgs& a) {
     a.acc = make_accept_rule(a.p.theta);
     a.fast_sqrt = a.p.r2_floor >= 0x1p-96f ? 1u : 0u;
@@ -76,7 +128,8 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 }
 
 // MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64 potential per target.
-template <int NT, bool KAHAN, int MODE>
+// QUAD: accepted internal nodes add their quadrupole term (second moments in a.qmom).
+template <int NT, bool KAHAN, int MODE, bool QUAD>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t depth) {
     extern __shared__ uint32_t s_stack[];  // [depth][NT]
 
@@ -139,8 +192,14 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
             const bool open = !leaf && !acc;
             // the term is computed for every lane and kept only where it interacts
             const float gm = G * cm.w;
-            const float s = (gm / (r2 * rr));
-            if (KAHAN) {
+            const float D = r2 * rr;
+            const float s = (gm / D);
+            if (QUAD && interact && !leaf) {
+                const float4* qp = a.qmom + 2 * (size_t)n;
+                const QuadTerm qt = quad_term(qp[0], qp[1], dx, dy, dz, r2, D, G);
+                accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
+                if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+            } else if (KAHAN) {
                 Acc3 T = F;
                 accumulate<true>(T, s, dx, dy, dz);
                 F.x = interact ? T.x : F.x;
@@ -160,7 +219,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
                 monos += !leaf && interact;
                 opens += open;
             }
-            if (MODE == 2 && interact) phi -= (double)gm / (double)rr;
+            if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
             if (open) {
                 const bool push = top_cnt > 0u;
                 if (push && sp >= depth) {
@@ -201,17 +260,22 @@ static void launch_force_nt(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t depth = a.p.bits_per_axis + 2u;  // open levels <= layers - 1
     const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
     const bool k = a.p.kahan != 0;
-#define ASTRO_FORCE_LAUNCH(KH, MD) \
-    hipLaunchKernelGGL((k_force_compact<NT, KH, MD>), dim3(blocks), dim3(NT), lds, s, a, depth)
-    if (k) {
-        if (mode == 0) ASTRO_FORCE_LAUNCH(true, 0);
-        else if (mode == 1) ASTRO_FORCE_LAUNCH(true, 1);
-        else ASTRO_FORCE_LAUNCH(true, 2);
+#define ASTRO_FORCE_LAUNCH(KH, MD, QD) \
+    hipLaunchKernelGGL((k_force_compact<NT, KH, MD, QD>), dim3(blocks), dim3(NT), lds, s, a, depth)
+#define ASTRO_FORCE_MODES(KH, QD)                    \
+    do {                                             \
+        if (mode == 0) ASTRO_FORCE_LAUNCH(KH, 0, QD); \
+        else if (mode == 1) ASTRO_FORCE_LAUNCH(KH, 1, QD); \
+        else ASTRO_FORCE_LAUNCH(KH, 2, QD);          \
+    } while (0)
+    if (a.p.multipole == 2) {
+        if (k) ASTRO_FORCE_MODES(true, true);
+        else ASTRO_FORCE_MODES(false, true);
     } else {
-        if (mode == 0) ASTRO_FORCE_LAUNCH(false, 0);
-        else if (mode == 1) ASTRO_FORCE_LAUNCH(false, 1);
-        else ASTRO_FORCE_LAUNCH(false, 2);
+        if (k) ASTRO_FORCE_MODES(true, false);
+        else ASTRO_FORCE_MODES(false, false);
     }
+#undef ASTRO_FORCE_MODES
 #undef ASTRO_FORCE_LAUNCH
 }
 
@@ -280,9 +344,20 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
             }
         }
         if (interact) {
-            const float s = ((a.p.G * nd->totalMass) / (r2 * rr));
-            if (a.p.kahan) accumulate<true>(F, s, dx, dy, dz);
-            else accumulate<false>(F, s, dx, dy, dz);
+            const float D = r2 * rr;
+            const float s = ((a.p.G * nd->totalMass) / D);
+            if (a.p.multipole == 2 && layer != 0u) {
+                // second moments sit in emittedColor (xx, yy, zz, xy) / emittedColorCenter
+                const float4 s0 = *reinterpret_cast<const float4*>(nd->emittedColor);
+                const float4 s1 = *reinterpret_cast<const float4*>(nd->emittedColorCenter);
+                const QuadTerm qt = quad_term(s0, s1, dx, dy, dz, r2, D, a.p.G);
+                if (a.p.kahan) accumulate_quad<true>(F, s, dx, dy, dz, qt);
+                else accumulate_quad<false>(F, s, dx, dy, dz, qt);
+            } else if (a.p.kahan) {
+                accumulate<true>(F, s, dx, dy, dz);
+            } else {
+                accumulate<false>(F, s, dx, dy, dz);
+            }
         }
     }
     const float c = a.p.force_clamp;
@@ -376,6 +451,61 @@ void launch_diag(const float4* pos, const float4* vel, const double* phi,
                  hipStream_t s) {
     hipLaunchKernelGGL(k_diag, dim3(chunks, nsys), dim3(256), 0, s, pos, vel, phi, d_sys_off,
                        chunks, partial);
+}
+
+// FP64 direct sum for accuracy audits: a_i = sum_j G m_j d / (|d|^2 + eps2)^(3/2), d = x_j - x_i,
+// over the sources of target i's system (the self term has d = 0 and adds nothing).  Block
+// (chunk, t) sums one chunk of the sources for target t; partial[(t * chunks + chunk) * 3 + k].
+__global__ __launch_bounds__(256) void k_direct_fp64(const float4* __restrict__ pos,
+                                                     const uint32_t* __restrict__ sys_of,
+                                                     const uint32_t* __restrict__ sys_off,
+                                                     uint32_t nt, const uint32_t* __restrict__ targets,
+                                                     uint32_t chunks, double G, double eps2,
+                                                     double* __restrict__ partial) {
+    __shared__ double s_red[3][4];
+    const uint32_t t = blockIdx.y, chunk = blockIdx.x;
+    const uint32_t i = targets[t];
+    uint32_t b = 0, e = nt;
+    if (sys_of) {
+        const uint32_t sys = sys_of[i];
+        b = sys_off[sys];
+        e = sys_off[sys + 1];
+    }
+    const uint32_t len = e - b, per = (len + chunks - 1) / chunks;
+    const uint32_t lo = b + min(len, chunk * per), hi = b + min(len, (chunk + 1) * per);
+    const float4 p = pos[i];
+    const double xi = p.x, yi = p.y, zi = p.z;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += 256) {
+        const float4 q = pos[j];
+        const double dx = (double)q.x - xi, dy = (double)q.y - yi, dz = (double)q.z - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+        const double sc = G * (double)q.w / (r2 * sqrt(r2));
+        ax += sc * dx;
+        ay += sc * dy;
+        az += sc * dz;
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v[3] = {ax, ay, az};
+    for (int k = 0; k < 3; k++) {
+        double x = v[k];
+        for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (lane == 0) s_red[k][wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        partial[((size_t)t * chunks + chunk) * 3 + k] =
+            ((s_red[k][0] + s_red[k][1]) + s_red[k][2]) + s_red[k][3];
+    }
+}
+
+void launch_direct_fp64(const float4* pos, const uint32_t* sys_of, const uint32_t* d_sys_off,
+                        uint32_t nt, const uint32_t* targets, uint32_t n_targets, uint32_t chunks,
+                        double G, double eps2, double* partial, hipStream_t s) {
+    if (!n_targets) return;
+    hipLaunchKernelGGL(k_direct_fp64, dim3(chunks, n_targets), dim3(256), 0, s, pos, sys_of,
+                       d_sys_off, nt, targets, chunks, G, eps2, partial);
 }
 
 }  // namespace astro

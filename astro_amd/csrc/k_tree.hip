@@ -210,16 +210,18 @@ struct Agg64 {
     }
 };
 
+// emittedColor / emittedColorCenter hold the second moments S (xx, yy, zz, xy | xz, yz) in
+// quadrupole mode and stay zero otherwise: the physics reads nothing else from them.
 __device__ inline void write_onode(OctreeNode* o, uint64_t code, const float* cm,
-                                   uint32_t first, uint32_t nchild, uint32_t layer) {
-    // emittedColor / emittedColorCenter stay zero: the physics never reads them.
+                                   uint32_t first, uint32_t nchild, uint32_t layer,
+                                   const float* S = nullptr) {
     float4* q = reinterpret_cast<float4*>(o);
     q[0] = make_float4(__uint_as_float((uint32_t)code), __uint_as_float((uint32_t)(code >> 32)),
                        0.0f, 0.0f);
     q[1] = make_float4(cm[0], cm[1], cm[2], 0.0f);
     q[2] = make_float4(cm[3], 0.0f, 0.0f, 0.0f);
-    q[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    q[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q[3] = S ? make_float4(S[0], S[1], S[2], S[3]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q[4] = S ? make_float4(S[4], S[5], 0.0f, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t ch[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) ch[c] = (uint32_t)c < nchild ? first + c : ASTRO_INVALID_CHILD;
@@ -228,6 +230,47 @@ __device__ inline void write_onode(OctreeNode* o, uint64_t code, const float* cm
     q[6] = make_float4(__uint_as_float(ch[4]), __uint_as_float(ch[5]), __uint_as_float(ch[6]),
                        __uint_as_float(ch[7]));
     q[7] = make_float4(__uint_as_float(layer), 0.0f, 0.0f, 0.0f);
+}
+
+// Second moments about the node's float centre of mass P: with d = x - P,
+// S_e = S_e + fma(m*d_i, d_j, Sc_e) for e = xx, yy, zz, xy, xz, yz (the oracle's order).
+struct Mom32 {
+    float s[6] = {0, 0, 0, 0, 0, 0};
+    __device__ void add(float x, float y, float z, float w, const float* P, const float* Sc) {
+        const float dx = x - P[0], dy = y - P[1], dz = z - P[2];
+        const float mx = w * dx, my = w * dy, mz = w * dz;
+        s[0] = s[0] + __builtin_fmaf(mx, dx, Sc ? Sc[0] : 0.0f);
+        s[1] = s[1] + __builtin_fmaf(my, dy, Sc ? Sc[1] : 0.0f);
+        s[2] = s[2] + __builtin_fmaf(mz, dz, Sc ? Sc[2] : 0.0f);
+        s[3] = s[3] + __builtin_fmaf(mx, dy, Sc ? Sc[3] : 0.0f);
+        s[4] = s[4] + __builtin_fmaf(mx, dz, Sc ? Sc[4] : 0.0f);
+        s[5] = s[5] + __builtin_fmaf(my, dz, Sc ? Sc[5] : 0.0f);
+    }
+    __device__ void fin(float* o) const {
+        for (int e = 0; e < 6; e++) o[e] = s[e];
+    }
+};
+struct Mom64 {
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    __device__ void add(float x, float y, float z, float w, const float* P, const float* Sc) {
+        const double dx = (double)x - (double)P[0], dy = (double)y - (double)P[1],
+                     dz = (double)z - (double)P[2];
+        const double m = w;
+        s[0] = s[0] + __builtin_fma(m * dx, dx, Sc ? (double)Sc[0] : 0.0);
+        s[1] = s[1] + __builtin_fma(m * dy, dy, Sc ? (double)Sc[1] : 0.0);
+        s[2] = s[2] + __builtin_fma(m * dz, dz, Sc ? (double)Sc[2] : 0.0);
+        s[3] = s[3] + __builtin_fma(m * dx, dy, Sc ? (double)Sc[3] : 0.0);
+        s[4] = s[4] + __builtin_fma(m * dx, dz, Sc ? (double)Sc[4] : 0.0);
+        s[5] = s[5] + __builtin_fma(m * dy, dz, Sc ? (double)Sc[5] : 0.0);
+    }
+    __device__ void fin(float* o) const {
+        for (int e = 0; e < 6; e++) o[e] = (float)s[e];
+    }
+};
+
+__device__ inline void store_moments(float4* qm, uint64_t n, const float* S) {
+    qm[2 * n] = make_float4(S[0], S[1], S[2], S[3]);
+    qm[2 * n + 1] = make_float4(S[4], S[5], 0.0f, 0.0f);
 }
 
 __device__ inline uint32_t head_level(uint64_t a, uint64_t b, uint32_t bits) {
@@ -275,7 +318,26 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     t.pad = 0;
     a.tnodes[n] = t;
     a.levels[n] = (uint8_t)h;
-    if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0);
+    float S[6];
+    if (a.quad) {
+        if (sizeof(AGG) == sizeof(Agg64)) {
+            Mom64 q;
+            for (uint32_t i = j; i < end; i++) {
+                float4 x = a.pos[a.sorted_idx[i]];
+                q.add(x.x, x.y, x.z, x.w, cm, nullptr);
+            }
+            q.fin(S);
+        } else {
+            Mom32 q;
+            for (uint32_t i = j; i < end; i++) {
+                float4 x = a.pos[a.sorted_idx[i]];
+                q.add(x.x, x.y, x.z, x.w, cm, nullptr);
+            }
+            q.fin(S);
+        }
+        store_moments(a.qmom, n, S);
+    }
+    if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0, a.quad ? S : nullptr);
     return h;
 }
 
@@ -315,8 +377,33 @@ __device__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint3
     t.pad = 0;
     a.tnodes[n] = t;
     a.levels[n] = (uint8_t)level;
+    float S[6];
+    if (a.quad) {
+        const float4* cq = a.qmom + 2 * (a.off[l - 1] + c);
+        if (sizeof(AGG) == sizeof(Agg64)) {
+            Mom64 q;
+            for (uint32_t i = 0; i < k; i++) {
+                const TNode u = ch[c + i];
+                const float4 q0 = cq[2 * i], q1 = cq[2 * i + 1];
+                const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                q.add(u.x, u.y, u.z, u.m, cm, Sc);
+            }
+            q.fin(S);
+        } else {
+            Mom32 q;
+            for (uint32_t i = 0; i < k; i++) {
+                const TNode u = ch[c + i];
+                const float4 q0 = cq[2 * i], q1 = cq[2 * i + 1];
+                const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                q.add(u.x, u.y, u.z, u.m, cm, Sc);
+            }
+            q.fin(S);
+        }
+        store_moments(a.qmom, n, S);
+    }
     if (a.onodes)
-        write_onode(&a.onodes[n], a.sorted_keys[head] >> (3 * l), cm, t.first, k, l);
+        write_onode(&a.onodes[n], a.sorted_keys[head] >> (3 * l), cm, t.first, k, l,
+                    a.quad ? S : nullptr);
     return level;
 }
 

@@ -65,6 +65,8 @@ struct TreeArgs {
     const uint32_t* sorted_idx;
     TNode* tnodes;
     uint8_t* levels;            // head level of every node (parallel to tnodes)
+    float4* qmom;               // second moments, 2 float4 per node (quadrupole mode)
+    uint32_t quad;              // 1: compute second moments
     OctreeNode* onodes;         // may be null (no emission)
     uint32_t* counts;           // [layers] valid nodes per layer
     uint32_t* leaf_of_sorted;   // [nt]
@@ -122,6 +124,7 @@ void launch_clear_onodes(OctreeNode* nodes, uint64_t count, hipStream_t s);
 // ---- k_force.hip --------------------------------------------------------------------------
 struct ForceArgs {
     const TNode* tnodes;
+    const float4* qmom;         // second moments per node (quadrupole mode)
     const OctreeNode* onodes;
     const float4* pos;
     const uint32_t* type;
@@ -153,5 +156,9 @@ void launch_diag(const float4* pos, const float4* vel, const double* phi,
                  hipStream_t s);
 void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,
                    uint32_t n, float dt, const astro_params& p, hipStream_t s);
+
+void launch_direct_fp64(const float4* pos, const uint32_t* sys_of, const uint32_t* d_sys_off,
+                        uint32_t nt, const uint32_t* targets, uint32_t n_targets, uint32_t chunks,
+                        double G, double eps2, double* partial, hipStream_t s);
 
 }  // namespace astro

@@ -54,6 +54,7 @@ class AstroParams(ctypes.Structure):
         ("quirk_last_leaf", ctypes.c_uint32),
         ("fp64_top_layers", ctypes.c_uint32),
         ("kahan", ctypes.c_uint32),
+        ("multipole", ctypes.c_uint32),
     ]
 
     def copy(self):

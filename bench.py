@@ -39,6 +39,23 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup + steps only, no JSON extras (for rocprofv3 runs)")
+    ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-steps-min", type=int, default=10)
+    ap.add_argument("--serve-steps-max", type=int, default=100)
+    ap.add_argument("--serve-pool", type=int, default=24)
+    ap.add_argument("--serve-capacity", type=int, default=8 << 20)
+    ap.add_argument("--acc-steps", type=int, default=50,
+                    help="timed steps of the accuracy-matched configuration")
+    ap.add_argument("--acc-targets", type=int, default=4096)
+    ap.add_argument("--overload-rate", type=float, default=200.0)
+    ap.add_argument("--overload-seconds", type=float, default=1.5)
+    ap.add_argument("--skip-accuracy", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="profile mode: also copy this many MiB device-to-device (a plain copy "
                          "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -73,9 +90,165 @@ def calibration_copy(nbytes):
         hip.hipFree(p)
 
 
-def cpu_baseline(pos, vel, col, dt, steps, threads):
+def rms_rel(f, ref):
+    d = np.asarray(f, np.float64)[:, :3] - ref
+    return float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))
+
+
+def bench_batched(astro, a):
+    """256 independent Plummer systems, N log-uniform in [1024, 65536]: one batched step per
+    graph replay vs the plain loop (each system in its own context, stepped in turn)."""
+    sizes = astro.ic_batch_sizes(a.batch_systems, 1024, 65536, 7)
+    parts = [astro.ic_plummer(int(n), 1000 + k) for k, n in enumerate(sizes)]
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    col = np.concatenate([p[2] for p in parts])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    ntot = int(off[-1])
+    P = astro.params_physics(21)
+    out = {"systems": len(sizes), "particles": ntot}
+    with astro.Context(ntot, bits_per_axis=21, max_systems=len(sizes),
+                       emit_octree_nodes=False) as ctx:
+        ctx.set_params(P)
+        ctx.upload_systems(pos, vel, col, off)
+        out["autotuned_block"], _ = ctx.autotune(3)
+        ctx.step(a.dt, 2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.dt, a.batch_steps)
+        ctx.sync()
+        tb = time.perf_counter() - t0
+        ctx.check()
+    out["batched"] = {"value": ntot * a.batch_steps / tb, "ms_per_step": tb / a.batch_steps * 1e3,
+                      "steps": a.batch_steps}
+    ctxs = []
+    for p_, v_, c_ in parts:
+        c = astro.Context(len(p_), bits_per_axis=21, emit_octree_nodes=False)
+        c.set_params(P)
+        c.upload(p_, v_, c_)
+        c.step(a.dt, 1)
+        ctxs.append(c)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.plain_steps):
+        for c in ctxs:
+            c.step(a.dt, 1)
+    for c in ctxs:
+        c.sync()
+    tp = time.perf_counter() - t0
+    for c in ctxs:
+        c.check()
+        c.close()
+    out["plain_loop"] = {"value": ntot * a.plain_steps / tp, "ms_per_step": tp / a.plain_steps * 1e3,
+                         "steps": a.plain_steps}
+    out["batched_over_plain"] = out["batched"]["value"] / out["plain_loop"]["value"]
+    # FP32 vs Kahan accumulation against the FP64 direct sum (on the GPU), on a few systems
+    errs = {"fp32": [], "kahan": []}
+    for k in np.linspace(0, len(parts) - 1, 4).astype(int):
+        p_, v_, c_ = parts[k]
+        tg = np.random.default_rng(k).choice(len(p_), min(2048, len(p_)), replace=False)
+        for kahan in (0, 1):
+            Pk = astro.params_physics(21)
+            Pk.kahan = kahan
+            with astro.Context(len(p_), bits_per_axis=21, emit_octree_nodes=False) as c:
+                c.set_params(Pk)
+                c.upload(p_, v_, c_)
+                ref = c.direct_forces_fp64(tg)
+                c.build_tree()
+                c.compute_forces()
+                errs["kahan" if kahan else "fp32"].append(rms_rel(c.forces()[tg], ref))
+    out["rms_force_error_vs_fp64"] = {k: float(np.mean(v)) for k, v in errs.items()}
+    out["rms_systems"] = "4 systems spread over the size range, 2048 random targets each"
+    return out
+
+
+def serving_pool(astro, a):
+    rng = np.random.default_rng(5)
+    pool_sizes = np.exp(rng.uniform(np.log(2048), np.log(1 << 20), a.serve_pool)).astype(int)
+    pool = []
+    for k, n in enumerate(pool_sizes):
+        if k % 2 == 0:
+            pool.append(astro.ic_plummer(int(n), 500 + k))
+        else:
+            pool.append(astro.ic_two_disks(max(1, int(n) // 2), 500 + k))
+    return pool
+
+
+def bench_serving(astro, a, pool, rate, seconds, seed):
+    """Poisson arrivals of Plummer / two-disk systems (N log-uniform in [2048, 2^20]), served
+    with continuous batching; the same request sequence through the plain FIFO loop."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        reqs.append((len(reqs), t, int(rng.integers(0, len(pool))),
+                     int(rng.integers(a.serve_steps_min, a.serve_steps_max + 1))))
+    work = sum(len(pool[k][0]) * st for _, _, k, st in reqs)
+    P = astro.params_physics(21)
+    out = {"requests": len(reqs), "arrival_rate_per_s": rate,
+           "arrival_seconds": seconds,
+           "steps_range": [a.serve_steps_min, a.serve_steps_max],
+           "offered_particle_steps_per_s": work / seconds,
+           "pool": "%d seeded systems, N log-uniform in [2048, 1048576], Plummer and two-disk "
+                   "alternating" % a.serve_pool}
+    srv = astro.Server(P, a.serve_capacity, 512)
+    pending = list(reqs)
+    lat = []
+    t0 = time.perf_counter()
+    while pending or srv.counters()["active"] or srv.counters()["queued"]:
+        now = time.perf_counter() - t0
+        while pending and pending[0][1] <= now:
+            rid, _, k, st = pending.pop(0)
+            srv.submit(rid, pool[k][0], pool[k][1], pool[k][2], st)
+        if srv.counters()["active"] or srv.counters()["queued"]:
+            srv.tick(a.dt)
+        else:
+            time.sleep(0.0005)
+        for rec in srv.poll():
+            lat.append(rec["t_done"] - rec["t_submit"])
+            srv.fetch(rec["id"], rec["n"])
+    el = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(lat)
+    out["served"] = {"value": work / el, "seconds": el, "p50_latency_s": float(np.percentile(lat, 50)),
+                     "p99_latency_s": float(np.percentile(lat, 99)), "completed": len(lat)}
+    # plain loop: one request at a time, FIFO, each in a single-system context
+    ctx = astro.Context(int(max(len(p[0]) for p in pool)), bits_per_axis=21, emit_octree_nodes=False)
+    ctx.set_params(P)
+    pending = list(reqs)
+    lat = []
+    t0 = time.perf_counter()
+    while pending:
+        now = time.perf_counter() - t0
+        if pending[0][1] > now:
+            time.sleep(pending[0][1] - now)
+        rid, tarr, k, st = pending.pop(0)
+        ctx.upload(*pool[k])
+        ctx.step(a.dt, st)
+        ctx.sync()
+        ctx.download()
+        lat.append(time.perf_counter() - t0 - tarr)
+    el = time.perf_counter() - t0
+    ctx.check()
+    ctx.close()
+    lat = np.array(lat)
+    out["plain_loop"] = {"value": work / el, "seconds": el,
+                         "p50_latency_s": float(np.percentile(lat, 50)),
+                         "p99_latency_s": float(np.percentile(lat, 99))}
+    out["served_over_plain"] = out["served"]["value"] / out["plain_loop"]["value"]
+    return out
+
+
+def cpu_baseline(pos, vel, col, dt, steps, threads, theta=None, multipole=0):
     from oracle import oracle
     P = oracle.params_physics(21)
+    if theta is not None:
+        P.theta = theta
+    P.multipole = multipole
     p, v = pos, vel
     oracle.step(p, v, col, P, dt, threads=threads)  # warm-up step
     t0 = time.perf_counter()
@@ -83,6 +256,55 @@ def cpu_baseline(pos, vel, col, dt, steps, threads):
         p, v, _, _ = oracle.step(p, v, col, P, dt, threads=threads)
     el = time.perf_counter() - t0
     return len(pos) * steps / el, el
+
+
+def accuracy(astro, ctx, pos, vel, col, P, a):
+    """RMS relative force error against the FP64 direct sum (on the GPU) on a random target
+    subset of the initial state, for the headline configuration (theta 0.7, monopoles) and for
+    the accuracy-matched one (theta 0.5 with quadrupoles), and the step rate of the latter."""
+    n = len(pos)
+    tg = np.sort(np.random.default_rng(1).choice(n, a.acc_targets, replace=False))
+    ctx.upload(pos, vel, col)
+    ref = ctx.direct_forces_fp64(tg)
+    out = {"targets": int(a.acc_targets), "reference": "FP64 direct sum over all 500k "
+           "particles (astro_direct_forces_fp64), initial state"}
+    rows = {}
+    for name, theta, mp, kahan in (("theta0.7_monopole", 0.7, 0, 0),
+                                   ("theta0.5_monopole", 0.5, 0, 0),
+                                   ("theta0.7_quadrupole", 0.7, 2, 0),
+                                   ("theta0.5_quadrupole", 0.5, 2, 0),
+                                   ("theta0.5_quadrupole_kahan", 0.5, 2, 1)):
+        Q = P.copy()
+        Q.theta, Q.multipole, Q.kahan = theta, mp, kahan
+        ctx.set_params(Q)
+        ctx.upload(pos, vel, col)
+        ctx.build_tree()
+        ctx.compute_forces()
+        ctx.check()
+        rows[name] = rms_rel(ctx.forces()[tg], ref)
+    out["rms_rel_force_error"] = rows
+    # the accuracy-matched configuration, timed like the headline one
+    Q = P.copy()
+    Q.theta, Q.multipole = 0.5, 2
+    ctx.set_params(Q)
+    ctx.upload(pos, vel, col)
+    ctx.build_tree()
+    st = ctx.traversal_stats()
+    ctx.step_timed(a.dt, 2)
+    ctx.sync()
+    t0 = time.perf_counter()
+    stage_ms = ctx.step_timed(a.dt, a.acc_steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.check()
+    out["matched"] = {"config": "theta 0.5, quadrupoles, otherwise the headline workload",
+                      "rms_rel_force_error": rows["theta0.5_quadrupole"],
+                      "value": n * a.acc_steps / wall, "unit": "particle-steps/s",
+                      "ms_per_step": wall / a.acc_steps * 1e3, "steps": a.acc_steps,
+                      "stage_ms": {k: round(v / a.acc_steps, 4) for k, v in stage_ms.items()},
+                      "visits_per_particle": st["visits"] / n}
+    ctx.set_params(P)
+    return out
 
 
 def main():
@@ -189,6 +411,16 @@ def main():
         "without_octree_nodes": {"value": n * K / wall_noemit,
                                  "ms_per_step": wall_noemit / K * 1e3},
     }
+    if not a.skip_accuracy:
+        out["accuracy"] = accuracy(astro_amd, ctx, pos, vel, col, P, a)
+    ctx.close()
+    if not a.skip_batched:
+        out["batched"] = bench_batched(astro_amd, a)
+    if not a.skip_serving:
+        pool = serving_pool(astro_amd, a)
+        out["serving"] = bench_serving(astro_amd, a, pool, a.serve_rate, a.serve_seconds, 5)
+        out["serving_overload"] = bench_serving(astro_amd, a, pool, a.overload_rate,
+                                                a.overload_seconds, 6)
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         cv, el = cpu_baseline(pos, vel, col, a.dt, a.cpu_steps, threads)
@@ -199,6 +431,13 @@ def main():
                                          f"build single-threaded as transcribed, force loop on "
                                          f"{threads} OpenMP threads"}
         out["gpu_vs_cpu"] = value / cv
+        if "accuracy" in out:
+            cq, elq = cpu_baseline(pos, vel, col, a.dt, max(1, a.cpu_steps - 1), threads,
+                                   theta=0.5, multipole=2)
+            out["cpu_baseline"]["matched"] = {
+                "value": cq, "sample": f"{max(1, a.cpu_steps - 1)} oracle steps at theta 0.5 "
+                                       f"with quadrupoles after one warm-up step ({elq:.1f} s)"}
+            out["accuracy"]["matched"]["gpu_vs_cpu"] = out["accuracy"]["matched"]["value"] / cq
     print(json.dumps(out))
 
 

@@ -168,9 +168,16 @@ int astro_set_force_block(astro_ctx* ctx, uint32_t block);
  * keeps the fastest; returns it and its time. */
 int astro_autotune(astro_ctx* ctx, uint32_t reps, uint32_t* chosen, float* best_ms);
 /* Builds the tree and returns, per system, 9 FP64 sums: mass, kinetic energy, potential
- * energy (1/2 sum m phi, phi from the same monopole interactions as the forces), momentum
- * x/y/z and mass-weighted position x/y/z. */
+ * energy (1/2 sum m phi, phi from the same interactions as the forces, quadrupole terms
+ * included when astro_params.multipole is 2), momentum x/y/z and mass-weighted position
+ * x/y/z. */
 int astro_diagnostics(astro_ctx* ctx, double* out);
+/* FP64 direct-sum accelerations (3 doubles per target) of the given particles over all tree
+ * particles of their system, with the context's G and eps2: the accuracy audit for the tree
+ * forces (the reference has no counterpart; its README states no error bound).  Synchronous;
+ * allocates temporaries. */
+int astro_direct_forces_fp64(astro_ctx* ctx, const uint32_t* targets, uint32_t n_targets,
+                             double* out_xyz);
 /* Device memory the context allocated at creation. */
 int astro_device_bytes(const astro_ctx* ctx, uint64_t* bytes);
 

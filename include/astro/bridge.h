@@ -100,6 +100,8 @@ typedef struct {
     uint32_t quirk_last_leaf; /* 1: last leaf run ends at numUnique (Octree.metal:71)        */
     uint32_t fp64_top_layers; /* aggregate the top k layers' mass/COM in FP64                */
     uint32_t kahan;           /* 1: Kahan-compensated force accumulation                     */
+    uint32_t multipole;       /* 0: monopoles (the reference); 2: + quadrupoles of accepted    *
+                               * internal nodes (second moments kept per node)                  */
 } astro_params;
 
 #ifdef __cplusplus

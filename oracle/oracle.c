@@ -42,6 +42,7 @@ void orc_params_compat(astro_params* p) {
     p->quirk_last_leaf = 1;
     p->fp64_top_layers = 0;
     p->kahan = 0;
+    p->multipole = 0;
 }
 
 /* The scaled profile for self-gravitating systems (G=1, exact cells, no cut-offs). */
@@ -273,6 +274,54 @@ static void agg_final(const agg_t* a, float com[3], float* mass) {
     }
 }
 
+/* Second moments S = sum m d d^T about the node's (float) centre of mass P, from terms at x
+ * with mass m and their own moments Sc (zero for particles): with d = x - P,
+ * S_ij = S_ij + fma(m*d_i, d_j, Sc_ij), in the order xx, yy, zz, xy, xz, yz.  FP64 layers do
+ * the same in double and round once.  S is kept in the node's emittedColor (xx, yy, zz, xy)
+ * and emittedColorCenter (xz, yz) slots, which the physics otherwise never reads. */
+static const int SI[6] = {0, 1, 2, 0, 0, 1}, SJ[6] = {0, 1, 2, 1, 2, 2};
+typedef struct {
+    int fp64;
+    float s[6];
+    double sd[6];
+} mom_t;
+static void mom_init(mom_t* q, int fp64) {
+    memset(q, 0, sizeof(*q));
+    q->fp64 = fp64;
+}
+static void mom_add(mom_t* q, const float x[3], float m, const float P[3], const float Sc[6]) {
+    if (q->fp64) {
+        double d[3];
+        for (int k = 0; k < 3; k++) d[k] = (double)x[k] - (double)P[k];
+        for (int e = 0; e < 6; e++)
+            q->sd[e] = q->sd[e] + fma((double)m * d[SI[e]], d[SJ[e]], Sc ? (double)Sc[e] : 0.0);
+    } else {
+        float d[3], md[3];
+        for (int k = 0; k < 3; k++) d[k] = x[k] - P[k];
+        for (int k = 0; k < 3; k++) md[k] = m * d[k];
+        for (int e = 0; e < 6; e++)
+            q->s[e] = q->s[e] + fmaf(md[SI[e]], d[SJ[e]], Sc ? Sc[e] : 0.0f);
+    }
+}
+static void mom_store(const mom_t* q, OctreeNode* nd) {
+    float s[6];
+    for (int e = 0; e < 6; e++) s[e] = q->fp64 ? (float)q->sd[e] : q->s[e];
+    nd->emittedColor[0] = s[0];
+    nd->emittedColor[1] = s[1];
+    nd->emittedColor[2] = s[2];
+    nd->emittedColor[3] = s[3];
+    nd->emittedColorCenter[0] = s[4];
+    nd->emittedColorCenter[1] = s[5];
+}
+static void mom_load(const OctreeNode* nd, float s[6]) {
+    s[0] = nd->emittedColor[0];
+    s[1] = nd->emittedColor[1];
+    s[2] = nd->emittedColor[2];
+    s[3] = nd->emittedColor[3];
+    s[4] = nd->emittedColorCenter[0];
+    s[5] = nd->emittedColorCenter[1];
+}
+
 /*
  * Renderer.swift:511-534 + 714-755 -- the layered leaf-to-root build:
  *   generateMortonCodes, fillIndices, sort, countUnique, aggregateLeafNodes, then for each
@@ -281,7 +330,8 @@ static void agg_final(const agg_t* a, float com[3], float* mass) {
  * nodes must hold L->total entries.  codes receives the per-particle (unsorted) keys.
  * resort_layers=1 runs the per-layer identity sorts as the reference does and returns -2 if
  * any of them is not the identity (they never are: see the aggregateNodes note below).
- * emittedColor/emittedColorCenter are left zero: the physics never reads them.
+ * emittedColor/emittedColorCenter are left zero (the physics never reads them), or hold the
+ * node's second moments when p->multipole == 2.
  */
 int orc_build_tree(const PositionMass* pos, uint32_t nt, const astro_params* p,
                    const float cube[4], int resort_layers, OctreeNode* nodes, orc_layers* L,
@@ -320,6 +370,15 @@ int orc_build_tree(const PositionMass* pos, uint32_t nt, const astro_params* p,
         }
         OctreeNode* nd = &nodes[L->off[0] + g];
         agg_final(&a, nd->centerOfMass, &nd->totalMass);
+        if (p->multipole == 2) {
+            mom_t q;
+            mom_init(&q, 0 >= fp64_from);
+            for (uint32_t i = start; i < end; i++) {
+                const PositionMass* s = &pos[sorted_idx[i]];
+                mom_add(&q, s->position, s->mass, nd->centerOfMass, NULL);
+            }
+            mom_store(&q, nd);
+        }
         nd->mortonCode = sorted_codes[start];
         nd->layer = 0;
         layer_codes[g] = sorted_codes[start];
@@ -354,6 +413,17 @@ int orc_build_tree(const PositionMass* pos, uint32_t nt, const astro_params* p,
                 if (nc < 8) nd->children[nc++] = (uint32_t)ci;
             }
             agg_final(&a, nd->centerOfMass, &nd->totalMass);
+            if (p->multipole == 2) {
+                mom_t q;
+                mom_init(&q, l >= fp64_from);
+                for (uint32_t i = start; i < end; i++) {
+                    const OctreeNode* c = &nodes[L->off[l - 1] + i];
+                    float Sc[6];
+                    mom_load(c, Sc);
+                    mom_add(&q, c->centerOfMass, c->totalMass, nd->centerOfMass, Sc);
+                }
+                mom_store(&q, nd);
+            }
             nd->mortonCode = sk[start] >> 3;
             nd->layer = l;
             layer_codes[g] = nd->mortonCode;
@@ -442,19 +512,37 @@ void orc_barnes_hut(const OctreeNode* nodes, uint64_t root, const PositionMass* 
             }
             if (interact) {
                 float gm = p->G * nd->totalMass;
-                float s = gm / (r2 * r);
+                float D = r2 * r;
+                float s = gm / D;
+                float q[3] = {0.0f, 0.0f, 0.0f}, u = 0.0f;
+                int quad = p->multipole == 2 && !leaf;
+                if (quad) {
+                    /* a_Q = 3G / (D r2) * (w d - (S d + T/2 d)),  w = 2.5 (d.S.d) / r2 */
+                    float S[6];
+                    mom_load(nd, S);
+                    float Sd0 = fmaf(S[4], dz, fmaf(S[3], dy, S[0] * dx));
+                    float Sd1 = fmaf(S[5], dz, fmaf(S[1], dy, S[3] * dx));
+                    float Sd2 = fmaf(S[2], dz, fmaf(S[5], dy, S[4] * dx));
+                    float dSd = fmaf(dz, Sd2, fmaf(dy, Sd1, dx * Sd0));
+                    float T = (S[0] + S[1]) + S[2];
+                    float ht = 0.5f * T;
+                    float w = (2.5f * dSd) / r2;
+                    u = (3.0f * p->G) / (D * r2);
+                    q[0] = fmaf(w, dx, -fmaf(ht, dx, Sd0));
+                    q[1] = fmaf(w, dy, -fmaf(ht, dy, Sd1));
+                    q[2] = fmaf(w, dz, -fmaf(ht, dz, Sd2));
+                }
+                float d[3] = {dx, dy, dz};
                 if (p->kahan) {
-                    float d[3] = {dx, dy, dz};
                     for (int k = 0; k < 3; k++) {
-                        float y = fmaf(s, d[k], -C[k]);
+                        float y = quad ? fmaf(s, d[k], fmaf(u, q[k], -C[k])) : fmaf(s, d[k], -C[k]);
                         float t = F[k] + y;
                         C[k] = (t - F[k]) - y;
                         F[k] = t;
                     }
                 } else {
-                    F[0] = fmaf(s, dx, F[0]);
-                    F[1] = fmaf(s, dy, F[1]);
-                    F[2] = fmaf(s, dz, F[2]);
+                    for (int k = 0; k < 3; k++)
+                        F[k] = quad ? fmaf(u, q[k], fmaf(s, d[k], F[k])) : fmaf(s, d[k], F[k]);
                 }
             }
         }

@@ -24,12 +24,13 @@ def make_batch(astro, sizes, seed0=100):
     return parts, pos, vel, col, off
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batched_equals_alone_and_oracle(astro, orc, graph):
-    say("batched vs alone, graph", graph)
+@pytest.mark.parametrize("graph,multipole", [(True, 0), (False, 0), (True, 2)])
+def test_batched_equals_alone_and_oracle(astro, orc, graph, multipole):
+    say("batched vs alone, graph", graph, "multipole", multipole)
     sizes = [1024, 3000, 1500, 7777, 2048, 1]
     parts, pos, vel, col, off = make_batch(astro, sizes)
     P = astro.params_physics(21)
+    P.multipole = multipole
     steps = 3
     with astro.Context(len(pos), bits_per_axis=21, max_systems=8) as ctx:
         ctx.set_params(P)
@@ -39,6 +40,7 @@ def test_batched_equals_alone_and_oracle(astro, orc, graph):
         ctx.check()
         pb, vb, _ = ctx.download()
     Po = orc.params_physics(21)
+    Po.multipole = multipole
     with astro.Context(max(sizes), bits_per_axis=21) as one:
         one.set_params(P)
         for k, (p, v, c) in enumerate(parts):
@@ -187,3 +189,22 @@ def test_energy_drift_small(astro):
     E0, E1 = d0[1] + d0[2], d1[1] + d1[2]
     assert abs(E1 - E0) < 1e-3 * abs(E0)
     assert np.all(np.abs(d1[3:6] - d0[3:6]) < 1e-4)
+
+
+def test_direct_fp64_matches_oracle(astro, orc):
+    # the accuracy-audit direct sum: per target over its own system's particles, in FP64
+    say("FP64 direct sum")
+    sizes = [3000, 5000, 1]
+    parts, pos, vel, col, off = make_batch(astro, sizes, seed0=300)
+    tg = np.array([0, 17, 2999, 3000, 4567, 7999, 8000], np.uint32)
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=4) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload_systems(pos, vel, col, off)
+        got = ctx.direct_forces_fp64(tg)
+        with pytest.raises(astro.AstroError):
+            ctx.direct_forces_fp64([len(pos)])
+    eps2 = float(np.float32(0.01))
+    for t, g in zip(tg, got):
+        k = int(np.searchsorted(off, t, side="right") - 1)
+        want = orc.direct_sum_fp64(parts[k][0], [t - off[k]], 1.0, eps2)[0]
+        assert np.abs(g - want).max() <= 1e-12 * max(np.linalg.norm(want), 1e-300), (t, g, want)

@@ -258,14 +258,18 @@ def test_radix_sort_generic(astro, small_ctx, kind):
     assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), idx[:count][o])
 
 
-def test_two_disks_500k(astro, orc):
+@pytest.mark.parametrize("multipole", [0, 2])
+def test_two_disks_500k(astro, orc, multipole):
     # the 500,000-particle two-disk system at 63-bit keys: whole tree bit-exact, forces on a
     # 4,096-target subset bit-exact, sortedness and permutation at full size
-    say("two disks 500k")
+    say("two disks 500k, multipole", multipole)
     pos, vel, col = astro.ic_two_disks(250000, 2024)
     n = len(pos)
     with astro.Context(n, bits_per_axis=21, emit_octree_nodes=True) as ctx:
         P, Po = astro.params_physics(21), orc.params_physics(21)
+        P.multipole = Po.multipole = multipole
+        if multipole:
+            P.theta = Po.theta = 0.5
         ctx.set_params(P)
         ctx.upload(pos, vel, col)
         ctx.build_tree()
@@ -293,3 +297,79 @@ def test_two_disks_500k(astro, orc):
 def test_astro_check_reports_clean(astro, small_ctx):
     say("error word clean")
     small_ctx.check()
+
+
+# ------------------------------------------------- quadrupole option (multipole = 2)
+QUAD_CASES = [
+    # name, bits, n, fp64_top_layers, kahan
+    ("B21_16384", 21, 16384, 4, 0),
+    ("B21_16384_fp32", 21, 16384, 0, 0),
+    ("B7_16384_fp64", 7, 16384, 99, 0),
+    ("B21_16384_kahan", 21, 16384, 4, 1),
+    ("B5_3001", 5, 3001, 4, 0),
+]
+
+
+def quad_params(astro, orc, bits, fp64, kahan, theta=0.5):
+    P, Po = astro.params_physics(bits), orc.params_physics(bits)
+    for q in (P, Po):
+        q.multipole, q.fp64_top_layers, q.kahan, q.theta = 2, fp64, kahan, theta
+    return P, Po
+
+
+@pytest.mark.parametrize("case", QUAD_CASES, ids=[c[0] for c in QUAD_CASES])
+def test_quadrupole_bitexact(astro, orc, small_ctx, case):
+    # second moments (in the emitted nodes' emittedColor/emittedColorCenter slots) and the
+    # quadrupole forces equal the oracle's bit for bit
+    name, bits, n, fp64, kahan = case
+    say("quadrupole", name)
+    ctx = small_ctx
+    pos, vel, col = plummer(astro, n, 9)
+    P, Po = quad_params(astro, orc, bits, fp64, kahan)
+    ctx.set_params(P)
+    ctx.set_emit_octree_nodes(True)
+    ctx.upload(pos, vel, col)
+    ctx.build_tree()
+    ctx.compute_forces()
+    ctx.check()
+    t = orc.build_tree(pos, Po)
+    got = ctx.octree_nodes()
+    assert got.tobytes() == t["nodes"].tobytes()
+    assert got["emittedColor"].any()
+    want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
+    f = ctx.forces()
+    e = rel_err(f, want)
+    assert e.max() <= FORCE_RTOL, f"max rel err {e.max():.3e}"
+    assert np.array_equal(f[:, :3], want[:, :3])
+
+
+def test_quadrupole_steps_and_back_to_monopole(astro, orc, small_ctx):
+    say("quadrupole 5 graph steps, then monopole")
+    ctx = small_ctx
+    pos, vel, col = plummer(astro, 20000, 13)
+    P, Po = quad_params(astro, orc, 21, 4, 0)
+    ctx.set_params(P)
+    ctx.set_use_graph(True)
+    ctx.upload(pos, vel, col)
+    ctx.step(1e-3, 5)
+    ctx.check()
+    p2, v2, _ = ctx.download()
+    p, v = pos, vel
+    for _ in range(5):
+        p, v, _, _ = orc.step(p, v, col, Po, 1e-3, threads=8)
+    assert np.array_equal(p2["position"], p["position"])
+    assert np.array_equal(v2["velocity"], v["velocity"])
+    # the stage traversal reads no second moments and refuses the option
+    with pytest.raises(astro.AstroError):
+        ctx.barnes_hut()
+    # switching back restores the reference's monopole tree (zero emitted slots)
+    P.multipole = Po.multipole = 0
+    ctx.set_params(P)
+    ctx.set_emit_octree_nodes(True)
+    ctx.upload(pos, vel, col)
+    ctx.build_tree()
+    ctx.compute_forces()
+    t = orc.build_tree(pos, Po)
+    assert ctx.octree_nodes().tobytes() == t["nodes"].tobytes()
+    want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
+    assert np.array_equal(ctx.forces()[:, :3], want[:, :3])

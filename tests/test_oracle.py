@@ -368,3 +368,114 @@ def test_step_conserves_momentum_roughly(orc, astro):
     e1 = orc.energy_momentum(p, v)
     assert np.all(np.abs(e1[1:4] - e0[1:4]) < 1e-4)
     assert abs(e1[0] - e0[0]) < 1e-2 * e0[0]
+
+
+# ------------------------------------------------- quadrupole option (multipole = 2)
+def node_moments_fp64(t, pos, layer, g):
+    """Second moments of node g of a layer about its stored centre of mass, from its particles."""
+    node = t["nodes"][t["off"][layer] + g]
+    sc = t["sorted_codes"] >> np.uint64(3 * layer)
+    members = t["sorted_idx"][sc == node["mortonCode"]]
+    x = pos["position"][members].astype(np.float64)
+    m = pos["mass"][members].astype(np.float64)
+    d = x - node["centerOfMass"][:3].astype(np.float64)
+    S = (m[:, None, None] * d[:, :, None] * d[:, None, :]).sum(0)
+    got = np.zeros((3, 3))
+    e, c = node["emittedColor"], node["emittedColorCenter"]
+    got[0, 0], got[1, 1], got[2, 2] = e[0], e[1], e[2]
+    got[0, 1] = got[1, 0] = e[3]
+    got[0, 2] = got[2, 0] = c[0]
+    got[1, 2] = got[2, 1] = c[1]
+    return got, S, len(members)
+
+
+@pytest.mark.parametrize("bits,fp64_top", [(21, 0), (21, 99), (5, 0)])
+def test_quadrupole_moments_brute_force(orc, bits, fp64_top):
+    # S = sum m (x - P)(x - P)^T over each node's particles, P the node's stored COM
+    # (the parallel-axis recursion over children must reproduce it); a swapped index or a
+    # dropped child term fails.
+    rng = np.random.default_rng(21)
+    xyz = rng.normal(0, 1, (3000, 3)) * [1.0, 0.5, 0.2]
+    xyz = xyz @ np.linalg.qr(rng.normal(size=(3, 3)))[0]  # general orientation: S has xz, yz
+    pos = particles(xyz.astype(np.float32), rng.uniform(0.5, 1.5, 3000))
+    P = orc.params_physics(bits)
+    P.multipole = 2
+    P.fp64_top_layers = fp64_top
+    t = orc.build_tree(pos, P)
+    checked = 0
+    for layer in sorted({0, 1, 2} | set(range(max(bits - 9, 0), bits + 1))):
+        cnt = t["count"][layer]
+        for g in np.unique(np.linspace(0, cnt - 1, 12).astype(int)):
+            got, S, k = node_moments_fp64(t, pos, layer, g)
+            if k == 1:  # a single particle sits at its own COM, up to the COM's rounding
+                x = np.abs(t["nodes"][t["off"][layer] + g]["centerOfMass"][:3]).max()
+                assert np.abs(got).max() <= 2.0 * (2e-7 * x) ** 2
+                continue
+            tr = np.trace(S)
+            assert np.abs(got - S).max() <= 2e-5 * tr + 1e-30, (layer, g, got, S)
+            checked += 1
+    assert checked >= 30
+    # monopole mode keeps the slots zero
+    P.multipole = 0
+    t0 = orc.build_tree(pos, P)
+    assert not t0["nodes"]["emittedColor"].any() and not t0["nodes"]["emittedColorCenter"].any()
+
+
+@pytest.mark.parametrize("axis", ["along", "across"])
+def test_quadrupole_pair_closed_form(orc, axis):
+    # two masses m at distance +-a from their centre, target at distance R (no softening):
+    # the truncated expansion is exactly 2m/R^2 (1 + 3 a^2/R^2) along the pair's axis and
+    # 2m/R^2 (1 - 3/2 a^2/R^2) across it (the latter fixes the trace term's sign).
+    R, a, m = 20.0, 1.0, 0.5
+    xyz = [[0, 0, 0], [R - a, 0, 0], [R + a, 0, 0]] if axis == "along" else \
+          [[0, 0, 0], [-a, R, 0], [a, R, 0]]
+    pos = particles(xyz, [0.0, m, m])  # a massless target: the root is exactly the pair
+    col = np.zeros(3, COLOR_TYPE)
+    P = orc.params_physics(21)
+    P.theta, P.eps2, P.multipole = 1e30, 0.0, 2
+    t = orc.build_tree(pos, P)
+    f, st = orc.barnes_hut(t, pos, col, P, targets=(0, 1))
+    assert st["monopole_terms"] == 1 and st["visits"] == 1
+    k = 0 if axis == "along" else 1
+    want = 2 * m / R ** 2 * (1 + 3 * a * a / R ** 2 if axis == "along" else 1 - 1.5 * a * a / R ** 2)
+    assert abs(f[0, k] - want) <= 2e-6 * want
+    assert f[0, 1 - k] == 0 and f[0, 2] == 0
+
+
+def test_quadrupole_beats_monopole_on_tilted_cluster(orc):
+    # an anisotropic, tilted cluster seen from a general direction: the quadrupole error is a
+    # higher order in size/distance, so it must be far below the monopole error; a wrong
+    # off-diagonal moment would leave an error of the monopole's order
+    rng = np.random.default_rng(23)
+    cl = rng.normal(0, 1, (200, 3)) * [2.0, 0.6, 0.3]
+    cl = cl @ np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    d = np.array([0.6, -0.5, 0.62])
+    xyz = np.vstack([[[0, 0, 0]], cl + 60 * d / np.linalg.norm(d)])
+    pos = particles(xyz.astype(np.float32), np.r_[0.0, np.full(200, 0.005)])
+    col = np.zeros(201, COLOR_TYPE)
+    exact = orc.direct_sum_fp64(pos, [0], 1.0, float(np.float32(0.01)))[0]
+    err = {}
+    for mp in (0, 2):
+        P = orc.params_physics(21)
+        P.theta, P.multipole = 1e30, mp
+        t = orc.build_tree(pos, P)
+        f, st = orc.barnes_hut(t, pos, col, P, targets=(0, 1))
+        assert st["visits"] == 1
+        err[mp] = np.linalg.norm(f[0, :3] - exact) / np.linalg.norm(exact)
+    assert err[0] > 1e-4 and err[2] < err[0] / 30, err
+
+
+def test_quadrupole_reaches_1e3_rms_at_theta_half(orc, astro):
+    pos, vel, col = astro.ic_plummer(16384, 42)
+    tg = np.arange(0, 16384, 8)
+    ds = orc.direct_sum_fp64(pos, tg, 1.0, float(np.float32(0.01)), threads=8)
+    rms = {}
+    for mp, kahan in ((0, 0), (2, 0), (2, 1)):
+        P = orc.params_physics(21)
+        P.theta, P.multipole, P.kahan = 0.5, mp, kahan
+        t = orc.build_tree(pos, P)
+        f, _ = orc.barnes_hut(t, pos, col, P, threads=8)
+        d = f[tg, :3] - ds
+        rms[(mp, kahan)] = np.sqrt((d ** 2).sum() / (ds ** 2).sum())
+    assert rms[(2, 0)] < 1e-3 and rms[(0, 0)] > 3 * rms[(2, 0)], rms
+    assert abs(rms[(2, 1)] - rms[(2, 0)]) < 0.05 * rms[(2, 0)]
