@@ -15,8 +15,7 @@ struct alignas(16) TNode {
     float x, y, z, m;   // centre of mass, total mass
     uint32_t first;     // absolute index of the first child (leaves: ASTRO_INVALID_CHILD)
     uint32_t info;      // layer | nchild << 5 | head level << 9
-    uint32_t head;      // sorted position of the node's first particle
-    uint32_t pad;
+    uint64_t key;       // full Morton key of the node's first particle (node key = key >> 3 layer)
 };
 static_assert(sizeof(TNode) == 32, "TNode must be 32 B");
 
@@ -77,6 +76,38 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint3
     *total = tmp[NW];
     __syncthreads();
     return r;
+}
+
+// One pass, two results: the block-wide sum of r, and the exclusive scan (and total) of v.
+template <int NT>
+__device__ inline uint32_t block_sum_and_scan(uint32_t r, uint32_t v, uint32_t* tmp,
+                                              uint32_t* rsum, uint32_t* total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    const uint32_t rinc = wave_inclusive_scan(r);
+    if (lane == 63) {
+        tmp[wave] = inc;
+        tmp[NW + 1 + wave] = rinc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0, racc = 0;
+        for (int w = 0; w < NW; w++) {
+            const uint32_t t = tmp[w];
+            tmp[w] = acc;
+            acc += t;
+            racc += tmp[NW + 1 + w];
+        }
+        tmp[NW] = acc;
+        tmp[2 * NW + 1] = racc;
+    }
+    __syncthreads();
+    const uint32_t out = tmp[wave] + inc - v;
+    *total = tmp[NW];
+    *rsum = tmp[2 * NW + 1];
+    __syncthreads();
+    return out;
 }
 
 // Float <-> order-preserving unsigned key, for atomic min/max of floats.

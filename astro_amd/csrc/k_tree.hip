@@ -2,10 +2,11 @@
 //
 // The reference builds each layer with clears, an identity re-sort, mark/scan/scatter of
 // segment heads, a count copy and an aggregation dispatch (Renderer.swift:570-755).  Here a
-// layer is a pair of kernels: head flags (a node starts a parent when its key differs from its
-// predecessor above bit 3l) counted per tile, then a tile scan for the parent index and the
-// head thread aggregating its (at most 8) contiguous children in the reference's serial order.
-// Aggregation order and arithmetic match the oracle exactly.
+// layer is one kernel (plus a count kernel for the leaf layer): a node starts a parent when its
+// key differs from its predecessor's above bit 3l (its "head level" byte is >= l); a tile
+// prefix and scan give the parent index, and the head thread aggregates its (at most 8)
+// contiguous children in the reference's serial order.  Aggregation order and arithmetic match
+// the oracle exactly.
 #include "launch.hpp"
 
 namespace astro {
@@ -289,7 +290,7 @@ __device__ inline bool is_leaf_head(const TreeArgs& a, uint32_t j) {
     return a.sorted_keys[j] != a.sorted_keys[j - 1];
 }
 
-template <typename AGG>
+template <typename AGG, bool QUAD>
 __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     uint32_t end = j + 1;
     while (end < a.nt && !is_leaf_head(a, end)) end++;
@@ -314,12 +315,11 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     t.m = cm[3];
     t.first = ASTRO_INVALID_CHILD;
     t.info = tn_info(0, 0, h);
-    t.head = j;
-    t.pad = 0;
+    t.key = key;
     a.tnodes[n] = t;
     a.levels[n] = (uint8_t)h;
     float S[6];
-    if (a.quad) {
+    if (QUAD) {
         if (sizeof(AGG) == sizeof(Agg64)) {
             Mom64 q;
             for (uint32_t i = j; i < end; i++) {
@@ -337,83 +337,18 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
         }
         store_moments(a.qmom, n, S);
     }
-    if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0, a.quad ? S : nullptr);
+    if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0, QUAD ? S : nullptr);
     return h;
 }
 
-// -------------------------------------------------------------------------- internal layer
-// Octree.metal:282-334 + 120-202 fused for layer l: child c (of layer l-1) starts a parent
-// when its head level is >= l; the parent aggregates children c, c+1, ... up to the next
-// head, in order, and records them as a contiguous (first, count) run.
-template <typename AGG>
-__device__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, uint32_t c, uint32_t p,
-                                uint32_t nchild_layer) {
-    const TNode* ch = a.tnodes + a.off[l - 1];
-    AGG g;
-    uint32_t k = 0;
-    uint32_t head = 0, level = 0;
-    while (c + k < nchild_layer) {
-        const TNode t = ch[c + k];
-        if (k > 0 && tn_level(t.info) >= l) break;
-        if (k == 0) {
-            head = t.head;
-            level = tn_level(t.info);
-        }
-        g.add(t.x, t.y, t.z, t.m);
-        k++;
-        if (k == 8) break;  // a parent holds at most 8 children (3 key bits per layer)
-    }
-    float cm[4];
-    g.fin(cm);
-    const uint64_t n = a.off[l] + p;
-    TNode t;
-    t.x = cm[0];
-    t.y = cm[1];
-    t.z = cm[2];
-    t.m = cm[3];
-    t.first = (uint32_t)(a.off[l - 1] + c);
-    t.info = tn_info(l, k, level);
-    t.head = head;
-    t.pad = 0;
-    a.tnodes[n] = t;
-    a.levels[n] = (uint8_t)level;
-    float S[6];
-    if (a.quad) {
-        const float4* cq = a.qmom + 2 * (a.off[l - 1] + c);
-        if (sizeof(AGG) == sizeof(Agg64)) {
-            Mom64 q;
-            for (uint32_t i = 0; i < k; i++) {
-                const TNode u = ch[c + i];
-                const float4 q0 = cq[2 * i], q1 = cq[2 * i + 1];
-                const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                q.add(u.x, u.y, u.z, u.m, cm, Sc);
-            }
-            q.fin(S);
-        } else {
-            Mom32 q;
-            for (uint32_t i = 0; i < k; i++) {
-                const TNode u = ch[c + i];
-                const float4 q0 = cq[2 * i], q1 = cq[2 * i + 1];
-                const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                q.add(u.x, u.y, u.z, u.m, cm, Sc);
-            }
-            q.fin(S);
-        }
-        store_moments(a.qmom, n, S);
-    }
-    if (a.onodes)
-        write_onode(&a.onodes[n], a.sorted_keys[head] >> (3 * l), cm, t.first, k, l,
-                    a.quad ? S : nullptr);
-    return level;
-}
-
-// ------------------------------------------------------------------- layer kernel pair
-// Layer l is built by two launches over tiles of kLayerTile candidates (sorted particles for
-// the leaf layer, nodes of layer l-1 otherwise).  k_layer_count stores each tile's number of
-// heads and adds it to its supertile's sum (kSuper tiles); k_layer_emit takes its prefix from
-// the supertile sums and the tiles before it in its own supertile, scans its tile in LDS and
-// writes one node per head.  No block waits on another.  Head tests read the per-node level
-// bytes, not the 32-B nodes.
+// ------------------------------------------------------------------------- layer kernels
+// Every layer is built over tiles of kLayerTile candidates (sorted particles for the leaf
+// layer, nodes of layer l-1 otherwise).  A tile's node offset is the sum of the head counts of
+// the supertiles (kSuper tiles) before it plus those of the tiles before it in its own
+// supertile; the block then scans its tile and writes one node per head.  k_layer_count
+// counts the leaf layer's heads; each emit kernel counts the next layer's heads among the
+// nodes it writes (their level bytes), so every layer above the leaves costs one launch.
+// No block waits on another.
 constexpr uint32_t kLayerItems = 2;
 constexpr uint32_t kLayerTile = 256 * kLayerItems;
 constexpr uint32_t kSuper = 64;
@@ -470,15 +405,16 @@ __device__ inline void count_next_heads(uint32_t l, uint32_t first, uint32_t tot
     }
 }
 
-// next_cnt / next_super: the next layer's tile counts and supertile sums (zeroed), or null
-// for the top layer.
-__global__ __launch_bounds__(256) void k_layer_emit(TreeArgs a, uint32_t l,
-                                                    const uint32_t* __restrict__ tile_cnt,
-                                                    const uint32_t* __restrict__ super,
-                                                    uint32_t* next_cnt, uint32_t* next_super) {
-    __shared__ uint32_t s_tmp[8];
+// The leaf layer: heads of equal-key runs among the sorted particles, their leaf index by
+// tile prefix + scan, one leaf per head.  next_cnt / next_super: layer 1's tile counts and
+// supertile sums (zeroed), or null when the leaf layer is the top layer.
+template <bool QUAD>
+__global__ __launch_bounds__(256) void k_leaf_emit(TreeArgs a, const uint32_t* __restrict__ tile_cnt,
+                                                   const uint32_t* __restrict__ super,
+                                                   uint32_t* next_cnt, uint32_t* next_super) {
+    __shared__ uint32_t s_tmp[16];
     __shared__ uint32_t s_cnt[3];
-    const uint32_t n = layer_candidates(a, l);
+    const uint32_t n = a.nt;
     const uint32_t tile = blockIdx.x;
     const uint64_t base = (uint64_t)tile * kLayerTile;
     if (base >= n) return;
@@ -486,50 +422,159 @@ __global__ __launch_bounds__(256) void k_layer_emit(TreeArgs a, uint32_t l,
     uint32_t part = 0;
     for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t];
     if (threadIdx.x < tile - sup * kSuper) part += tile_cnt[sup * kSuper + threadIdx.x];
-    uint32_t prefix;
-    block_exclusive_scan<256>(part, s_tmp, &prefix);
     const uint32_t c0 = (uint32_t)base + threadIdx.x * kLayerItems;
     uint32_t hmask = 0, cnt = 0;
 #pragma unroll
     for (uint32_t i = 0; i < kLayerItems; i++) {
         const uint32_t c = c0 + i;
-        if (c < n && layer_head(a, l, c)) {
+        if (c < n && is_leaf_head(a, c)) {
             hmask |= 1u << i;
             cnt++;
         }
     }
-    uint32_t total;
-    uint32_t p = prefix + block_exclusive_scan<256>(cnt, s_tmp, &total);
+    uint32_t prefix, total;
+    uint32_t p = block_sum_and_scan<256>(part, cnt, s_tmp, &prefix, &total);
+    p += prefix;
+    if (threadIdx.x == 0 && base + kLayerTile >= n) {
+        a.counts[0] = prefix + total;  // the last tile knows the layer's size
+        if (prefix + total > a.cap[0]) atomicOr(a.err, ERR_CAPACITY);
+    }
+    uint32_t pos[kLayerItems], lev[kLayerItems], k = 0;
+    const bool fp64 = a.fp64_from == 0;
+    for (uint32_t i = 0; i < kLayerItems; i++) {
+        const uint32_t j = c0 + i;
+        if (j >= n) break;
+        if (hmask & (1u << i)) {
+            const uint32_t h = fp64 ? emit_leaf<Agg64, QUAD>(a, j, p) : emit_leaf<Agg32, QUAD>(a, j, p);
+            pos[k] = p;
+            lev[k++] = h;
+            p++;
+        }
+        a.leaf_of_sorted[j] = p - 1;
+    }
+    if (next_cnt)
+        count_next_heads(0, prefix, total, pos, lev, k, s_cnt, next_cnt, next_super);
+}
+
+// Layers l >= 1.  Every global load whose address depends only on the block and thread
+// position (the candidate count, the prefix sums, the candidates' level bytes with the seven
+// after them, each candidate node) is issued up front, so a block
+// waits on one round of memory latency instead of a chain of them; only the further children
+// of a parent with more than one child are loaded afterwards.  Reads past the candidate count
+// stay inside the layer's capacity and are masked out.
+template <bool QUAD>
+__global__ __launch_bounds__(256) void k_layer_emit_int(TreeArgs a, uint32_t l,
+                                                        const uint32_t* __restrict__ tile_cnt,
+                                                        const uint32_t* __restrict__ super,
+                                                        uint32_t* next_cnt, uint32_t* next_super) {
+    constexpr uint32_t NI = kLayerItems, NL = kLayerItems + 7;
+    __shared__ uint32_t s_tmp[16];
+    __shared__ uint32_t s_cnt[3];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * kLayerTile;
+    const uint32_t capc = (uint32_t)a.cap[l - 1];
+    const uint64_t cb = a.off[l - 1];
+    const uint32_t c0 = base + threadIdx.x * NI;
+    const uint32_t n = a.counts[l - 1];
+    const uint32_t sup = tile / kSuper;
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t];
+    if (threadIdx.x < tile - sup * kSuper) part += tile_cnt[sup * kSuper + threadIdx.x];
+    uint32_t lv[NL];
+#pragma unroll
+    for (uint32_t i = 0; i < NL; i++) lv[i] = c0 + i < capc ? a.levels[cb + c0 + i] : 0u;
+    TNode u0[NI];
+#pragma unroll
+    for (uint32_t i = 0; i < NI; i++) u0[i] = a.tnodes[cb + (c0 + i < capc ? c0 + i : 0u)];
+    if (base >= n) return;
+    uint32_t hmask = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < NI; i++)
+        if (c0 + i < n && lv[i] >= l) {
+            hmask |= 1u << i;
+            cnt++;
+        }
+    uint32_t prefix, total;
+    uint32_t p = block_sum_and_scan<256>(part, cnt, s_tmp, &prefix, &total);
+    p += prefix;
     if (threadIdx.x == 0 && base + kLayerTile >= n) {
         a.counts[l] = prefix + total;  // the last tile knows the layer's size
         if (prefix + total > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
     }
-    uint32_t pos[kLayerItems], lev[kLayerItems], k = 0;
-    if (l == 0) {
-        const bool fp64 = a.fp64_from == 0;
-        for (uint32_t i = 0; i < kLayerItems; i++) {
-            const uint32_t j = c0 + i;
-            if (j >= n) break;
-            if (hmask & (1u << i)) {
-                const uint32_t h = fp64 ? emit_leaf<Agg64>(a, j, p) : emit_leaf<Agg32>(a, j, p);
-                pos[k] = p;
-                lev[k++] = h;
-                p++;
+    const bool fp64 = l >= a.fp64_from;
+    uint32_t pos[NI], lev[NI], k = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < NI; i++) {
+        if (!(hmask & (1u << i))) continue;
+        if (p < a.cap[l]) {
+            const uint32_t c = c0 + i;
+            // children c, c+1, ... up to the next head (level >= l), at most 8
+            uint32_t kc = 1;
+#pragma unroll
+            for (uint32_t j = 1; j < 8; j++)
+                kc += (kc == j && c + j < n && lv[i + j] < l) ? 1u : 0u;
+            const TNode* ch = a.tnodes + cb + c;
+            float cm[4];
+            if (fp64) {
+                Agg64 g;
+                g.add(u0[i].x, u0[i].y, u0[i].z, u0[i].m);
+                for (uint32_t j = 1; j < kc; j++) {
+                    const TNode v = ch[j];
+                    g.add(v.x, v.y, v.z, v.m);
+                }
+                g.fin(cm);
+            } else {
+                Agg32 g;
+                g.add(u0[i].x, u0[i].y, u0[i].z, u0[i].m);
+                for (uint32_t j = 1; j < kc; j++) {
+                    const TNode v = ch[j];
+                    g.add(v.x, v.y, v.z, v.m);
+                }
+                g.fin(cm);
             }
-            a.leaf_of_sorted[j] = p - 1;
-        }
-    } else {
-        const bool fp64 = l >= a.fp64_from;
-        for (uint32_t i = 0; i < kLayerItems; i++) {
-            if (!(hmask & (1u << i))) continue;
-            if (p < a.cap[l]) {
-                const uint32_t h = fp64 ? emit_parent<Agg64>(a, l, c0 + i, p, n)
-                                        : emit_parent<Agg32>(a, l, c0 + i, p, n);
-                pos[k] = p;
-                lev[k++] = h;
+            const uint32_t level = lv[i];
+            const uint64_t nd = a.off[l] + p;
+            TNode t;
+            t.x = cm[0];
+            t.y = cm[1];
+            t.z = cm[2];
+            t.m = cm[3];
+            t.first = (uint32_t)(cb + c);
+            t.info = tn_info(l, kc, level);
+            t.key = u0[i].key;
+            a.tnodes[nd] = t;
+            a.levels[nd] = (uint8_t)level;
+            float S[6];
+            if (QUAD) {
+                const float4* cq = a.qmom + 2 * (cb + c);
+                if (fp64) {
+                    Mom64 q;
+                    for (uint32_t j = 0; j < kc; j++) {
+                        const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
+                        const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                        const TNode v = ch[j];
+                        q.add(v.x, v.y, v.z, v.m, cm, Sc);
+                    }
+                    q.fin(S);
+                } else {
+                    Mom32 q;
+                    for (uint32_t j = 0; j < kc; j++) {
+                        const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
+                        const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                        const TNode v = ch[j];
+                        q.add(v.x, v.y, v.z, v.m, cm, Sc);
+                    }
+                    q.fin(S);
+                }
+                store_moments(a.qmom, nd, S);
             }
-            p++;
+            if (a.onodes)
+                write_onode(&a.onodes[nd], t.key >> (3 * l), cm, t.first, kc, l,
+                            QUAD ? S : nullptr);
+            pos[k] = p;
+            lev[k++] = level;
         }
+        p++;
     }
     if (next_cnt)
         count_next_heads(l, prefix, total, pos, lev, k, s_cnt, next_cnt, next_super);
@@ -553,8 +598,19 @@ void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_c
                        hipStream_t s) {
     const uint64_t cand_cap = layer == 0 ? a.nt : a.cap[layer - 1];
     if (!cand_cap) return;
-    hipLaunchKernelGGL(k_layer_emit, dim3(layer_tiles(cand_cap)), dim3(256), 0, s, a, layer,
-                       tile_cnt, super, next_cnt, next_super);
+    const dim3 grid(layer_tiles(cand_cap));
+    if (layer == 0 && a.quad)
+        hipLaunchKernelGGL(k_leaf_emit<true>, grid, dim3(256), 0, s, a, tile_cnt, super, next_cnt,
+                           next_super);
+    else if (layer == 0)
+        hipLaunchKernelGGL(k_leaf_emit<false>, grid, dim3(256), 0, s, a, tile_cnt, super, next_cnt,
+                           next_super);
+    else if (a.quad)
+        hipLaunchKernelGGL(k_layer_emit_int<true>, grid, dim3(256), 0, s, a, layer, tile_cnt,
+                           super, next_cnt, next_super);
+    else
+        hipLaunchKernelGGL(k_layer_emit_int<false>, grid, dim3(256), 0, s, a, layer, tile_cnt,
+                           super, next_cnt, next_super);
 }
 
 // Empty every OctreeNode slot that held a node in the previous build but not in this one
