@@ -109,6 +109,7 @@ SIGNATURES = [
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_F)]),
     ("astro_diagnostics", _I, [_P, _P]),
     ("astro_direct_forces_fp64", _I, [_P, _P, _U32, _P]),
+    ("astro_check_division", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
     ("astro_device_bytes", _I, [_P, ctypes.POINTER(_U64)]),
     ("astro_server_create", _I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(AstroParams),
                                  ctypes.POINTER(_P)]),
@@ -427,6 +428,12 @@ class Context:
         out = np.zeros((len(t), 3), np.float64)
         self._chk(self.lib.astro_direct_forces_fp64(self.h, ptr(t), len(t), ptr(out)))
         return out
+
+    def check_division(self, n, seed=1):
+        """Mismatches of the traversal's fast division against IEEE division on n pairs."""
+        m = ctypes.c_uint64()
+        self._chk(self.lib.astro_check_division(self.h, n, seed, ctypes.byref(m)))
+        return m.value
 
     def device_bytes(self):
         b = ctypes.c_uint64()

@@ -294,9 +294,27 @@ static int set_counts(astro_ctx* c, uint32_t ns, uint32_t nt) {
     return ASTRO_OK;
 }
 
+void astro_mass_bounds(astro_ctx* c, const float* m, uint32_t n, size_t stride, bool reset) {
+    if (reset) {
+        c->mass_ok = true;
+        c->mass_min_pos = INFINITY;
+        c->mass_total_max = 0.0;
+    }
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; i++) {
+        const float v = m[i * stride];
+        if (!(v >= 0.0f) || !std::isfinite(v)) c->mass_ok = false;
+        if (v > 0.0f && v < c->mass_min_pos) c->mass_min_pos = v;
+        total += v;
+    }
+    c->mass_total_max = std::max(c->mass_total_max, total);
+}
+
 static int upload_particles(astro_ctx* c, const PositionMass* pos, const VelocityRadius* vel,
                             const ColorType* col, uint32_t n) {
     HIPCHK(c, hipSetDevice(c->device));
+    // the whole upload's total bounds every system's total
+    astro_mass_bounds(c, &pos[0].mass, n, sizeof(PositionMass) / sizeof(float), true);
     std::vector<float4> p4(n), v4(n);
     std::vector<uint32_t> ty(n, 0u);
     c->colors.assign(4ull * n, 0.0f);
@@ -742,6 +760,19 @@ static ForceArgs force_args(astro_ctx* c) {
     a.nt = c->nt;
     a.p = c->params;
     set_accept_threshold(a);  // needs a.p
+    // Domain of the traversal's fast division G M / (r2 r): with every cube within +-L,
+    // targets and centres of mass differ by at most 2L per axis, so r2 <= 12 L^2 + eps2 <= 2^30
+    // and r2 r <= 2^45; r2 >= max(eps2, r2_floor) >= 2^-30 gives r2 r >= 2^-45; G M lies in
+    // [G m_min, G M_total] or is 0.  Targets must be the tree sources (inside the cubes).
+    const double L = 8192.0, lo = 0x1p-44, hi = 0x1p44;
+    const astro_params& p = c->params;
+    const double r2min = std::max((double)p.eps2, (double)p.r2_floor);
+    const double G = p.G;
+    const bool ok = p.quantizer == ASTRO_QUANT_BBOX_POW2 && c->ns == c->nt && c->mass_ok &&
+                    std::isfinite(G) && G > 0.0 && r2min >= 0x1p-30 && p.eps2 <= 0x1p20 &&
+                    (c->mass_min_pos == INFINITY || G * c->mass_min_pos >= lo) &&
+                    G * c->mass_total_max <= hi && 12.0 * L * L + p.eps2 <= 0x1p30;
+    a.div_lmax = ok ? (float)L : 0.0f;
     return a;
 }
 
@@ -1104,6 +1135,21 @@ extern "C" int astro_direct_forces_fp64(astro_ctx* c, const uint32_t* targets, u
             for (uint32_t ch = 0; ch < kChunks; ch++) acc += part[((size_t)t * kChunks + ch) * 3 + k];
             out_xyz[(size_t)t * 3 + k] = acc;
         }
+    return ASTRO_OK;
+}
+
+// Checks the traversal's fast division against the IEEE division on n pseudo-random pairs of
+// its domain; *mismatches receives the number of differing results (0 expected).  Synchronous.
+extern "C" int astro_check_division(astro_ctx* c, uint64_t n, uint64_t seed, uint64_t* mismatches) {
+    if (!c || !mismatches) return ASTRO_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(c->stats);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(d), 0ull, 1, c->stream);
+    if (check_division(n, seed, d, c->stream)) return fail(c, ASTRO_ERR_HIP, "division check launch");
+    unsigned long long h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *mismatches = h;
     return ASTRO_OK;
 }
 

@@ -56,6 +56,11 @@ struct astro_ctx {
     uint64_t scratch_words = 0;
     bool emit = true;
     bool onodes_dirty = true;          // stage calls wrote onodes outside the fused build
+    // bounds of the uploaded masses, for the traversal's division domain: every mass finite
+    // and >= 0, the smallest positive one, the largest system total (conservative)
+    bool mass_ok = false;
+    float mass_min_pos = 0.0f;
+    double mass_total_max = 0.0;
 
     // layer table of the current particles
     uint32_t layers = 0;
@@ -88,6 +93,10 @@ inline int fail(astro_ctx* c, int code, const std::string& msg) {
                                                 hipGetErrorString(e_));                   \
     } while (0)
 
+
+// Folds masses m[0..n) (stride floats apart) into the context's mass bounds; reset first when
+// the uploaded set replaces the previous one.
+void astro_mass_bounds(astro_ctx* c, const float* m, uint32_t n, size_t stride, bool reset);
 
 // host-side pieces shared by capi.hip and serve.hip
 int astro_refresh_layers(astro_ctx* c);

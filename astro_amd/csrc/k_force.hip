@@ -99,7 +99,12 @@ __device__ inline void accumulate_quad(Acc3& F, float s, float dx, float dy, flo
 
 void set_accept_threshold(ForceArgs& a) {
     a.acc = make_accept_rule(a.p.theta);
-    a.fast_sqrt = a.p.r2_floor >= 0x1p-96f ? 1u : 0u;
+    // r2 = max(., r2_floor) >= r2_floor, so RN(sqrt(r2)) >= RN(sqrt(r2_floor)): when that is
+    // already >= r_floor, max(sqrt(r2), r_floor) == sqrt(r2) for every r2.
+    const float r2f = a.p.r2_floor, rf = a.p.r_floor;
+    const bool sqrt_ok = r2f >= 0x1p-96f;
+    const bool rf_noop = r2f >= 0.0f && std::sqrt(r2f) >= rf;
+    a.fast = (sqrt_ok && rf_noop) ? 1u : 0u;
 }
 
 // Correctly rounded sqrt for normal x >= 2^-96: the hardware estimate corrected by one ulp
@@ -127,9 +132,115 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
     return v;
 }
 
-// MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64 potential per target.
-// QUAD: accepted internal nodes add their quadrupole term (second moments in a.qmom).
-template <int NT, bool KAHAN, int MODE, bool QUAD>
+// q = RN(n / d) for d and n in [2^-45, 2^45] (n may also be 0): the compiler's IEEE division
+// sequence (refined reciprocal, two remainder corrections) without its operand scaling and
+// special-case fix-up, which change nothing on that domain.
+__device__ inline float div_rn_mid(float n, float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.0f);
+    r = __builtin_fmaf(e, r, r);
+    float q = n * r;
+    float t = __builtin_fmaf(-d, q, n);
+    q = __builtin_fmaf(t, r, q);
+    t = __builtin_fmaf(-d, q, n);
+    return __builtin_fmaf(t, r, q);
+}
+
+struct Visits {
+    unsigned long long visits = 0, leafs = 0, monos = 0, opens = 0;
+};
+
+// One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
+// potential.  QUAD: accepted internal nodes add their quadrupole term.  FAST: square root
+// without denormal scaling and no r floor.  FDIV: every G M / (r2 r) is in div_rn_mid's domain.
+template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV>
+__device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uint32_t* s_stack,
+                                         uint32_t tid, float4 p, uint32_t self, uint32_t sys,
+                                         float rmin0, Acc3& F, double& phi, Visits& st) {
+    const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
+    const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
+    const uint32_t node_total = (uint32_t)a.node_total;  // node indices are 32-bit
+    const char* tn_base = reinterpret_cast<const char*>(a.tnodes);
+    // The loop leaves only at its head: an overflow or a bad index empties the stack instead
+    // of breaking out of a divergent branch.
+    uint32_t top_first = (uint32_t)(a.root_off + sys), top_cnt = 1, sp = 0;
+    while (true) {
+        if (top_cnt == 0) {
+            if (sp == 0) break;
+            sp--;
+            const uint32_t e = s_stack[sp * NT + tid];
+            top_first = e >> 3;
+            top_cnt = (e & 7u) + 1u;
+        }
+        top_cnt--;
+        uint32_t n = top_first + top_cnt;
+        // a bad index contributes nothing and empties the stack (the loop then ends at its
+        // head); every other path updates F the same way, so F needs no copies
+        const bool bad = n >= node_total;
+        if (bad) atomicOr(a.err, ERR_NODE_INDEX);
+        n = bad ? 0u : n;
+        // 32-bit byte offset from a uniform base (n < 2^27 for any context that fits)
+        const float4* np = reinterpret_cast<const float4*>(tn_base + (size_t)(n << 5));
+        const float4 cm = np[0];
+        const uint2 li = reinterpret_cast<const uint2*>(np)[2];
+        const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
+        float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
+        r2 = fmaxf(r2, r2f);
+        const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
+        const uint32_t layer = tn_layer(li.y);
+        const bool leaf = layer == 0u;
+        const bool far = rr >= ldexpf(rmin0, (int)layer);
+        const bool hit = !bad & (n != self) & (r2 > cut);
+        const bool acc = !bad & far;
+        const bool interact = leaf ? hit : acc;
+        const bool open = !bad & !leaf & !far;
+        // the term is computed for every lane and kept only where it interacts
+        const float gm = G * cm.w;
+        const float D = r2 * rr;
+        const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
+        if (QUAD && interact && !leaf) {
+            const float4* qp = a.qmom + 2 * (size_t)n;
+            const QuadTerm qt = quad_term(qp[0], qp[1], dx, dy, dz, r2, D, G);
+            accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
+            if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+        } else if (KAHAN) {
+            Acc3 T = F;
+            accumulate<true>(T, s, dx, dy, dz);
+            F.x = interact ? T.x : F.x;
+            F.y = interact ? T.y : F.y;
+            F.z = interact ? T.z : F.z;
+            F.cx = interact ? T.cx : F.cx;
+            F.cy = interact ? T.cy : F.cy;
+            F.cz = interact ? T.cz : F.cz;
+        } else {
+            // F never holds -0 (it starts at +0, and x + (-x) rounds to +0), so the zero term
+            // fma(0, d, F) leaves it unchanged for finite d; the selects take lane masks
+            const float s_hit = hit ? s : 0.0f, s_acc = acc ? s : 0.0f;
+            const float sk = leaf ? s_hit : s_acc;
+            F.x = __builtin_fmaf(sk, dx, F.x);
+            F.y = __builtin_fmaf(sk, dy, F.y);
+            F.z = __builtin_fmaf(sk, dz, F.z);
+        }
+        if (MODE == 1) {
+            st.visits += !bad;
+            st.leafs += leaf && interact;
+            st.monos += !leaf && interact;
+            st.opens += open;
+        }
+        if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
+        const bool push = open & (top_cnt > 0u);
+        const bool overflow = push & (sp >= depth);
+        if (overflow) atomicOr(a.err, ERR_STACK);
+        if (push & !overflow) s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
+        sp = (bad | overflow) ? 0u : sp + (push ? 1u : 0u);
+        top_first = open ? li.x : top_first;
+        top_cnt = (bad | overflow) ? 0u : (open ? tn_nchild(li.y) : top_cnt);
+    }
+}
+
+// Targets in Morton order (sorted position j), one per thread; the stack of (first child,
+// remaining count) runs lives in LDS as [depth][NT].
+template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t depth) {
     extern __shared__ uint32_t s_stack[];  // [depth][NT]
 
@@ -139,144 +250,113 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
     const uint32_t tile = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
     const uint32_t tid = threadIdx.x;
     const uint32_t j = tile * NT + tid;
+    const bool active = j < a.ns;
+    const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
+    // the fast division holds for the whole wave when every lane's system cube lies within
+    // +-div_lmax (then |d| <= 2 div_lmax per axis; masses are bounded on the host)
+    bool fdiv_lane = true;
+    if (active) {
+        const float L = a.div_lmax;
+        const float4 cu = a.cubes[sys];
+        fdiv_lane = L > 0.0f && fabsf(cu.x) <= L && fabsf(cu.y) <= L && fabsf(cu.z) <= L &&
+                    fabsf(cu.x + cu.w) <= L && fabsf(cu.y + cu.w) <= L && fabsf(cu.z + cu.w) <= L;
+    }
+    const bool fdiv = __builtin_amdgcn_readfirstlane(__all(fdiv_lane ? 1 : 0)) != 0;
 
-    unsigned long long visits = 0, leafs = 0, monos = 0, opens = 0;
-    if (j < a.ns) {
+    Visits st;
+    if (active) {
         const uint32_t pidx = j < a.nt ? a.sorted_idx[j] : j;
         const float4 p = a.pos[pidx];
         const uint32_t type = a.type[pidx];
         const uint32_t self =
             (type != 2u && j < a.nt) ? (uint32_t)(a.off0 + a.leaf_of_sorted[j]) : 0xFFFFFFFFu;
-        const uint32_t sys = a.sys_of ? a.sys_of[j] : 0u;
         float side0;
         if (a.p.side_rule == ASTRO_SIDE_SHIFTED) side0 = 2097152.0f;  // 2^21
         else side0 = ldexpf(a.cubes[sys].w, -(int)a.p.bits_per_axis);
-        const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
-        const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
-        const bool fast_sqrt = a.fast_sqrt != 0u;
         // a node at layer l is accepted iff r >= rmin0 * 2^l (accept.hpp)
         const float rmin0 = accept_rmin(side0, a.acc);
-
         Acc3 F;
         double phi = 0.0;
-        // The loop leaves only at its head: an overflow or a bad index empties the stack
-        // instead of breaking out of a divergent branch.
-        uint32_t top_first = (uint32_t)(a.root_off + sys), top_cnt = 1, sp = 0;
-        while (true) {
-            if (top_cnt == 0) {
-                if (sp == 0) break;
-                sp--;
-                const uint32_t e = s_stack[sp * NT + tid];
-                top_first = e >> 3;
-                top_cnt = (e & 7u) + 1u;
-            }
-            top_cnt--;
-            const uint32_t n = top_first + top_cnt;
-            if (n >= a.node_total) {
-                atomicOr(a.err, ERR_NODE_INDEX);
-                top_cnt = 0;
-                sp = 0;
-                continue;
-            }
-            const float4* np = reinterpret_cast<const float4*>(a.tnodes + n);
-            const float4 cm = np[0];
-            const uint2 li = reinterpret_cast<const uint2*>(np)[2];
-            const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
-            float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
-            r2 = fmaxf(r2, r2f);
-            const float rr = fmaxf(fast_sqrt ? sqrt_rn_normal(r2) : sqrtf(r2), rf);
-            const uint32_t layer = tn_layer(li.y);
-            const bool leaf = layer == 0u;
-            const bool acc = rr >= ldexpf(rmin0, (int)layer);
-            const bool interact = leaf ? ((n != self) && (r2 > cut)) : acc;
-            const bool open = !leaf && !acc;
-            // the term is computed for every lane and kept only where it interacts
-            const float gm = G * cm.w;
-            const float D = r2 * rr;
-            const float s = (gm / D);
-            if (QUAD && interact && !leaf) {
-                const float4* qp = a.qmom + 2 * (size_t)n;
-                const QuadTerm qt = quad_term(qp[0], qp[1], dx, dy, dz, r2, D, G);
-                accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
-                if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
-            } else if (KAHAN) {
-                Acc3 T = F;
-                accumulate<true>(T, s, dx, dy, dz);
-                F.x = interact ? T.x : F.x;
-                F.y = interact ? T.y : F.y;
-                F.z = interact ? T.z : F.z;
-                F.cx = interact ? T.cx : F.cx;
-                F.cy = interact ? T.cy : F.cy;
-                F.cz = interact ? T.cz : F.cz;
-            } else {
-                F.x = interact ? __builtin_fmaf(s, dx, F.x) : F.x;
-                F.y = interact ? __builtin_fmaf(s, dy, F.y) : F.y;
-                F.z = interact ? __builtin_fmaf(s, dz, F.z) : F.z;
-            }
-            if (MODE == 1) {
-                visits++;
-                leafs += leaf && interact;
-                monos += !leaf && interact;
-                opens += open;
-            }
-            if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
-            if (open) {
-                const bool push = top_cnt > 0u;
-                if (push && sp >= depth) {
-                    atomicOr(a.err, ERR_STACK);
-                    top_cnt = 0;
-                    sp = 0;
-                } else {
-                    if (push) {
-                        s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
-                        sp++;
-                    }
-                    top_first = li.x;
-                    top_cnt = tn_nchild(li.y);
-                }
-            }
-        }
+        if (fdiv)
+            traverse<NT, KAHAN, MODE, QUAD, FAST, true>(a, depth, s_stack, tid, p, self, sys,
+                                                        rmin0, F, phi, st);
+        else
+            traverse<NT, KAHAN, MODE, QUAD, FAST, false>(a, depth, s_stack, tid, p, self, sys,
+                                                         rmin0, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
     }
     if (MODE == 1) {
-        visits = wave_sum_u64(visits);
-        leafs = wave_sum_u64(leafs);
-        monos = wave_sum_u64(monos);
-        opens = wave_sum_u64(opens);
+        const unsigned long long v = wave_sum_u64(st.visits), lf = wave_sum_u64(st.leafs),
+                                 mo = wave_sum_u64(st.monos), op = wave_sum_u64(st.opens);
         if ((tid & 63u) == 0u) {
-            atomicAdd(&a.stats[0], visits);
-            atomicAdd(&a.stats[1], leafs);
-            atomicAdd(&a.stats[2], monos);
-            atomicAdd(&a.stats[3], opens);
+            atomicAdd(&a.stats[0], v);
+            atomicAdd(&a.stats[1], lf);
+            atomicAdd(&a.stats[2], mo);
+            atomicAdd(&a.stats[3], op);
         }
     }
 }
 
-template <int NT>
-static void launch_force_nt(const ForceArgs& a, int mode, hipStream_t s) {
+// Self-check: div_rn_mid against '/' on n pairs drawn log-uniformly from its domain
+// (numerator and denominator in [2^-45, 2^45]), counting mismatching bit patterns.
+__global__ void k_check_division(uint64_t n, uint64_t seed, unsigned long long* mismatch) {
+    unsigned long long bad = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t w[2];
+        for (int k = 0; k < 2; k++) {  // SplitMix64, two draws per pair
+            uint64_t z = seed + (2 * i + k + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            w[k] = z ^ (z >> 31);
+        }
+        // exponents in [-45, 44], random 23-bit mantissas
+        const uint32_t ex = (uint32_t)(w[0] >> 32) % 90u, ey = (uint32_t)(w[1] >> 32) % 90u;
+        const float x = __uint_as_float(((ex + 127u - 45u) << 23) | ((uint32_t)w[0] & 0x7FFFFFu));
+        const float y = __uint_as_float(((ey + 127u - 45u) << 23) | ((uint32_t)w[1] & 0x7FFFFFu));
+        const float q1 = div_rn_mid(x, y), q2 = x / y;
+        bad += __float_as_uint(q1) != __float_as_uint(q2);
+    }
+    bad = wave_sum_u64(bad);
+    if ((threadIdx.x & 63u) == 0u && bad) atomicAdd(mismatch, bad);
+}
+int check_division(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_division, dim3(2048), dim3(256), 0, s, n, seed, d_mismatch);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NT, bool KAHAN, bool QUAD, bool FAST>
+static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t blocks = (a.ns + NT - 1) / NT;
     const uint32_t depth = a.p.bits_per_axis + 2u;  // open levels <= layers - 1
     const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
-    const bool k = a.p.kahan != 0;
-#define ASTRO_FORCE_LAUNCH(KH, MD, QD) \
-    hipLaunchKernelGGL((k_force_compact<NT, KH, MD, QD>), dim3(blocks), dim3(NT), lds, s, a, depth)
-#define ASTRO_FORCE_MODES(KH, QD)                    \
-    do {                                             \
-        if (mode == 0) ASTRO_FORCE_LAUNCH(KH, 0, QD); \
-        else if (mode == 1) ASTRO_FORCE_LAUNCH(KH, 1, QD); \
-        else ASTRO_FORCE_LAUNCH(KH, 2, QD);          \
-    } while (0)
-    if (a.p.multipole == 2) {
-        if (k) ASTRO_FORCE_MODES(true, true);
-        else ASTRO_FORCE_MODES(false, true);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
+                           s, a, depth);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
+                           s, a, depth);
+    else
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
+                           s, a, depth);
+}
+
+template <int NT>
+static void launch_force_nt(const ForceArgs& a, int mode, hipStream_t s) {
+    const bool k = a.p.kahan != 0, q = a.p.multipole == 2, f = a.fast != 0;
+    if (k) {
+        if (q) f ? launch_force_variant<NT, true, true, true>(a, mode, s)
+               : launch_force_variant<NT, true, true, false>(a, mode, s);
+        else f ? launch_force_variant<NT, true, false, true>(a, mode, s)
+               : launch_force_variant<NT, true, false, false>(a, mode, s);
     } else {
-        if (k) ASTRO_FORCE_MODES(true, false);
-        else ASTRO_FORCE_MODES(false, false);
+        if (q) f ? launch_force_variant<NT, false, true, true>(a, mode, s)
+               : launch_force_variant<NT, false, true, false>(a, mode, s);
+        else f ? launch_force_variant<NT, false, false, true>(a, mode, s)
+               : launch_force_variant<NT, false, false, false>(a, mode, s);
     }
-#undef ASTRO_FORCE_MODES
-#undef ASTRO_FORCE_LAUNCH
 }
 
 bool force_block_supported(uint32_t nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 512; }

@@ -349,7 +349,7 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
 // counts the leaf layer's heads; each emit kernel counts the next layer's heads among the
 // nodes it writes (their level bytes), so every layer above the leaves costs one launch.
 // No block waits on another.
-constexpr uint32_t kLayerItems = 2;
+constexpr uint32_t kLayerItems = 1;
 constexpr uint32_t kLayerTile = 256 * kLayerItems;
 constexpr uint32_t kSuper = 64;
 

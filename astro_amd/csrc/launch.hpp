@@ -143,10 +143,16 @@ struct ForceArgs {
     uint32_t ns, nt;
     uint32_t block;             // traversal block size (64, 128, 256 or 512)
     AcceptRule acc;             // opening test (accept.hpp), from p.theta
-    uint32_t fast_sqrt;         // 1: r2 >= r2_floor >= 2^-96, so no denormal scaling needed
+    uint32_t fast;              // 1: r2 >= r2_floor >= 2^-96 (no denormal scaling in sqrt) and
+                                //    sqrt(r2_floor) >= r_floor (the r floor never binds)
+    float div_lmax;             // > 0: when a system's cube lies within [-div_lmax, div_lmax]^3,
+                                //    every G M / (r2 r) of its targets is in the domain of
+                                //    div_rn_mid; 0: always use the full division
     astro_params p;
 };
-// Fills acc and fast_sqrt from p.
+// Self-check of div_rn_mid against the IEEE division on n pseudo-random pairs of its domain.
+int check_division(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s);
+// Fills acc and fast from p.
 void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);
 void launch_force_compact(const ForceArgs& a, hipStream_t s);

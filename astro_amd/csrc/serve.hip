@@ -105,6 +105,7 @@ extern "C" int astro_server_create(const astro_server_config* cfg, const astro_p
         astro_server_destroy(s);
         return rc;
     }
+    astro_mass_bounds(s->ctx, nullptr, 0, 4, true);  // no systems yet; admissions fold in
     const uint64_t NP = cfg->max_particles;
     if (hipMalloc(&s->stage_pos, NP * sizeof(float4)) != hipSuccess ||
         hipMalloc(&s->stage_vel, NP * sizeof(float4)) != hipSuccess ||
@@ -209,6 +210,7 @@ static uint32_t admit(astro_server* s) {
             break;
         }
         parts += r->n;
+        astro_mass_bounds(s->ctx, &r->pos[0].w, r->n, 4, false);  // kept conservative
         r->t_admit = t;
         s->active.push_back(std::move(s->queue.front()));
         s->queue.pop_front();

@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup + steps only, no JSON extras (for rocprofv3 runs)")
     ap.add_argument("--batch-systems", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--overload-rate", type=float, default=200.0)
     ap.add_argument("--overload-seconds", type=float, default=1.5)
     ap.add_argument("--skip-accuracy", action="store_true")
+    ap.add_argument("--large-n", type=int, default=4194304)
+    ap.add_argument("--large-steps", type=int, default=20)
+    ap.add_argument("--skip-large", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -62,12 +65,13 @@ def parse():
     return ap.parse_args()
 
 
-def stage_bytes(n, V):
-    """Algorithmic HBM bytes per step of the HBM-bound stages (documented in DESIGN.md)."""
+def stage_bytes(n, V, passes=8, emit=True, quad=False):
+    """Algorithmic HBM bytes per step of the HBM-bound stages."""
+    node = 66 + (128 if emit else 0) + (64 if quad else 0)  # write 33 + read 33 (+ OctreeNode, moments)
     return {
-        "bbox_morton": n * (16 + 16 + 8 + 8 + 4),  # bbox read + key read/ 2 key writes + iota
-        "sort": n * (24 * 8 + 8),                  # 8 digit passes of 12-B pairs + histogram
-        "build": n * (4 + 16 + 8 + 4) + V * (32 + 128 + 32),
+        "bbox_morton": n * (16 + 16 + 8 + 8 + 4),  # bbox read; Morton read, 2 key writes, iota
+        "sort": n * 32 * passes,                   # per pass: count reads 8, scatter moves 2 x 12
+        "build": n * 40 + V * node,                # leaf gather + per-node traffic
         "update": n * (16 + 16 + 16 + 4 + 16 + 16),
     }
 
@@ -307,6 +311,43 @@ def accuracy(astro, ctx, pos, vel, col, P, a):
     return out
 
 
+def bench_large(astro, a):
+    """4,194,304 particles: Plummer sphere plus a 10% uniform background cube of side 20,
+    Salpeter masses on [0.1, 10]; theta 0.7 and 0.5, monopoles and quadrupoles; RMS error on
+    4,096 random targets against the FP64 direct sum over all particles."""
+    pos, vel, col = astro.ic_plummer_background(a.large_n, 77)
+    n = len(pos)
+    tg = np.sort(np.random.default_rng(3).choice(n, 4096, replace=False))
+    out = {"n_particles": n, "steps": a.large_steps,
+           "ic": "Plummer (a = 1) + 10% uniform cube of side 20, masses power law -2.35 on "
+                 "[0.1, 10], seed 77"}
+    with astro.Context(n, bits_per_axis=21, emit_octree_nodes=False) as ctx:
+        ctx.set_params(astro.params_physics(21))  # G and eps2 of the direct sum
+        ctx.upload(pos, vel, col)
+        ref = ctx.direct_forces_fp64(tg)
+        for theta, mp in ((0.7, 0), (0.5, 0), (0.5, 2)):
+            P = astro.params_physics(21)
+            P.theta, P.multipole = theta, mp
+            ctx.set_params(P)
+            ctx.upload(pos, vel, col)
+            ctx.build_tree()
+            ctx.compute_forces()
+            ctx.check()
+            err = rms_rel(ctx.forces()[tg], ref)
+            ctx.step(a.dt, 2)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.step(a.dt, a.large_steps)
+            ctx.sync()
+            el = time.perf_counter() - t0
+            ctx.check()
+            out["theta%.1f_%s" % (theta, "quadrupole" if mp else "monopole")] = {
+                "value": n * a.large_steps / el, "ms_per_step": el / a.large_steps * 1e3,
+                "rms_rel_force_error": err}
+        out["device_bytes"] = ctx.device_bytes()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -340,6 +381,14 @@ def main():
         return
 
     st1 = ctx.traversal_stats()
+    # PCIe-inclusive: host arrays in, one step, host arrays out
+    p_h, v_h, c_h = ctx.download()
+    ctx.sync()
+    tp = time.perf_counter()
+    ctx.upload(p_h, v_h, c_h)
+    ctx.step(a.dt, 1)
+    p_h, v_h, c_h = ctx.download()
+    pcie_s = time.perf_counter() - tp
     # graph-replayed steps of the same state, for reference
     ctx.set_use_graph(True)
     ctx.step(a.dt, 2)
@@ -407,6 +456,9 @@ def main():
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
         "stages": stages,
         "tree_nodes": V,
+        "device_bytes": ctx.device_bytes(),
+        "pcie_inclusive": {"value": n / pcie_s, "ms": pcie_s * 1e3,
+                           "what": "astro_upload of the host arrays + one step + astro_download"},
         "graph_replay": {"value": n * K / wall_graph, "ms_per_step": wall_graph / K * 1e3},
         "without_octree_nodes": {"value": n * K / wall_noemit,
                                  "ms_per_step": wall_noemit / K * 1e3},
@@ -414,6 +466,8 @@ def main():
     if not a.skip_accuracy:
         out["accuracy"] = accuracy(astro_amd, ctx, pos, vel, col, P, a)
     ctx.close()
+    if not a.skip_large:
+        out["large"] = bench_large(astro_amd, a)
     if not a.skip_batched:
         out["batched"] = bench_batched(astro_amd, a)
     if not a.skip_serving:

@@ -178,6 +178,11 @@ int astro_diagnostics(astro_ctx* ctx, double* out);
  * allocates temporaries. */
 int astro_direct_forces_fp64(astro_ctx* ctx, const uint32_t* targets, uint32_t n_targets,
                              double* out_xyz);
+/* Self-check of the traversal's fast division (used when every quotient G M / (r^2 r) lies
+ * in [2^-90, 2^90]) against the IEEE division, on n pseudo-random operand pairs of its domain;
+ * *mismatches receives the count of differing results (0 expected).  No reference
+ * counterpart.  Synchronous. */
+int astro_check_division(astro_ctx* ctx, uint64_t n, uint64_t seed, uint64_t* mismatches);
 /* Device memory the context allocated at creation. */
 int astro_device_bytes(const astro_ctx* ctx, uint64_t* bytes);
 

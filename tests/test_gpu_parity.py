@@ -373,3 +373,29 @@ def test_quadrupole_steps_and_back_to_monopole(astro, orc, small_ctx):
     assert ctx.octree_nodes().tobytes() == t["nodes"].tobytes()
     want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
     assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
+
+
+def test_fast_division_matches_ieee(astro, small_ctx):
+    # the traversal's division without operand scaling, on 2^26 pairs of its domain
+    say("fast division self-check")
+    assert small_ctx.check_division(1 << 26, seed=12345) == 0
+
+
+@pytest.mark.parametrize("scale", [1.0, 3.0e4])
+def test_forces_bitexact_both_division_paths(astro, orc, small_ctx, scale):
+    # scale 1: cubes inside +-8192, masses bounded: the fast division; scale 3e4: cubes
+    # beyond it, the full IEEE division -- both equal the oracle bit for bit
+    say("division path, scale", scale)
+    ctx = small_ctx
+    pos, vel, col = plummer(astro, 16384, 21)
+    pos = pos.copy()
+    pos["position"] *= np.float32(scale)
+    P, Po = astro.params_physics(21), orc.params_physics(21)
+    ctx.set_params(P)
+    ctx.upload(pos, vel, col)
+    ctx.build_tree()
+    ctx.compute_forces()
+    ctx.check()
+    t = orc.build_tree(pos, Po)
+    want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
+    assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
