@@ -99,12 +99,14 @@ __device__ inline void accumulate_quad(Acc3& F, float s, float dx, float dy, flo
 
 void set_accept_threshold(ForceArgs& a) {
     a.acc = make_accept_rule(a.p.theta);
-    // r2 = max(., r2_floor) >= r2_floor, so RN(sqrt(r2)) >= RN(sqrt(r2_floor)): when that is
-    // already >= r_floor, max(sqrt(r2), r_floor) == sqrt(r2) for every r2.
-    const float r2f = a.p.r2_floor, rf = a.p.r_floor;
-    const bool sqrt_ok = r2f >= 0x1p-96f;
-    const bool rf_noop = r2f >= 0.0f && std::sqrt(r2f) >= rf;
-    a.fast = (sqrt_ok && rf_noop) ? 1u : 0u;
+    // For finite positions r2 = RN(|d|^2 + eps2) >= eps2 (rounding is monotone).  So when
+    // eps2 >= r2_floor the r2 floor never binds; when RN(sqrt(eps2)) >= r_floor the r floor
+    // never binds either; eps2 >= 2^-96 lets the square root skip its denormal scaling; and
+    // leaf_cutoff_r2 < eps2 makes every leaf pass r2 > leaf_cutoff_r2.
+    const float e2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor, cut = a.p.leaf_cutoff_r2;
+    const bool ok = std::isfinite(e2) && e2 >= 0x1p-96f && e2 >= r2f && std::sqrt(e2) >= rf &&
+                    cut < e2;
+    a.fast = ok ? 1u : 0u;
 }
 
 // Correctly rounded sqrt for normal x >= 2^-96: the hardware estimate corrected by one ulp
@@ -151,8 +153,9 @@ struct Visits {
 };
 
 // One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
-// potential.  QUAD: accepted internal nodes add their quadrupole term.  FAST: square root
-// without denormal scaling and no r floor.  FDIV: every G M / (r2 r) is in div_rn_mid's domain.
+// potential.  QUAD: accepted internal nodes add their quadrupole term.  FAST: a.fast holds (no
+// floor or cut-off can bind, square root without denormal scaling; exact for finite positions).
+// FDIV: every G M / (r2 r) is in div_rn_mid's domain.
 template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV>
 __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uint32_t* s_stack,
                                          uint32_t tid, float4 p, uint32_t self, uint32_t sys,
@@ -185,12 +188,12 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
         const uint2 li = reinterpret_cast<const uint2*>(np)[2];
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
-        r2 = fmaxf(r2, r2f);
+        if (!FAST) r2 = fmaxf(r2, r2f);
         const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
         const uint32_t layer = tn_layer(li.y);
         const bool leaf = layer == 0u;
         const bool far = rr >= ldexpf(rmin0, (int)layer);
-        const bool hit = !bad & (n != self) & (r2 > cut);
+        const bool hit = !bad & (n != self) & (FAST || r2 > cut);
         const bool acc = !bad & far;
         const bool interact = leaf ? hit : acc;
         const bool open = !bad & !leaf & !far;

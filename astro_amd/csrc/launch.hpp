@@ -143,8 +143,8 @@ struct ForceArgs {
     uint32_t ns, nt;
     uint32_t block;             // traversal block size (64, 128, 256 or 512)
     AcceptRule acc;             // opening test (accept.hpp), from p.theta
-    uint32_t fast;              // 1: r2 >= r2_floor >= 2^-96 (no denormal scaling in sqrt) and
-                                //    sqrt(r2_floor) >= r_floor (the r floor never binds)
+    uint32_t fast;              // 1: eps2 >= max(r2_floor, 2^-96, leaf_cutoff_r2+),
+                                //    sqrt(eps2) >= r_floor: no floor or cut-off can bind
     float div_lmax;             // > 0: when a system's cube lies within [-div_lmax, div_lmax]^3,
                                 //    every G M / (r2 r) of its targets is in the domain of
                                 //    div_rn_mid; 0: always use the full division
