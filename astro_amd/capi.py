@@ -106,7 +106,9 @@ SIGNATURES = [
     ("astro_upload_systems", _I, [_P, _P, _P, _P, _P, _U32]),
     ("astro_system_count", _I, [_P, ctypes.POINTER(_U32)]),
     ("astro_set_force_block", _I, [_P, _U32]),
-    ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_F)]),
+    ("astro_set_force_xcd_chunk", _I, [_P, _U32]),
+    ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32),
+                            ctypes.POINTER(_F)]),
     ("astro_diagnostics", _I, [_P, _P]),
     ("astro_direct_forces_fp64", _I, [_P, _P, _U32, _P]),
     ("astro_check_division", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
@@ -408,11 +410,16 @@ class Context:
     def set_force_block(self, block):
         self._chk(self.lib.astro_set_force_block(self.h, block))
 
+    def set_force_xcd_chunk(self, chunk):
+        self._chk(self.lib.astro_set_force_xcd_chunk(self.h, chunk))
+
     def autotune(self, reps=3):
-        b = ctypes.c_uint32()
+        """Picks the traversal block size and XCD chunk; returns (block, chunk, ms)."""
+        b, k = ctypes.c_uint32(), ctypes.c_uint32()
         t = ctypes.c_float()
-        self._chk(self.lib.astro_autotune(self.h, reps, ctypes.byref(b), ctypes.byref(t)))
-        return b.value, t.value
+        self._chk(self.lib.astro_autotune(self.h, reps, ctypes.byref(b), ctypes.byref(k),
+                                          ctypes.byref(t)))
+        return b.value, k.value, t.value
 
     def diagnostics(self):
         """Per system: mass, kinetic, potential, px, py, pz, mx, my, mz (FP64)."""

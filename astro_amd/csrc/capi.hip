@@ -756,6 +756,7 @@ static ForceArgs force_args(astro_ctx* c) {
     a.node_total = c->total;
     a.phi = nullptr;
     a.block = c->force_block;
+    a.xcd_chunk = c->force_chunk;
     a.ns = c->ns;
     a.nt = c->nt;
     a.p = c->params;
@@ -1026,42 +1027,57 @@ extern "C" int astro_set_force_block(astro_ctx* c, uint32_t block) {
 
 // Times the force traversal over the current tree for every supported block size (median of
 // `reps` launches each) and keeps the fastest.  Builds the tree first.
-extern "C" int astro_autotune(astro_ctx* c, uint32_t reps, uint32_t* chosen, float* best_ms) {
+extern "C" int astro_set_force_xcd_chunk(astro_ctx* c, uint32_t chunk) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (chunk > 4096) return fail(c, ASTRO_ERR_ARG, "xcd_chunk must be 0..4096");
+    c->force_chunk = chunk;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_autotune(astro_ctx* c, uint32_t reps, uint32_t* chosen_block,
+                              uint32_t* chosen_chunk, float* best_ms) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
     if (reps == 0) reps = 3;
     if (int rc = enqueue_build(c)) return rc;
-    const uint32_t cand[4] = {64, 128, 256, 512};
+    const uint32_t blocks[4] = {64, 128, 256, 512};
+    const uint32_t chunks[4] = {0, 8, 32, 128};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    uint32_t best = c->force_block;
+    uint32_t best_b = c->force_block, best_k = c->force_chunk;
     float best_t = 1e30f;
-    for (uint32_t b : cand) {
-        c->force_block = b;
-        if (int rc = enqueue_forces(c, nullptr)) return rc;  // warm
-        std::vector<float> ts;
-        for (uint32_t r = 0; r < reps; r++) {
-            hipEventRecord(e0, c->stream);
-            enqueue_forces(c, nullptr);
-            hipEventRecord(e1, c->stream);
-            HIPCHK(c, hipEventSynchronize(e1));
-            float ms = 0.0f;
-            hipEventElapsedTime(&ms, e0, e1);
-            ts.push_back(ms);
+    for (uint32_t b : blocks)
+        for (uint32_t k : chunks) {
+            c->force_block = b;
+            c->force_chunk = k;
+            if (int rc = enqueue_forces(c, nullptr)) return rc;  // warm
+            std::vector<float> ts;
+            for (uint32_t r = 0; r < reps; r++) {
+                hipEventRecord(e0, c->stream);
+                enqueue_forces(c, nullptr);
+                hipEventRecord(e1, c->stream);
+                HIPCHK(c, hipEventSynchronize(e1));
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, e0, e1);
+                ts.push_back(ms);
+            }
+            std::sort(ts.begin(), ts.end());
+            const float med = ts[ts.size() / 2];
+            if (med < best_t) {
+                best_t = med;
+                best_b = b;
+                best_k = k;
+            }
         }
-        std::sort(ts.begin(), ts.end());
-        const float med = ts[ts.size() / 2];
-        if (med < best_t) {
-            best_t = med;
-            best = b;
-        }
-    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    c->force_block = best;
+    c->force_block = best_b;
+    c->force_chunk = best_k;
     invalidate_graph(c);
-    if (chosen) *chosen = best;
+    if (chosen_block) *chosen_block = best_b;
+    if (chosen_chunk) *chosen_chunk = best_k;
     if (best_ms) *best_ms = best_t;
     return ASTRO_OK;
 }

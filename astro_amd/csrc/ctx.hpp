@@ -51,6 +51,7 @@ struct astro_ctx {
     double* phi = nullptr;             // [max_particles] FP64 potential (diagnostics)
     double* diag_partial = nullptr;    // [max_systems * kDiagChunks * 9]
     uint32_t force_block = 256;        // traversal block size (autotuned)
+    uint32_t force_chunk = 0;          // traversal tile runs per XCD (autotuned; 0: eighths)
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation
     uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
     uint64_t scratch_words = 0;

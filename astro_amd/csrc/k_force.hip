@@ -247,10 +247,19 @@ template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t depth) {
     extern __shared__ uint32_t s_stack[];  // [depth][NT]
 
-    // XCD-aware mapping: hardware places block b on XCD b % 8; give XCD x the x-th
-    // contiguous eighth of the target tiles.
-    const uint32_t nb = gridDim.x, b = blockIdx.x, xcd = b & 7u, q = nb >> 3, r = nb & 7u;
-    const uint32_t tile = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
+    // x-th contiguous eighth of the target tiles (best L2 locality); xcd_chunk K deals runs of
+    // K consecutive tiles to the XCDs in turn (better balance when work varies along the
+    // curve).  Either way the mapping is a bijection on [0, gridDim.x).
+    const uint32_t nb = gridDim.x, b = blockIdx.x, K = a.xcd_chunk;
+    uint32_t tile;
+    if (K == 0u) {
+        const uint32_t xcd = b & 7u, q = nb >> 3, r = nb & 7u;
+        tile = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    } else {
+        const uint32_t full = (nb / (8u * K)) * 8u * K, i = b >> 3;
+        tile = b < full ? ((i / K) * 8u + (b & 7u)) * K + (i % K) : b;
+    }
     const uint32_t tid = threadIdx.x;
     const uint32_t j = tile * NT + tid;
     const bool active = j < a.ns;

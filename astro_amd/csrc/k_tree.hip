@@ -213,10 +213,8 @@ struct Agg64 {
 
 // emittedColor / emittedColorCenter hold the second moments S (xx, yy, zz, xy | xz, yz) in
 // quadrupole mode and stay zero otherwise: the physics reads nothing else from them.
-__device__ inline void write_onode(OctreeNode* o, uint64_t code, const float* cm,
-                                   uint32_t first, uint32_t nchild, uint32_t layer,
-                                   const float* S = nullptr) {
-    float4* q = reinterpret_cast<float4*>(o);
+__device__ inline void onode_fields(float4* q, uint64_t code, const float* cm, uint32_t first,
+                                    uint32_t nchild, uint32_t layer, const float* S) {
     q[0] = make_float4(__uint_as_float((uint32_t)code), __uint_as_float((uint32_t)(code >> 32)),
                        0.0f, 0.0f);
     q[1] = make_float4(cm[0], cm[1], cm[2], 0.0f);
@@ -231,6 +229,15 @@ __device__ inline void write_onode(OctreeNode* o, uint64_t code, const float* cm
     q[6] = make_float4(__uint_as_float(ch[4]), __uint_as_float(ch[5]), __uint_as_float(ch[6]),
                        __uint_as_float(ch[7]));
     q[7] = make_float4(__uint_as_float(layer), 0.0f, 0.0f, 0.0f);
+}
+__device__ inline void write_onode(OctreeNode* o, uint64_t code, const float* cm,
+                                   uint32_t first, uint32_t nchild, uint32_t layer,
+                                   const float* S = nullptr) {
+    float4 f[8];
+    onode_fields(f, code, cm, first, nchild, layer, S);
+    float4* q = reinterpret_cast<float4*>(o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = f[k];
 }
 
 // Second moments about the node's float centre of mass P: with d = x - P,
@@ -800,16 +807,22 @@ void launch_aggregate_nodes(OctreeNode* nodes, const uint64_t* sorted, const uin
 
 // Octree.metal:253-269 (clearOctreeNodeBuffer), guarded.
 __global__ void k_clear_onodes(OctreeNode* nodes, uint64_t count) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+    // one thread per 16 B of the array: field k of an empty node
+    float4* d = reinterpret_cast<float4*>(nodes);
+    const float inv = __uint_as_float(0xFFFFFFFFu);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count * 8;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        float cm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        write_onode(&nodes[i], ASTRO_INVALID_MORTON_CODE, cm, 0, 0, 0);
+        const uint32_t k = (uint32_t)(i & 7u);
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (k == 0) v = make_float4(inv, inv, 0.0f, 0.0f);  // mortonCode = INVALID
+        if (k == 5 || k == 6) v = make_float4(inv, inv, inv, inv);  // children = INVALID
+        d[i] = v;
     }
 }
 void launch_clear_onodes(OctreeNode* nodes, uint64_t count, hipStream_t s) {
     if (!count) return;
-    uint64_t b = (count + 255) / 256;
-    if (b > 4096) b = 4096;
+    uint64_t b = (count * 8 + 255) / 256;
+    if (b > 16384) b = 16384;
     hipLaunchKernelGGL(k_clear_onodes, dim3((uint32_t)b), dim3(256), 0, s, nodes, count);
 }
 

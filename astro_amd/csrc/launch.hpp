@@ -142,6 +142,7 @@ struct ForceArgs {
     uint64_t node_total;        // nodes in the current layer table (index bound)
     uint32_t ns, nt;
     uint32_t block;             // traversal block size (64, 128, 256 or 512)
+    uint32_t xcd_chunk;         // tiles per run dealt to the XCDs in turn (0: contiguous eighths)
     AcceptRule acc;             // opening test (accept.hpp), from p.theta
     uint32_t fast;              // 1: eps2 >= max(r2_floor, 2^-96, leaf_cutoff_r2+),
                                 //    sqrt(eps2) >= r_floor: no floor or cut-off can bind

@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="warmup + steps only, no JSON extras (for rocprofv3 runs)")
+    ap.add_argument("--matched", action="store_true",
+                    help="time the accuracy-matched configuration (theta 0.5 + quadrupoles)")
+    ap.add_argument("--force-block", type=int, default=0, help="0: autotune")
+    ap.add_argument("--xcd-chunk", type=int, default=-1, help="-1: autotune")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=4)
@@ -115,7 +119,7 @@ def bench_batched(astro, a):
                        emit_octree_nodes=False) as ctx:
         ctx.set_params(P)
         ctx.upload_systems(pos, vel, col, off)
-        out["autotuned_block"], _ = ctx.autotune(3)
+        out["autotuned_block"], out["autotuned_xcd_chunk"], _ = ctx.autotune(3)
         ctx.step(a.dt, 2)
         ctx.sync()
         t0 = time.perf_counter()
@@ -293,6 +297,7 @@ def accuracy(astro, ctx, pos, vel, col, P, a):
     ctx.set_params(Q)
     ctx.upload(pos, vel, col)
     ctx.build_tree()
+    blk, chunk, _ = ctx.autotune(3)
     st = ctx.traversal_stats()
     ctx.step_timed(a.dt, 2)
     ctx.sync()
@@ -302,6 +307,7 @@ def accuracy(astro, ctx, pos, vel, col, P, a):
     wall = time.perf_counter() - t0
     ctx.check()
     out["matched"] = {"config": "theta 0.5, quadrupoles, otherwise the headline workload",
+                      "force_block": blk, "force_xcd_chunk": chunk,
                       "rms_rel_force_error": rows["theta0.5_quadrupole"],
                       "value": n * a.acc_steps / wall, "unit": "particle-steps/s",
                       "ms_per_step": wall / a.acc_steps * 1e3, "steps": a.acc_steps,
@@ -334,6 +340,7 @@ def bench_large(astro, a):
             ctx.compute_forces()
             ctx.check()
             err = rms_rel(ctx.forces()[tg], ref)
+            ctx.autotune(3)
             ctx.step(a.dt, 2)
             ctx.sync()
             t0 = time.perf_counter()
@@ -359,11 +366,19 @@ def main():
     n = len(pos)
     ctx = astro_amd.Context(n, bits_per_axis=21, emit_octree_nodes=True)
     P = astro_amd.params_physics(21)
+    if a.matched:
+        P.theta, P.multipole = 0.5, 2
     ctx.set_params(P)
     ctx.upload(pos, vel, col)
 
     ctx.build_tree()
     ctx.check()
+    if a.force_block and a.xcd_chunk >= 0:
+        tuned_block, tuned_chunk = a.force_block, a.xcd_chunk
+        ctx.set_force_block(tuned_block)
+        ctx.set_force_xcd_chunk(tuned_chunk)
+    else:
+        tuned_block, tuned_chunk, _ = ctx.autotune(3)
     V = int(ctx.layer_counts().astype(np.int64).sum())
     st0 = ctx.traversal_stats()
     ctx.step_timed(a.dt, max(a.warmup, 1))
@@ -377,8 +392,12 @@ def main():
     if a.profile_only:
         if a.calibrate_copy_mb:
             calibration_copy(a.calibrate_copy_mb << 20)
-        print(json.dumps({"wall_s": wall, "stage_ms": stage_ms}))
+        print(json.dumps({"wall_s": wall, "stage_ms": stage_ms, "force_block": tuned_block,
+                          "force_xcd_chunk": tuned_chunk}))
         return
+    if a.matched:
+        print(json.dumps({"error": "--matched is a profiling mode; use it with --profile-only"}))
+        sys.exit(2)
 
     st1 = ctx.traversal_stats()
     # PCIe-inclusive: host arrays in, one step, host arrays out
@@ -446,6 +465,7 @@ def main():
         "config": {"workload": "two_disks_500k", "n_particles": n, "theta": P.theta,
                    "eps2": P.eps2, "bits_per_axis": 21, "layers": 22, "dt": a.dt,
                    "fp64_top_layers": P.fp64_top_layers, "octree_nodes_emitted": True,
+                   "force_block": tuned_block, "force_xcd_chunk": tuned_chunk,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_force_compact", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

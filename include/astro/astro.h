@@ -164,9 +164,13 @@ int astro_step_timed(astro_ctx* ctx, float dt, uint32_t nsteps, float* stage_ms)
 int astro_traversal_stats(astro_ctx* ctx, astro_stats* out);
 /* Traversal block size (64, 128, 256 or 512; default 256). */
 int astro_set_force_block(astro_ctx* ctx, uint32_t block);
-/* Builds the tree, times the traversal at every block size (median of reps launches) and
- * keeps the fastest; returns it and its time. */
-int astro_autotune(astro_ctx* ctx, uint32_t reps, uint32_t* chosen, float* best_ms);
+/* How target tiles are dealt to the 8 XCDs: 0 (default) gives each XCD one contiguous eighth
+ * of the Morton curve; K > 0 deals runs of K consecutive tiles to the XCDs in turn. */
+int astro_set_force_xcd_chunk(astro_ctx* ctx, uint32_t chunk);
+/* Builds the tree, times the traversal for every block size x XCD chunk (median of reps
+ * launches each) and keeps the fastest; returns the choice and its time. */
+int astro_autotune(astro_ctx* ctx, uint32_t reps, uint32_t* chosen_block, uint32_t* chosen_chunk,
+                   float* best_ms);
 /* Builds the tree and returns, per system, 9 FP64 sums: mass, kinetic energy, potential
  * energy (1/2 sum m phi, phi from the same interactions as the forces, quadrupole terms
  * included when astro_params.multipole is 2), momentum x/y/z and mass-weighted position

@@ -168,12 +168,15 @@ def test_autotune_keeps_results(astro):
         ctx.build_tree()
         ctx.compute_forces()
         f0 = ctx.forces()
-        b, ms = ctx.autotune(3)
-        assert b in (64, 128, 256, 512) and ms > 0
-        for blk in (64, 128, 512):
+        b, k, ms = ctx.autotune(3)
+        assert b in (64, 128, 256, 512) and k in (0, 8, 32, 128) and ms > 0
+        for blk, chunk in ((64, 0), (128, 8), (512, 32), (256, 1), (256, 4096)):
             ctx.set_force_block(blk)
+            ctx.set_force_xcd_chunk(chunk)
             ctx.compute_forces()
             assert ctx.forces().tobytes() == f0.tobytes()
+        with pytest.raises(astro.AstroError):
+            ctx.set_force_xcd_chunk(4097)
         ctx.check()
 
 
