@@ -4,7 +4,8 @@ The compute path is libastro.so (hand-written HIP kernels behind a C-ABI, includ
 this package holds its ctypes binding and the reference-layout struct mirrors.
 """
 from .structs import (AstroParams, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_NODE,  # noqa
-                      VELOCITY_RADIUS, INVALID_CHILD, INVALID_MORTON_CODE, make_particles)
+                      VELOCITY_RADIUS, INVALID_CHILD, INVALID_MORTON_CODE, SNAPSHOT_VERSION,
+                      SnapshotHeader, make_particles)
 from .capi import (AstroError, Context, Server, load, device_count, params_compat,  # noqa
                    params_physics, ic_plummer, ic_two_disks, ic_plummer_background,
-                   ic_batch_sizes)
+                   ic_batch_sizes, snapshot_info)

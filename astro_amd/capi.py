@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .structs import (AstroParams, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_NODE,
+from .structs import (AstroParams, SnapshotHeader, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_NODE,
                       VELOCITY_RADIUS, ptr)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -97,6 +97,7 @@ SIGNATURES = [
     ("astro_aggregate_internal_nodes", _I, [_P, _U32, _U32, _U32, _U32]),
     ("astro_build_internal_nodes", _I, [_P, _U32]),
     ("astro_barnes_hut", _I, [_P]),
+    ("astro_lighting_pass", _I, [_P, _U32]),
     ("astro_update_spheres", _I, [_P, _F]),
     ("astro_build_tree", _I, [_P]),
     ("astro_compute_forces", _I, [_P]),
@@ -111,7 +112,11 @@ SIGNATURES = [
                             ctypes.POINTER(_F)]),
     ("astro_diagnostics", _I, [_P, _P]),
     ("astro_direct_forces_fp64", _I, [_P, _P, _U32, _P]),
+    ("astro_particle_count", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     ("astro_check_division", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
+    ("astro_snapshot_write", _I, [_P, ctypes.c_char_p, ctypes.c_double]),
+    ("astro_snapshot_read", _I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
+    ("astro_snapshot_info", _I, [ctypes.c_char_p, _P]),
     ("astro_device_bytes", _I, [_P, ctypes.POINTER(_U64)]),
     ("astro_server_create", _I, [ctypes.POINTER(ServerConfig), ctypes.POINTER(AstroParams),
                                  ctypes.POINTER(_P)]),
@@ -203,6 +208,16 @@ def ic_plummer_background(n, seed, bg_fraction=0.1, cube_side=20.0, slope=2.35, 
     return pos, vel, col
 
 
+def snapshot_info(path):
+    """Header of a snapshot file (host only): dict of its fields."""
+    h = SnapshotHeader()
+    rc = load().astro_snapshot_info(os.fsencode(path), ctypes.byref(h))
+    if rc != 0:
+        raise AstroError(rc, f"{path} is not a readable snapshot")
+    return dict(version=h.version, n_particles=h.n_particles, n_tree=h.n_tree,
+                n_systems=h.n_systems, time=h.time, params=h.params.copy())
+
+
 def ic_batch_sizes(n_systems, n_min, n_max, seed):
     out = np.zeros(n_systems, np.uint32)
     _check_ic(load().astro_ic_batch_sizes(n_systems, n_min, n_max, seed, ptr(out)),
@@ -286,11 +301,13 @@ class Context:
         self.n_tree = self.n
 
     def download(self):
+        self._sync_counts()
         pos, vel, col = _ic_arrays(self.n)
         self._chk(self.lib.astro_download(self.h, ptr(pos), ptr(vel), ptr(col)))
         return pos, vel, col
 
     def forces(self):
+        self._sync_counts()
         f = np.zeros((self.n, 4), np.float32)
         self._chk(self.lib.astro_download_forces(self.h, ptr(f)))
         return f
@@ -383,6 +400,9 @@ class Context:
     def barnes_hut(self):
         self._chk(self.lib.astro_barnes_hut(self.h))
 
+    def lighting_pass(self, num_stars):
+        self._chk(self.lib.astro_lighting_pass(self.h, num_stars))
+
     def update_spheres(self, dt):
         self._chk(self.lib.astro_update_spheres(self.h, dt))
 
@@ -435,6 +455,22 @@ class Context:
         out = np.zeros((len(t), 3), np.float64)
         self._chk(self.lib.astro_direct_forces_fp64(self.h, ptr(t), len(t), ptr(out)))
         return out
+
+    def snapshot_write(self, path, time=0.0):
+        self._chk(self.lib.astro_snapshot_write(self.h, os.fsencode(path), time))
+
+    def snapshot_read(self, path):
+        """Loads a snapshot (parameters and particles); returns its stored time."""
+        t = ctypes.c_double()
+        self._chk(self.lib.astro_snapshot_read(self.h, os.fsencode(path), ctypes.byref(t)))
+        self._sync_counts()
+        return t.value
+
+    def _sync_counts(self):
+        """Array lengths from the library (what astro_download writes)."""
+        n, nt = ctypes.c_uint32(), ctypes.c_uint32()
+        self._chk(self.lib.astro_particle_count(self.h, ctypes.byref(n), ctypes.byref(nt)))
+        self.n, self.n_tree = n.value, nt.value
 
     def check_division(self, n, seed=1):
         """Mismatches of the traversal's fast division against IEEE division on n pairs."""

@@ -90,6 +90,7 @@ extern "C" void astro_params_compat(astro_params* p) {
     p->fp64_top_layers = 0;
     p->kahan = 0;
     p->multipole = 0;
+    p->lighting_stars = 5;    // Renderer.swift:26 (numStars), BarnesHut.metal:103-167
 }
 
 extern "C" void astro_params_physics(astro_params* p, uint32_t bits_per_axis) {
@@ -105,6 +106,7 @@ extern "C" void astro_params_physics(astro_params* p, uint32_t bits_per_axis) {
     p->type_scale[0] = p->type_scale[1] = p->type_scale[2] = 1.0f;
     p->quirk_last_leaf = 0;
     p->fp64_top_layers = 4;
+    p->lighting_stars = 0;
 }
 
 extern "C" int astro_device_count(int* count) {
@@ -120,7 +122,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     invalidate_graph(c);
-    void* ptrs[] = {c->pos, c->vel, c->force, c->type, c->sys_of, c->codes, c->unsorted_codes,
+    void* ptrs[] = {c->pos, c->vel, c->color, c->force, c->type, c->sys_of, c->codes, c->unsorted_codes,
                     c->sorted_codes, c->tmp_keys_a, c->tmp_keys_b, c->unsorted_idx,
                     c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
@@ -184,6 +186,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(c->device_bytes, &c->pos, NP);
     e = e ? e : dalloc(c->device_bytes, &c->vel, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->color, NP);
     e = e ? e : dalloc(c->device_bytes, &c->force, NP);
     e = e ? e : dalloc(c->device_bytes, &c->type, NP);
     e = e ? e : dalloc(c->device_bytes, &c->sys_of, NP);
@@ -315,9 +318,8 @@ static int upload_particles(astro_ctx* c, const PositionMass* pos, const Velocit
     HIPCHK(c, hipSetDevice(c->device));
     // the whole upload's total bounds every system's total
     astro_mass_bounds(c, &pos[0].mass, n, sizeof(PositionMass) / sizeof(float), true);
-    std::vector<float4> p4(n), v4(n);
+    std::vector<float4> p4(n), v4(n), c4(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     std::vector<uint32_t> ty(n, 0u);
-    c->colors.assign(4ull * n, 0.0f);
     for (uint32_t i = 0; i < n; i++) {
         p4[i] = make_float4(pos[i].position[0], pos[i].position[1], pos[i].position[2],
                             pos[i].mass);
@@ -328,10 +330,12 @@ static int upload_particles(astro_ctx* c, const PositionMass* pos, const Velocit
             v4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (col) {
             ty[i] = col[i].type;
-            for (int k = 0; k < 4; k++) c->colors[4ull * i + k] = col[i].color[k];
+            c4[i] = make_float4(col[i].color[0], col[i].color[1], col[i].color[2], col[i].color[3]);
         }
     }
     if (n) {
+        HIPCHK(c, hipMemcpyAsync(c->color, c4.data(), n * sizeof(float4), hipMemcpyHostToDevice,
+                                 c->stream));
         HIPCHK(c, hipMemcpyAsync(c->pos, p4.data(), n * sizeof(float4), hipMemcpyHostToDevice,
                                  c->stream));
         HIPCHK(c, hipMemcpyAsync(c->vel, v4.data(), n * sizeof(float4), hipMemcpyHostToDevice,
@@ -397,14 +401,23 @@ extern "C" int astro_system_count(const astro_ctx* c, uint32_t* n_systems) {
     return ASTRO_OK;
 }
 
+extern "C" int astro_particle_count(const astro_ctx* c, uint32_t* n_particles, uint32_t* n_tree) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (n_particles) *n_particles = c->ns;
+    if (n_tree) *n_tree = c->nt;
+    return ASTRO_OK;
+}
+
 extern "C" int astro_download(astro_ctx* c, PositionMass* pos, VelocityRadius* vel,
                               ColorType* col) {
     if (!c) return ASTRO_ERR_ARG;
     const uint32_t n = c->ns;
-    std::vector<float4> p4(n), v4(n);
+    std::vector<float4> p4(n), v4(n), c4(n);
     std::vector<uint32_t> ty(n);
     if (n) {
         HIPCHK(c, hipMemcpyAsync(p4.data(), c->pos, n * sizeof(float4), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(c4.data(), c->color, n * sizeof(float4), hipMemcpyDeviceToHost,
                                  c->stream));
         HIPCHK(c, hipMemcpyAsync(v4.data(), c->vel, n * sizeof(float4), hipMemcpyDeviceToHost,
                                  c->stream));
@@ -429,7 +442,10 @@ extern "C" int astro_download(astro_ctx* c, PositionMass* pos, VelocityRadius* v
         }
         if (col) {
             std::memset(&col[i], 0, sizeof(ColorType));
-            for (int k = 0; k < 4; k++) col[i].color[k] = c->colors[4ull * i + k];
+            col[i].color[0] = c4[i].x;
+            col[i].color[1] = c4[i].y;
+            col[i].color[2] = c4[i].z;
+            col[i].color[3] = c4[i].w;
             col[i].type = ty[i];
         }
     }
@@ -789,6 +805,19 @@ extern "C" int astro_barnes_hut(astro_ctx* c) {
     return ASTRO_OK;
 }
 
+static void enqueue_lighting(astro_ctx* c, uint32_t stars) {
+    launch_lighting(c->pos, c->color, c->type, c->ns, stars, c->nsys > 1 ? c->sys_of : nullptr,
+                    c->d_sys_off, c->stream);
+}
+
+extern "C" int astro_lighting_pass(astro_ctx* c, uint32_t num_stars) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    enqueue_lighting(c, num_stars);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
 extern "C" int astro_update_spheres(astro_ctx* c, float dt) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
@@ -914,6 +943,8 @@ static int enqueue_step(astro_ctx* c, float dt) {
     int rc;
     if ((rc = enqueue_build(c))) return rc;
     if ((rc = enqueue_forces(c, nullptr))) return rc;
+    // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
+    if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
     launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -974,6 +1005,7 @@ extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* 
         hipEventRecord(E[3], c->stream);
         if (!rc) rc = enqueue_forces(c, nullptr);
         hipEventRecord(E[4], c->stream);
+        if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
         launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
         hipEventRecord(E[5], c->stream);
     }
@@ -1166,6 +1198,86 @@ extern "C" int astro_check_division(astro_ctx* c, uint64_t n, uint64_t seed, uin
     HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *mismatches = h;
+    return ASTRO_OK;
+}
+
+// ------------------------------------------------------------------------- snapshots
+namespace {
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+int read_header(FILE* f, astro_snapshot_header* h) {
+    if (std::fread(h, sizeof(*h), 1, f) != 1) return ASTRO_ERR_ARG;
+    if (std::memcmp(h->magic, "ASTROSNP", 8) != 0 || h->version != ASTRO_SNAPSHOT_VERSION ||
+        h->n_systems == 0 || h->n_tree > h->n_particles)
+        return ASTRO_ERR_ARG;
+    return ASTRO_OK;
+}
+}  // namespace
+
+extern "C" int astro_snapshot_info(const char* path, astro_snapshot_header* out) {
+    if (!path || !out) return ASTRO_ERR_ARG;
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) return ASTRO_ERR_ARG;
+    return read_header(fc.f, out);
+}
+
+extern "C" int astro_snapshot_write(astro_ctx* c, const char* path, double time) {
+    if (!c || !path) return ASTRO_ERR_ARG;
+    if (int rc = need_particles(c)) return rc;
+    const uint32_t n = c->ns;
+    std::vector<PositionMass> pos(n);
+    std::vector<VelocityRadius> vel(n);
+    std::vector<ColorType> col(n);
+    if (int rc = astro_download(c, pos.data(), vel.data(), col.data())) return rc;
+    astro_snapshot_header h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, "ASTROSNP", 8);
+    h.version = ASTRO_SNAPSHOT_VERSION;
+    h.n_particles = n;
+    h.n_tree = c->nt;
+    h.n_systems = c->nsys;
+    h.time = time;
+    h.params = c->params;
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) return fail(c, ASTRO_ERR_ARG, std::string("cannot open ") + path);
+    const bool ok = std::fwrite(&h, sizeof(h), 1, fc.f) == 1 &&
+                    std::fwrite(c->sys_off.data(), sizeof(uint32_t), c->nsys + 1, fc.f) ==
+                        c->nsys + 1 &&
+                    std::fwrite(pos.data(), sizeof(PositionMass), n, fc.f) == n &&
+                    std::fwrite(vel.data(), sizeof(VelocityRadius), n, fc.f) == n &&
+                    std::fwrite(col.data(), sizeof(ColorType), n, fc.f) == n;
+    if (!ok) return fail(c, ASTRO_ERR_ARG, std::string("short write to ") + path);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_snapshot_read(astro_ctx* c, const char* path, double* time) {
+    if (!c || !path) return ASTRO_ERR_ARG;
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) return fail(c, ASTRO_ERR_ARG, std::string("cannot open ") + path);
+    astro_snapshot_header h;
+    if (read_header(fc.f, &h)) return fail(c, ASTRO_ERR_ARG, "not a snapshot of this version");
+    const uint32_t n = h.n_particles;
+    std::vector<uint32_t> off(h.n_systems + 1ull);
+    std::vector<PositionMass> pos(n);
+    std::vector<VelocityRadius> vel(n);
+    std::vector<ColorType> col(n);
+    const bool ok = std::fread(off.data(), sizeof(uint32_t), off.size(), fc.f) == off.size() &&
+                    std::fread(pos.data(), sizeof(PositionMass), n, fc.f) == n &&
+                    std::fread(vel.data(), sizeof(VelocityRadius), n, fc.f) == n &&
+                    std::fread(col.data(), sizeof(ColorType), n, fc.f) == n;
+    if (!ok) return fail(c, ASTRO_ERR_ARG, "truncated snapshot");
+    if (int rc = astro_set_params(c, &h.params)) return rc;
+    int rc;
+    if (h.n_systems == 1)
+        rc = astro_upload(c, pos.data(), vel.data(), col.data(), n, h.n_tree);
+    else
+        rc = astro_upload_systems(c, pos.data(), vel.data(), col.data(), off.data(), h.n_systems);
+    if (rc) return rc;
+    if (time) *time = h.time;
     return ASTRO_OK;
 }
 

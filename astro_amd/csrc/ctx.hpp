@@ -21,10 +21,10 @@ struct astro_ctx {
 
     uint32_t ns = 0, nt = 0, nsys = 1;
     std::vector<uint32_t> sys_off;     // nsys + 1 particle offsets
-    std::vector<float> colors;         // host copy of ColorType.color (render-only data)
 
     // particle state (SoA float4: 16-B rows instead of the 32-B reference structs)
     float4 *pos = nullptr, *vel = nullptr, *force = nullptr;
+    float4* color = nullptr;           // ColorType.color (lit by the lighting pass)
     uint32_t* type = nullptr;
     uint32_t* sys_of = nullptr;
     // keys and sort

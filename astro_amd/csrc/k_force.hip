@@ -545,6 +545,49 @@ void launch_diag(const float4* pos, const float4* vel, const double* phi,
                        chunks, partial);
 }
 
+// BarnesHut.metal:103-167 (lightingPass): planets (type 1) and dust (type 2) take the light of
+// the first min(nstars, 5) particles of their system; stars keep their colours.  Same order of
+// operations as the oracle.
+__global__ __launch_bounds__(256) void k_lighting(const float4* __restrict__ pos,
+                                                  float4* __restrict__ color,
+                                                  const uint32_t* __restrict__ type, uint32_t n,
+                                                  uint32_t nstars, const uint32_t* __restrict__ sys_of,
+                                                  const uint32_t* __restrict__ sys_off) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = type[i];
+    if (t != 1u && t != 2u) return;
+    uint32_t b = 0, e = n;
+    if (sys_of) {
+        const uint32_t s = sys_of[i];
+        b = sys_off[s];
+        e = sys_off[s + 1];
+    }
+    const float4 p = pos[i];
+    const uint32_t kmax = nstars < 5u ? nstars : 5u;
+    float r = 0.0f, g = 0.0f, bl = 0.0f;
+    for (uint32_t k = 0; k < kmax && b + k < e; k++) {
+        const float4 sp = pos[b + k];
+        const float4 sc = color[b + k];
+        const float dx = sp.x - p.x, dy = sp.y - p.y, dz = sp.z - p.z;
+        const float dist = sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
+        const float q = fmaxf(dist / ASTRO_LIGHT_ATTENUATION_DISTANCE, 1e-9f);
+        const float att = 1.0f / (q * q);
+        r = r + sc.x * att;
+        g = g + sc.y * att;
+        bl = bl + sc.z * att;
+    }
+    const float a = t == 2u ? fminf(fmaxf(r, fmaxf(g, bl)), 0.25f) : 1.0f;
+    color[i] = make_float4(r, g, bl, a);
+}
+void launch_lighting(const float4* pos, float4* color, const uint32_t* type, uint32_t n,
+                     uint32_t nstars, const uint32_t* sys_of, const uint32_t* sys_off,
+                     hipStream_t s) {
+    if (!n || !nstars) return;
+    hipLaunchKernelGGL(k_lighting, dim3((n + 255) / 256), dim3(256), 0, s, pos, color, type, n,
+                       nstars, sys_of, sys_off);
+}
+
 // FP64 direct sum for accuracy audits: a_i = sum_j G m_j d / (|d|^2 + eps2)^(3/2), d = x_j - x_i,
 // over the sources of target i's system (the self term has d = 0 and adds nothing).  Block
 // (chunk, t) sums one chunk of the sources for target t; partial[(t * chunks + chunk) * 3 + k].

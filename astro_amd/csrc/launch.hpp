@@ -161,6 +161,9 @@ void launch_force_reference(const ForceArgs& a, hipStream_t s);
 void launch_diag(const float4* pos, const float4* vel, const double* phi,
                  const uint32_t* d_sys_off, uint32_t nsys, uint32_t chunks, double* partial,
                  hipStream_t s);
+void launch_lighting(const float4* pos, float4* color, const uint32_t* type, uint32_t n,
+                     uint32_t nstars, const uint32_t* sys_of, const uint32_t* sys_off,
+                     hipStream_t s);
 void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,
                    uint32_t n, float dt, const astro_params& p, hipStream_t s);
 

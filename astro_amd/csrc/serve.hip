@@ -86,6 +86,7 @@ extern "C" int astro_server_create(const astro_server_config* cfg, const astro_p
                                    astro_server** out) {
     if (!cfg || !p || !out || cfg->max_particles == 0 || cfg->max_systems == 0)
         return ASTRO_ERR_ARG;
+    if (p->lighting_stars) return ASTRO_ERR_ARG;  // requests carry no colours through the batch
     *out = nullptr;
     auto* s = new astro_server();
     s->cfg = *cfg;

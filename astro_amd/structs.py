@@ -56,6 +56,7 @@ class AstroParams(ctypes.Structure):
         ("fp64_top_layers", ctypes.c_uint32),
         ("kahan", ctypes.c_uint32),
         ("multipole", ctypes.c_uint32),
+        ("lighting_stars", ctypes.c_uint32),
     ]
 
     def copy(self):
@@ -69,6 +70,22 @@ class AstroParams(ctypes.Structure):
             v = getattr(self, name)
             d[name] = list(v) if name == "type_scale" else v
         return d
+
+
+class SnapshotHeader(ctypes.Structure):
+    """astro_snapshot_header (include/astro/astro.h)."""
+    _fields_ = [
+        ("magic", ctypes.c_char * 8),
+        ("version", ctypes.c_uint32),
+        ("n_particles", ctypes.c_uint32),
+        ("n_tree", ctypes.c_uint32),
+        ("n_systems", ctypes.c_uint32),
+        ("time", ctypes.c_double),
+        ("params", AstroParams),
+    ]
+
+
+SNAPSHOT_VERSION = 1
 
 
 def make_particles(n):

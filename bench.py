@@ -108,9 +108,10 @@ def bench_batched(astro, a):
     graph replay vs the plain loop (each system in its own context, stepped in turn)."""
     sizes = astro.ic_batch_sizes(a.batch_systems, 1024, 65536, 7)
     parts = [astro.ic_plummer(int(n), 1000 + k) for k, n in enumerate(sizes)]
-    pos = np.concatenate([p[0] for p in parts])
-    vel = np.concatenate([p[1] for p in parts])
-    col = np.concatenate([p[2] for p in parts])
+    # np.concatenate drops the structs' padding: back to the 32-B reference layouts
+    pos = np.ascontiguousarray(np.concatenate([p[0] for p in parts]), astro.POSITION_MASS)
+    vel = np.ascontiguousarray(np.concatenate([p[1] for p in parts]), astro.VELOCITY_RADIUS)
+    col = np.ascontiguousarray(np.concatenate([p[2] for p in parts]), astro.COLOR_TYPE)
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
     ntot = int(off[-1])
     P = astro.params_physics(21)

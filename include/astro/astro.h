@@ -99,6 +99,8 @@ int astro_upload(astro_ctx* ctx, const PositionMass* pos, const VelocityRadius* 
 int astro_upload_systems(astro_ctx* ctx, const PositionMass* pos, const VelocityRadius* vel,
                          const ColorType* col, const uint32_t* offsets, uint32_t n_systems);
 int astro_system_count(const astro_ctx* ctx, uint32_t* n_systems);
+/* Particles of the current state (the length astro_download writes) and tree sources. */
+int astro_particle_count(const astro_ctx* ctx, uint32_t* n_particles, uint32_t* n_tree);
 int astro_download(astro_ctx* ctx, PositionMass* pos, VelocityRadius* vel, ColorType* col);
 int astro_download_forces(astro_ctx* ctx, float* force4); /* float3 in 16-B slots */
 int astro_get_buffer(astro_ctx* ctx, int which, void** d_ptr, uint64_t* bytes);
@@ -139,6 +141,9 @@ int astro_aggregate_internal_nodes(astro_ctx* ctx, uint32_t layer, uint32_t inpu
 int astro_build_internal_nodes(astro_ctx* ctx, uint32_t count);
 /* Renderer.swift:851-869 -> BarnesHut.metal:22-101, over the OctreeNode array */
 int astro_barnes_hut(astro_ctx* ctx);
+/* Renderer.swift:871-883 -> BarnesHut.metal:103-167: planets and dust take the light of the
+ * first min(num_stars, 5) particles (of their system); colours only */
+int astro_lighting_pass(astro_ctx* ctx, uint32_t num_stars);
 /* Renderer.swift:885-899 -> BarnesHut.metal:169-196 */
 int astro_update_spheres(astro_ctx* ctx, float dt);
 
@@ -147,7 +152,8 @@ int astro_update_spheres(astro_ctx* ctx, float dt);
 int astro_build_tree(astro_ctx* ctx);
 /* Renderer.swift:536-538 over the compact tree: forces for every particle. */
 int astro_compute_forces(astro_ctx* ctx);
-/* Renderer.swift:495-542 with explicit dt: build + forces + update, nsteps times. */
+/* Renderer.swift:495-542 with explicit dt: build + forces (+ the lighting pass when
+ * astro_params.lighting_stars > 0) + update, nsteps times. */
 int astro_step(astro_ctx* ctx, float dt, uint32_t nsteps);
 /* Same as astro_step with hipEvents around each stage of each step; stage_ms receives the
  * per-stage sums (ASTRO_STAGE_COUNT floats). */
@@ -210,6 +216,7 @@ typedef struct {
     uint32_t admitted, completed, active, queued, particles, active_after;
     double step_ms;
 } astro_tick_info;
+/* p->lighting_stars must be 0: requests carry positions and velocities only. */
 int astro_server_create(const astro_server_config* cfg, const astro_params* p,
                         astro_server** out);
 int astro_server_destroy(astro_server* s);
@@ -223,6 +230,28 @@ int astro_server_poll(astro_server* s, astro_completion* out, uint32_t max, uint
 int astro_server_fetch(astro_server* s, uint64_t id, PositionMass* pos, VelocityRadius* vel);
 int astro_server_counters(const astro_server* s, uint64_t* ticks, uint64_t* particle_steps,
                           uint32_t* queued, uint32_t* active);
+
+/* ---- snapshots: the state of a context in the reference byte layout, for checkpoint and
+ *      resume (no reference counterpart: the reference keeps no state between runs).
+ *      File: this header, then uint32 offsets[n_systems + 1], PositionMass[n],
+ *      VelocityRadius[n], ColorType[n], all little-endian. ---------------------------------- */
+#define ASTRO_SNAPSHOT_VERSION 1u
+typedef struct {
+    char magic[8];            /* "ASTROSNP"                                                  */
+    uint32_t version;         /* ASTRO_SNAPSHOT_VERSION                                      */
+    uint32_t n_particles;
+    uint32_t n_tree;          /* tree sources (the first n_tree particles)                   */
+    uint32_t n_systems;       /* 1 for astro_upload, S for astro_upload_systems              */
+    double time;              /* the caller's simulation time                                */
+    astro_params params;
+} astro_snapshot_header;
+/* Synchronises and writes the context's particles, systems and parameters. */
+int astro_snapshot_write(astro_ctx* ctx, const char* path, double time);
+/* Replaces the context's parameters and particles by the snapshot's; *time (optional)
+ * receives the stored time.  Stepping on reproduces the run that wrote it bit for bit. */
+int astro_snapshot_read(astro_ctx* ctx, const char* path, double* time);
+/* Host only: reads and checks a snapshot's header. */
+int astro_snapshot_info(const char* path, astro_snapshot_header* out);
 
 /* ---- seeded initial conditions (host only; replace Renderer.makeSpheres,
  *      Renderer.swift:441-493, whose draws are unseeded) -------------------------------- */

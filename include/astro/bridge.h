@@ -102,7 +102,13 @@ typedef struct {
     uint32_t kahan;           /* 1: Kahan-compensated force accumulation                     */
     uint32_t multipole;       /* 0: monopoles (the reference); 2: + quadrupoles of accepted    *
                                * internal nodes (second moments kept per node)                  */
+    uint32_t lighting_stars;  /* stars lighting the other particles in the step's lighting     *
+                               * pass (BarnesHut.metal:103-167; the first particles, at most 5);*
+                               * 0: no lighting pass                                            */
 } astro_params;
+
+/* BarnesHut.metal:121 (bridge.h:98-100) */
+#define ASTRO_LIGHT_ATTENUATION_DISTANCE 4e5f
 
 #ifdef __cplusplus
 }

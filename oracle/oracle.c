@@ -43,6 +43,7 @@ void orc_params_compat(astro_params* p) {
     p->fp64_top_layers = 0;
     p->kahan = 0;
     p->multipole = 0;
+    p->lighting_stars = 5;      /* Renderer.swift:26 (numStars) */
 }
 
 /* The scaled profile for self-gravitating systems (G=1, exact cells, no cut-offs). */
@@ -59,6 +60,7 @@ void orc_params_physics(astro_params* p, uint32_t bits_per_axis) {
     p->type_scale[0] = p->type_scale[1] = p->type_scale[2] = 1.0f;
     p->quirk_last_leaf = 0;
     p->fp64_top_layers = 4;
+    p->lighting_stars = 0;
 }
 
 /* MortonCode.metal:7-23 -- bit i of x/y/z goes to bit 3i/3i+1/3i+2 of the key. */
@@ -582,8 +584,41 @@ void orc_update_spheres(PositionMass* pos, VelocityRadius* vel, const float* for
     }
 }
 
-/* Renderer.swift:495-542 (the compute part of draw(in:)) with an explicit dt. */
-int orc_step(PositionMass* pos, VelocityRadius* vel, const ColorType* types, uint32_t ns,
+/* BarnesHut.metal:103-167 (lightingPass): every planet (type 1) and dust particle (type 2)
+ * takes the light of the first min(num_stars, 5) particles (the stars) of its system:
+ * rgb = sum color_k * att_k with att = 1 / n^2, n = max(|x_k - x| / 4e5, 1e-9); alpha = 1 for
+ * planets, min(max(r, g, b), 0.25) for dust.  |d| = sqrt(fma(dz,dz, fma(dy,dy, dx*dx))); the
+ * light sum adds each product separately.  Stars beyond the system's end are skipped (the
+ * reference would read past its particle range).  sys_off: n_sys + 1 offsets, or NULL for
+ * one system of n. */
+void orc_lighting_pass(const PositionMass* pos, ColorType* col, uint32_t n, uint32_t num_stars,
+                       const uint32_t* sys_off, uint32_t n_sys) {
+    const uint32_t k_max = num_stars < 5u ? num_stars : 5u;
+    for (uint32_t s = 0; s < (sys_off ? n_sys : 1u); s++) {
+        const uint32_t b = sys_off ? sys_off[s] : 0u, e = sys_off ? sys_off[s + 1] : n;
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t type = col[i].type;
+            if (type != 1u && type != 2u) continue;
+            float tot[3] = {0.0f, 0.0f, 0.0f};
+            for (uint32_t k = 0; k < k_max && b + k < e; k++) {
+                const uint32_t si = b + k;
+                const float dx = pos[si].position[0] - pos[i].position[0];
+                const float dy = pos[si].position[1] - pos[i].position[1];
+                const float dz = pos[si].position[2] - pos[i].position[2];
+                const float dist = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+                const float q = fmaxf(dist / ASTRO_LIGHT_ATTENUATION_DISTANCE, 1e-9f);
+                const float att = 1.0f / (q * q);
+                for (int c = 0; c < 3; c++) tot[c] = tot[c] + col[si].color[c] * att;
+            }
+            for (int c = 0; c < 3; c++) col[i].color[c] = tot[c];
+            col[i].color[3] = type == 2u ? fminf(fmaxf(tot[0], fmaxf(tot[1], tot[2])), 0.25f) : 1.0f;
+        }
+    }
+}
+
+/* Renderer.swift:495-542 (the compute part of draw(in:)) with an explicit dt: build, forces,
+ * lighting pass when p->lighting_stars > 0 (col is then updated), integrator. */
+int orc_step(PositionMass* pos, VelocityRadius* vel, ColorType* types, uint32_t ns,
              uint32_t nt, const astro_params* p, float dt, float* force4, orc_stats* st,
              int threads, OctreeNode* nodes_out, orc_layers* L_out) {
     orc_layers L;
@@ -602,6 +637,7 @@ int orc_step(PositionMass* pos, VelocityRadius* vel, const ColorType* types, uin
                            ns, force4, st, threads);
         else
             memset(force4, 0, sizeof(float) * 4 * (size_t)ns);
+        if (p->lighting_stars > 0 && types) orc_lighting_pass(pos, types, ns, p->lighting_stars, NULL, 1);
         orc_update_spheres(pos, vel, force4, types, ns, dt, p);
     }
     if (L_out) *L_out = L;

@@ -69,7 +69,9 @@ void orc_barnes_hut(const OctreeNode* nodes, uint64_t root, const PositionMass* 
 void orc_update_spheres(PositionMass* pos, VelocityRadius* vel, const float* force4,
                         const ColorType* types, uint32_t n, float dt, const astro_params* p);
 
-int orc_step(PositionMass* pos, VelocityRadius* vel, const ColorType* types, uint32_t ns,
+void orc_lighting_pass(const PositionMass* pos, ColorType* col, uint32_t n, uint32_t num_stars,
+                       const uint32_t* sys_off, uint32_t n_sys);
+int orc_step(PositionMass* pos, VelocityRadius* vel, ColorType* types, uint32_t ns,
              uint32_t nt, const astro_params* p, float dt, float* force4, orc_stats* st,
              int threads, OctreeNode* nodes_out, orc_layers* L_out);
 

@@ -45,6 +45,7 @@ SIGS = [
     ("orc_barnes_hut", None, [_P, _U64, _P, _P, _P, _U32, _U32, _PP, _F, _U32, _U32, _P,
                               ctypes.POINTER(OrcStats), _I]),
     ("orc_update_spheres", None, [_P, _P, _P, _P, _U32, _F, _PP]),
+    ("orc_lighting_pass", None, [_P, _P, _U32, _U32, _P, _U32]),
     ("orc_step", _I, [_P, _P, _P, _U32, _U32, _PP, _F, _P, ctypes.POINTER(OrcStats), _I, _P,
                       ctypes.POINTER(OrcLayers)]),
     ("orc_direct_sum_fp64", None, [_P, _U32, _P, _U32, _D, _D, _P, _P, _I]),
@@ -52,6 +53,21 @@ SIGS = [
 ]
 
 _lib = None
+
+
+def _pm(a, copy=False):
+    """PositionMass array in the padded C layout (numpy drops padding when concatenating)."""
+    return np.array(a, POSITION_MASS) if copy else np.ascontiguousarray(a, POSITION_MASS)
+
+
+def _vr(a, copy=False):
+    return np.array(a, VELOCITY_RADIUS) if copy else np.ascontiguousarray(a, VELOCITY_RADIUS)
+
+
+def _ct(a, copy=False):
+    if a is None:
+        return None
+    return np.array(a, COLOR_TYPE) if copy else np.ascontiguousarray(a, COLOR_TYPE)
 
 
 def lib():
@@ -85,6 +101,7 @@ def interleave(x, y, z):
 
 
 def bbox_cube(pos, n=None):
+    pos = _pm(pos)
     n = len(pos) if n is None else n
     cube = np.zeros(4, np.float32)
     lib().orc_bbox_cube(ptr(pos), n, ptr(cube))
@@ -92,6 +109,7 @@ def bbox_cube(pos, n=None):
 
 
 def morton_codes(pos, params, cube=None, n=None):
+    pos = _pm(pos)
     n = len(pos) if n is None else n
     if cube is None:
         cube = bbox_cube(pos, n) if params.quantizer == 1 else np.array([0, 0, 0, 1], np.float32)
@@ -140,6 +158,7 @@ def layer_table(nt, bits):
 
 def build_tree(pos, params, n_tree=None, cube=None, resort_layers=False):
     """Returns dict(nodes, cap, off, count, codes, sorted_codes, sorted_idx, cube, status)."""
+    pos = _pm(pos)
     nt = len(pos) if n_tree is None else n_tree
     if cube is None:
         cube = bbox_cube(pos, nt) if params.quantizer == 1 else np.array([0, 0, 0, 1], np.float32)
@@ -162,6 +181,7 @@ def build_tree(pos, params, n_tree=None, cube=None, resort_layers=False):
 
 def barnes_hut(tree, pos, col, params, n_tree=None, targets=None, threads=0):
     """Forces (n, 4) for targets [t0, t1) (default: all); returns (force4, stats dict)."""
+    pos, col = _pm(pos), _ct(col)
     ns = len(pos)
     nt = ns if n_tree is None else n_tree
     t0, t1 = (0, ns) if targets is None else targets
@@ -176,17 +196,25 @@ def barnes_hut(tree, pos, col, params, n_tree=None, targets=None, threads=0):
 
 
 def update_spheres(pos, vel, force4, col, dt, params):
-    pos = pos.copy()
-    vel = vel.copy()
+    pos, vel, col = _pm(pos, True), _vr(vel, True), _ct(col)
     lib().orc_update_spheres(ptr(pos), ptr(vel), ptr(np.ascontiguousarray(force4, np.float32)),
                              ptr(col), len(pos), dt, ctypes.byref(params))
     return pos, vel
 
 
-def step(pos, vel, col, params, dt, n_tree=None, threads=0):
-    """One reference step on copies; returns (pos, vel, force4, stats)."""
-    pos = pos.copy()
-    vel = vel.copy()
+def lighting_pass(pos, col, num_stars, sys_off=None):
+    """BarnesHut.metal:103-167 on a copy of col; returns the lit colours."""
+    pos, col = _pm(pos), _ct(col, True)
+    off = None if sys_off is None else np.ascontiguousarray(sys_off, np.uint32)
+    lib().orc_lighting_pass(ptr(pos), ptr(col), len(pos), num_stars, ptr(off),
+                            0 if off is None else len(off) - 1)
+    return col
+
+
+def step(pos, vel, col, params, dt, n_tree=None, threads=0, return_col=False):
+    """One reference step on copies; returns (pos, vel, force4, stats), or
+    (pos, vel, col, force4, stats) with return_col (the lighting pass updates col)."""
+    pos, vel, col = _pm(pos, True), _vr(vel, True), _ct(col, True)
     ns = len(pos)
     nt = ns if n_tree is None else n_tree
     f = np.zeros((ns, 4), np.float32)
@@ -195,11 +223,15 @@ def step(pos, vel, col, params, dt, n_tree=None, threads=0):
                         ctypes.byref(st), threads, None, None)
     if rc not in (0,):
         raise RuntimeError(f"orc_step returned {rc}")
-    return pos, vel, f, dict(visits=st.visits, leaf_terms=st.leaf_terms,
-                             monopole_terms=st.monopole_terms, opened=st.opened)
+    stats = dict(visits=st.visits, leaf_terms=st.leaf_terms, monopole_terms=st.monopole_terms,
+                 opened=st.opened)
+    if return_col:
+        return pos, vel, col, f, stats
+    return pos, vel, f, stats
 
 
 def direct_sum_fp64(pos, targets, G=1.0, eps2=0.01, skip_keys=None, threads=0):
+    pos = _pm(pos)
     t = np.ascontiguousarray(targets, np.uint32)
     out = np.zeros((len(t), 3), np.float64)
     sk = None if skip_keys is None else np.ascontiguousarray(skip_keys, np.uint64)
@@ -209,6 +241,7 @@ def direct_sum_fp64(pos, targets, G=1.0, eps2=0.01, skip_keys=None, threads=0):
 
 
 def energy_momentum(pos, vel):
+    pos, vel = _pm(pos), _vr(vel)
     out = np.zeros(7, np.float64)
     lib().orc_energy_momentum(ptr(pos), ptr(vel), len(pos), ptr(out))
     return out

@@ -48,6 +48,8 @@ int main(void) {
   O(OctreeNode, layer);
   S(astro_params); O(astro_params, theta); O(astro_params, type_scale); O(astro_params, kahan);
   S(astro_config); S(astro_stats);
+  S(astro_snapshot_header); O(astro_snapshot_header, version); O(astro_snapshot_header, time);
+  O(astro_snapshot_header, params);
   return 0;
 }
 """
@@ -77,6 +79,10 @@ def test_struct_layouts_match_header(tmp_path, astro):
     assert vals["astro_params.kahan"] == structs.AstroParams.kahan.offset
     assert vals["astro_config"] == ctypes.sizeof(AstroConfig)
     assert vals["astro_stats"] == ctypes.sizeof(AstroStats)
+    H = structs.SnapshotHeader
+    assert vals["astro_snapshot_header"] == ctypes.sizeof(H)
+    for f in ("version", "time", "params"):
+        assert vals[f"astro_snapshot_header.{f}"] == getattr(H, f).offset, f
 
 
 def test_params_profiles_match_oracle(astro, orc):
@@ -111,3 +117,28 @@ def test_create_fails_cleanly_without_gpu(astro):
         pytest.skip("a GPU is present")
     with pytest.raises(astro.AstroError):
         astro.Context(1024, bits_per_axis=7)
+
+
+def test_snapshot_info_reads_the_documented_format(tmp_path, astro):
+    # a snapshot written from Python by the format of astro.h: header, offsets, the three
+    # reference-layout arrays
+    from astro_amd import structs
+    n = 10
+    h = structs.SnapshotHeader()
+    h.magic = b"ASTROSNP"
+    h.version = structs.SNAPSHOT_VERSION
+    h.n_particles, h.n_tree, h.n_systems, h.time = n, 7, 1, 2.5
+    h.params = astro.params_physics(21)
+    pos, vel, col = structs.make_particles(n)
+    path = tmp_path / "s.snap"
+    with open(path, "wb") as f:
+        f.write(bytes(h))
+        f.write(np.array([0, n], np.uint32).tobytes())
+        f.write(pos.tobytes() + vel.tobytes() + col.tobytes())
+    info = astro.snapshot_info(path)
+    assert (info["n_particles"], info["n_tree"], info["n_systems"], info["time"]) == (n, 7, 1, 2.5)
+    assert info["params"].as_dict() == astro.params_physics(21).as_dict()
+    bad = tmp_path / "bad.snap"
+    bad.write_bytes(b"NOTASNAP" + bytes(h)[8:])
+    with pytest.raises(astro.AstroError):
+        astro.snapshot_info(bad)

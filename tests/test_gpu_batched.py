@@ -16,10 +16,11 @@ def say(*a):
 
 
 def make_batch(astro, sizes, seed0=100):
+    # np.concatenate drops the structs' padding: convert back to the 32-B reference layouts
     parts = [astro.ic_plummer(int(n), seed0 + k) for k, n in enumerate(sizes)]
-    pos = np.concatenate([p[0] for p in parts])
-    vel = np.concatenate([p[1] for p in parts])
-    col = np.concatenate([p[2] for p in parts])
+    pos = np.ascontiguousarray(np.concatenate([p[0] for p in parts]), astro.POSITION_MASS)
+    vel = np.ascontiguousarray(np.concatenate([p[1] for p in parts]), astro.VELOCITY_RADIUS)
+    col = np.ascontiguousarray(np.concatenate([p[2] for p in parts]), astro.COLOR_TYPE)
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
     return parts, pos, vel, col, off
 
@@ -211,3 +212,73 @@ def test_direct_fp64_matches_oracle(astro, orc):
         k = int(np.searchsorted(off, t, side="right") - 1)
         want = orc.direct_sum_fp64(parts[k][0], [t - off[k]], 1.0, eps2)[0]
         assert np.abs(g - want).max() <= 1e-12 * max(np.linalg.norm(want), 1e-300), (t, g, want)
+
+
+def test_batched_lighting_per_system(astro, orc):
+    # each system is lit by its own first particles (here 2 stars per system)
+    say("batched lighting")
+    sizes = [500, 1200, 3]
+    parts, pos, vel, col, off = make_batch(astro, sizes, seed0=700)
+    rng = np.random.default_rng(9)
+    col = col.copy()
+    col["color"] = rng.uniform(0, 1, (len(col), 4)).astype(np.float32)
+    col["type"] = 1
+    for k in range(len(sizes)):
+        col["type"][off[k]:off[k] + 2] = 0
+        col["type"][off[k] + 2:off[k + 1]:3] = 2
+    P, Po = astro.params_physics(21), orc.params_physics(21)
+    P.lighting_stars = Po.lighting_stars = 2
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=4) as ctx:
+        ctx.set_params(P)
+        ctx.upload_systems(pos, vel, col, off)
+        ctx.step(1e-3, 2)
+        ctx.check()
+        pb, vb, cb = ctx.download()
+    for k in range(len(sizes)):
+        sl = slice(off[k], off[k + 1])
+        p, v, c = parts[k][0], parts[k][1], col[sl]
+        for _ in range(2):
+            p, v, c, _, _ = orc.step(p, v, c, Po, 1e-3, threads=8, return_col=True)
+        assert np.array_equal(cb[sl]["color"], c["color"]), f"system {k}"
+        assert np.array_equal(cb[sl]["type"], c["type"])
+        assert np.array_equal(pb[sl]["position"], p["position"])
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_snapshot_resume_is_bitexact(astro, tmp_path, batched):
+    # write after 3 steps, run 4 more; a fresh context resumed from the file and run 4 steps
+    # ends in the same bytes (positions, velocities, colours)
+    say("snapshot resume, batched", batched)
+    if batched:
+        sizes = [3000, 5000]
+        _, pos, vel, col, off = make_batch(astro, sizes, seed0=40)
+    else:
+        pos, vel, col = astro.ic_plummer(8000, 41)
+    col = col.copy()
+    col["type"] = 1
+    col["type"][:2] = 0
+    col["color"] = np.random.default_rng(4).uniform(0, 1, (len(col), 4)).astype(np.float32)
+    P = astro.params_physics(21)
+    P.lighting_stars = 2
+    path = str(tmp_path / "state.snap")
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=4) as ctx:
+        ctx.set_params(P)
+        if batched:
+            ctx.upload_systems(pos, vel, col, off)
+        else:
+            ctx.upload(pos, vel, col)
+        ctx.step(1e-3, 3)
+        ctx.snapshot_write(path, 0.003)
+        ctx.step(1e-3, 4)
+        ctx.check()
+        a = ctx.download()
+    info = astro.snapshot_info(path)
+    assert info["n_particles"] == len(pos) and info["n_systems"] == (2 if batched else 1)
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=4) as ctx2:
+        assert ctx2.snapshot_read(path) == 0.003
+        assert ctx2.params().as_dict() == P.as_dict()
+        ctx2.step(1e-3, 4)
+        ctx2.check()
+        b = ctx2.download()
+    for x, y in zip(a, b):
+        assert x.tobytes() == y.tobytes()

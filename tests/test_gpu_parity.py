@@ -399,3 +399,28 @@ def test_forces_bitexact_both_division_paths(astro, orc, small_ctx, scale):
     t = orc.build_tree(pos, Po)
     want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
     assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
+
+
+def test_lighting_stage_and_step(astro, orc, small_ctx):
+    # BarnesHut.metal:103-167 through the stage call and inside the compat step (5 stars)
+    say("lighting pass, compat 2000/205")
+    ctx = small_ctx
+    pos, vel, col, Po, nt = setup_case(astro, orc, ctx, CASES[0])
+    col = col.copy()
+    col["color"] = np.random.default_rng(8).uniform(0, 1, (len(col), 4)).astype(np.float32)
+    ctx.upload(pos, vel, col, n_tree=nt)
+    ctx.lighting_pass(5)
+    _, _, got = ctx.download()
+    want = orc.lighting_pass(pos, col, 5)
+    assert np.array_equal(got["color"], want["color"]) and np.array_equal(got["type"], want["type"])
+    ctx.set_use_graph(True)
+    ctx.upload(pos, vel, col, n_tree=nt)
+    ctx.step(np.float32(1 / 60), 3)
+    ctx.check()
+    p2, v2, c2 = ctx.download()
+    p, v, c = pos, vel, col
+    for _ in range(3):
+        p, v, c, _, _ = orc.step(p, v, c, Po, np.float32(1 / 60), n_tree=nt, threads=8,
+                                 return_col=True)
+    assert np.array_equal(c2["color"], c["color"])
+    assert np.array_equal(p2["position"], p["position"])

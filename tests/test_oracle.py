@@ -479,3 +479,52 @@ def test_quadrupole_reaches_1e3_rms_at_theta_half(orc, astro):
         rms[(mp, kahan)] = np.sqrt((d ** 2).sum() / (ds ** 2).sum())
     assert rms[(2, 0)] < 1e-3 and rms[(0, 0)] > 3 * rms[(2, 0)], rms
     assert abs(rms[(2, 1)] - rms[(2, 0)]) < 0.05 * rms[(2, 0)]
+
+
+# ------------------------------------------- lighting pass (BarnesHut.metal:103-167)
+def test_lighting_closed_form(orc):
+    # one star of colour c: a planet at distance d gets c * (4e5 / d)^2, alpha 1; dust gets the
+    # same light with alpha min(max(rgb), 0.25); stars keep their colours
+    pos = particles([[0, 0, 0], [8e5, 0, 0], [0, 4e6, 0], [0, 0, -2e6]], [1, 1, 1, 1])
+    col = np.zeros(4, COLOR_TYPE)
+    col["type"] = [0, 1, 2, 2]
+    col["color"][0] = [1.0, 0.5, 0.25, 1.0]
+    col["color"][1:] = 7.0  # overwritten
+    out = orc.lighting_pass(pos, col, 1)  # the first num_stars particles are the light sources
+    assert np.array_equal(out["color"][0], col["color"][0])
+    assert np.allclose(out["color"][1], [0.25, 0.125, 0.0625, 1.0], rtol=1e-6)
+    assert np.allclose(out["color"][2, :3], [0.01, 0.005, 0.0025], rtol=1e-6)
+    assert np.isclose(out["color"][2, 3], 0.01, rtol=1e-6)
+    assert np.allclose(out["color"][3], [0.04, 0.02, 0.01, 0.04], rtol=1e-6)
+
+
+def test_lighting_uses_at_most_five_stars_and_floors_distance(orc):
+    rng = np.random.default_rng(3)
+    xyz = np.vstack([rng.normal(0, 1e5, (7, 3)), [[0, 0, 0]]])
+    pos = particles(xyz.astype(np.float32), np.ones(8))
+    col = np.zeros(8, COLOR_TYPE)
+    col["type"] = [0] * 7 + [1]
+    col["color"][:7] = rng.uniform(0, 1, (7, 4))
+    a = orc.lighting_pass(pos, col, 7)["color"][7]
+    b = orc.lighting_pass(pos, col, 5)["color"][7]
+    assert np.array_equal(a, b)  # stars 6 and 7 never count
+    d = np.linalg.norm(xyz[:5] - xyz[7], axis=1)
+    want = (col["color"][:5, :3] * (4e5 / d)[:, None] ** 2).sum(0)
+    assert np.allclose(a[:3], want, rtol=1e-5) and a[3] == 1.0
+    # a planet on top of a star: distance floored at 1e-9 * 4e5
+    pos2 = particles([[0, 0, 0], [0, 0, 0]], [1, 1])
+    col2 = np.zeros(2, COLOR_TYPE)
+    col2["type"] = [0, 1]
+    col2["color"][0] = [1e-18, 0, 0, 1]
+    assert np.isclose(orc.lighting_pass(pos2, col2, 1)["color"][1, 0], 1.0, rtol=1e-6)
+
+
+def test_lighting_per_system(orc):
+    # batched: each system is lit by its own first particles only
+    pos = particles([[0, 0, 0], [4e5, 0, 0], [0, 0, 0], [0, 8e5, 0]], [1, 1, 1, 1])
+    col = np.zeros(4, COLOR_TYPE)
+    col["type"] = [0, 1, 0, 1]
+    col["color"][0] = [1, 1, 1, 1]
+    col["color"][2] = [0, 2, 0, 1]
+    out = orc.lighting_pass(pos, col, 1, sys_off=[0, 2, 4])["color"]
+    assert np.allclose(out[1], [1, 1, 1, 1]) and np.allclose(out[3], [0, 0.5, 0, 1])
