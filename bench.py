@@ -34,7 +34,9 @@ def parse():
     ap.add_argument("--n-per-disk", type=int, default=250000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--dt", type=float, default=1e-3)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=30,
+                    help="oracle steps timed for the CPU baseline (about 10 s on 16 threads)")
+    ap.add_argument("--cpu-steps-matched", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (<= 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -507,10 +509,10 @@ def main():
                                          f"{threads} OpenMP threads"}
         out["gpu_vs_cpu"] = value / cv
         if "accuracy" in out:
-            cq, elq = cpu_baseline(pos, vel, col, a.dt, max(1, a.cpu_steps - 1), threads,
+            cq, elq = cpu_baseline(pos, vel, col, a.dt, a.cpu_steps_matched, threads,
                                    theta=0.5, multipole=2)
             out["cpu_baseline"]["matched"] = {
-                "value": cq, "sample": f"{max(1, a.cpu_steps - 1)} oracle steps at theta 0.5 "
+                "value": cq, "sample": f"{a.cpu_steps_matched} oracle steps at theta 0.5 "
                                        f"with quadrupoles after one warm-up step ({elq:.1f} s)"}
             out["accuracy"]["matched"]["gpu_vs_cpu"] = out["accuracy"]["matched"]["value"] / cq
     print(json.dumps(out))
