@@ -65,6 +65,8 @@ static int refresh_layers(astro_ctx* c) {
         return fail(c, ASTRO_ERR_CAPACITY, "tree needs more nodes than the context holds");
     invalidate_graph(c);
     c->onodes_dirty = true;  // stale-tail clearing assumes an unchanged layer table
+    c->tree_valid = false;
+    c->onodes_behind = false;
     return ASTRO_OK;
 }
 
@@ -131,6 +133,10 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    if (c->aux) hipStreamSynchronize(c->aux);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->aux) hipStreamDestroy(c->aux);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return ASTRO_OK;
@@ -182,7 +188,11 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         return false;
     };
     if (bad(hipSetDevice(c->device))) return ASTRO_ERR_HIP;
-    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
+    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) ||
+        bad(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking)) ||
+        bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) ||
+        bad(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)))
+        return ASTRO_ERR_HIP;
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(c->device_bytes, &c->pos, NP);
     e = e ? e : dalloc(c->device_bytes, &c->vel, NP);
@@ -868,7 +878,34 @@ static int enqueue_sort(astro_ctx* c) {
                       c->nt, c->dev_consts + 1, plan, c->nsys > 1 ? c->sys_of : nullptr);
 }
 
-static int enqueue_layers(astro_ctx* c) {
+// The main stream waits for a deferred OctreeNode emission.
+static void join_aux(astro_ctx* c) {
+    if (!c->join_pending) return;
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    c->join_pending = false;
+}
+
+// The OctreeNode array of the tree in tnodes (and the emptied stale slots), on stream es.
+static void enqueue_emit(astro_ctx* c, hipStream_t es) {
+    TreeArgs a = tree_args(c);
+    launch_emit_onodes(a, c->layer_counts_prev, es);
+    launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, es);
+}
+
+// Starts the emission of the current tree on the second stream; the main stream carries on
+// (join_aux before anything rewrites the tree or its counts).
+static int fork_emit(astro_ctx* c) {
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    enqueue_emit(c, c->aux);
+    HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
+    c->join_pending = true;
+    return ASTRO_OK;
+}
+
+// emit: write the OctreeNode array right after the layers (on the main stream); false leaves
+// it to the caller (the pipelined step emits a tree while the next one is being sorted).
+static int enqueue_layers(astro_ctx* c, bool emit = true) {
     hipStream_t s = c->stream;
     TreeArgs a = tree_args(c);
     // scratch: all layers' supertile sums first (one clear), then per-layer tile counts
@@ -903,9 +940,10 @@ static int enqueue_layers(astro_ctx* c) {
         launch_layer_emit(a, l, tc[l], sup[l], top ? nullptr : tc[l + 1],
                           top ? nullptr : sup[l + 1], s);
     }
+    c->tree_valid = true;
     if (a.onodes) {
-        launch_clear_stale(a, c->layer_counts_prev, s);
-        launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, s);
+        if (emit) enqueue_emit(c, s);
+        c->onodes_behind = !emit;
     }
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -915,7 +953,7 @@ static int enqueue_build(astro_ctx* c) {
     int rc;
     if ((rc = enqueue_bbox_morton(c))) return rc;
     if ((rc = enqueue_sort(c))) return rc;
-    return enqueue_layers(c);
+    return enqueue_layers(c, true);
 }
 
 static int enqueue_forces(astro_ctx* c, unsigned long long* stats) {
@@ -939,13 +977,40 @@ extern "C" int astro_compute_forces(astro_ctx* c) {
     return enqueue_forces(c, nullptr);
 }
 
-static int enqueue_step(astro_ctx* c, float dt) {
+// One step, pipelined: the OctreeNode array of the previous step's tree is written on the
+// second stream while this step computes its bounding cube, keys and sort (which touch neither
+// the tree nor the node array); the layers wait for it.  The array of the last tree is written
+// by finish_emit at the end of the call.  ev: optional stage events (ASTRO_STAGE_COUNT + 1).
+static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     int rc;
-    if ((rc = enqueue_build(c))) return rc;
+    const bool emit = c->emit && c->onodes;
+    if (ev) hipEventRecord(ev[0], c->stream);
+    if (emit && c->onodes_behind && c->tree_valid)
+        if ((rc = fork_emit(c))) return rc;
+    if ((rc = enqueue_bbox_morton(c))) return rc;
+    if (ev) hipEventRecord(ev[1], c->stream);
+    if ((rc = enqueue_sort(c))) return rc;
+    if (ev) hipEventRecord(ev[2], c->stream);
+    join_aux(c);
+    if ((rc = enqueue_layers(c, false))) return rc;
+    if (ev) hipEventRecord(ev[3], c->stream);
     if ((rc = enqueue_forces(c, nullptr))) return rc;
+    if (ev) hipEventRecord(ev[4], c->stream);
     // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
     if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
     launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
+    if (ev) hipEventRecord(ev[5], c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// Brings the OctreeNode array up to the last tree after pipelined steps.
+static int finish_emit(astro_ctx* c) {
+    join_aux(c);
+    if (c->emit && c->onodes && c->onodes_behind && c->tree_valid) {
+        enqueue_emit(c, c->stream);
+        c->onodes_behind = false;
+    }
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -954,22 +1019,29 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
     HIPCHK(c, hipSetDevice(c->device));
+    if (c->emit && c->onodes_dirty) {  // the one-off full clear, outside any graph
+        launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
+        launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
+        c->onodes_dirty = false;
+    }
     if (!c->use_graph) {
         for (uint32_t k = 0; k < nsteps; k++)
             if (int rc = enqueue_step(c, dt)) return rc;
-        return ASTRO_OK;
+        return finish_emit(c);
     }
-    if (!c->graph_valid || c->graph_dt != dt || c->onodes_dirty) {
+    // the captured step starts with the emission of the previous tree: make sure there is one
+    uint32_t k = 0;
+    if (nsteps > 0 && c->emit && !c->tree_valid) {
+        if (int rc = enqueue_step(c, dt)) return rc;
+        k = 1;
+    }
+    if (k < nsteps && (!c->graph_valid || c->graph_dt != dt)) {
         invalidate_graph(c);
-        if (c->emit && c->onodes_dirty) {
-            // do the one-off full clear eagerly so that the graph holds only the steady step
-            launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
-            launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
-            c->onodes_dirty = false;
-        }
+        c->onodes_behind = c->emit;  // the captured step always emits the previous tree
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_step(c, dt);
+        join_aux(c);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) {
             if (g) hipGraphDestroy(g);
@@ -982,8 +1054,9 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
         c->graph_valid = true;
         c->graph_dt = dt;
     }
-    for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-    return ASTRO_OK;
+    for (; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    if (nsteps) c->onodes_behind = c->emit;
+    return finish_emit(c);
 }
 
 extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* stage_ms) {
@@ -994,21 +1067,13 @@ extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* 
     std::vector<hipEvent_t> ev((size_t)nsteps * ne);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     int rc = ASTRO_OK;
-    for (uint32_t k = 0; k < nsteps && !rc; k++) {
-        hipEvent_t* E = &ev[(size_t)k * ne];
-        hipEventRecord(E[0], c->stream);
-        rc = enqueue_bbox_morton(c);
-        hipEventRecord(E[1], c->stream);
-        if (!rc) rc = enqueue_sort(c);
-        hipEventRecord(E[2], c->stream);
-        if (!rc) rc = enqueue_layers(c);
-        hipEventRecord(E[3], c->stream);
-        if (!rc) rc = enqueue_forces(c, nullptr);
-        hipEventRecord(E[4], c->stream);
-        if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
-        launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
-        hipEventRecord(E[5], c->stream);
+    if (c->emit && c->onodes_dirty) {
+        launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
+        launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
+        c->onodes_dirty = false;
     }
+    for (uint32_t k = 0; k < nsteps && !rc; k++) rc = enqueue_step(c, dt, &ev[(size_t)k * ne]);
+    if (!rc) rc = finish_emit(c);
     hipError_t se = hipStreamSynchronize(c->stream);
     for (uint32_t s = 0; s < ASTRO_STAGE_COUNT; s++) stage_ms[s] = 0.0f;
     if (!rc && se == hipSuccess) {

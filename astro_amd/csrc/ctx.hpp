@@ -17,6 +17,14 @@ struct astro_ctx {
     astro_params params{};
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream: the OctreeNode emission of a step overlaps that step's traversal
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool join_pending = false;
+    // tnodes / layer_counts hold a tree built on the current layer table
+    bool tree_valid = false;
+    // the OctreeNode array lags the compact tree (a pipelined step emits it one step late)
+    bool onodes_behind = false;
     std::string err;
 
     uint32_t ns = 0, nt = 0, nsys = 1;

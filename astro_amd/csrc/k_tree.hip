@@ -344,7 +344,6 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
         }
         store_moments(a.qmom, n, S);
     }
-    if (a.onodes) write_onode(&a.onodes[n], key, cm, 0, 0, 0, QUAD ? S : nullptr);
     return h;
 }
 
@@ -575,9 +574,6 @@ __global__ __launch_bounds__(256) void k_layer_emit_int(TreeArgs a, uint32_t l,
                 }
                 store_moments(a.qmom, nd, S);
             }
-            if (a.onodes)
-                write_onode(&a.onodes[nd], t.key >> (3 * l), cm, t.first, kc, l,
-                            QUAD ? S : nullptr);
             pos[k] = p;
             lev[k++] = level;
         }
@@ -620,20 +616,73 @@ void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_c
                            super, next_cnt, next_super);
 }
 
-// Empty every OctreeNode slot that held a node in the previous build but not in this one
-// (Octree.metal:253-269 applied to the stale tail of each layer only).
-__global__ void k_clear_stale(TreeArgs a, const uint32_t* counts_prev) {
+// The OctreeNode array (the reference layout) from the compact nodes, after all layers are
+// built.  Block (tile, layer) takes 256 slots of one layer: each thread assembles one node in
+// registers and stores it into LDS field-major (rows padded by one float4, so neither the
+// per-node stores nor the copy-out conflict on banks); the block then writes its slots as one
+// contiguous run, 1 KB per wave store.  Slots that held a node in the previous build but not in
+// this one are emptied (Octree.metal:253-269 applied to the stale tail of each layer only);
+// slots never used stay empty.
+constexpr uint32_t kEmitNodes = 256;
+constexpr uint32_t kEmitRow = kEmitNodes + 1;
+__global__ __launch_bounds__(kEmitNodes) void k_emit_onodes(TreeArgs a,
+                                                           const uint32_t* counts_prev) {
+    __shared__ float4 s_f[8 * kEmitRow];
     const uint32_t l = blockIdx.y;
-    const uint32_t lo = a.counts[l], hi = counts_prev[l];
-    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi;
-         i += gridDim.x * blockDim.x) {
-        float cm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        write_onode(&a.onodes[a.off[l] + i], ASTRO_INVALID_MORTON_CODE, cm, 0, 0, 0);
+    const uint32_t cnt = a.counts[l], prev = counts_prev[l];
+    const uint32_t end = cnt > prev ? cnt : prev;
+    const uint32_t base = blockIdx.x * kEmitNodes;
+    if (base >= end) return;
+    const uint32_t nslots = min(kEmitNodes, end - base);
+    const uint32_t i = base + threadIdx.x;
+    const float inv = __uint_as_float(ASTRO_INVALID_CHILD);
+    if (threadIdx.x < nslots) {
+        float4 f[8];
+        if (i < cnt) {  // the fields of write_onode, from registers
+            const uint64_t node = a.off[l] + i;
+            const TNode t = a.tnodes[node];
+            const uint64_t code = t.key >> (3 * l);
+            const uint32_t nch = l ? tn_nchild(t.info) : 0u;
+            f[0] = make_float4(__uint_as_float((uint32_t)code),
+                               __uint_as_float((uint32_t)(code >> 32)), 0.0f, 0.0f);
+            f[1] = make_float4(t.x, t.y, t.z, 0.0f);
+            f[2] = make_float4(t.m, 0.0f, 0.0f, 0.0f);
+            f[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            f[4] = f[3];
+            if (a.quad) {
+                const float4 q1 = a.qmom[2 * node + 1];
+                f[3] = a.qmom[2 * node];
+                f[4] = make_float4(q1.x, q1.y, 0.0f, 0.0f);
+            }
+            f[5] = make_float4(0u < nch ? __uint_as_float(t.first) : inv,
+                               1u < nch ? __uint_as_float(t.first + 1) : inv,
+                               2u < nch ? __uint_as_float(t.first + 2) : inv,
+                               3u < nch ? __uint_as_float(t.first + 3) : inv);
+            f[6] = make_float4(4u < nch ? __uint_as_float(t.first + 4) : inv,
+                               5u < nch ? __uint_as_float(t.first + 5) : inv,
+                               6u < nch ? __uint_as_float(t.first + 6) : inv,
+                               7u < nch ? __uint_as_float(t.first + 7) : inv);
+            f[7] = make_float4(__uint_as_float(l), 0.0f, 0.0f, 0.0f);
+        } else {  // stale: an empty node
+            for (int k = 0; k < 8; k++) f[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            f[0] = make_float4(inv, inv, 0.0f, 0.0f);
+            f[5] = make_float4(inv, inv, inv, inv);
+            f[6] = f[5];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) s_f[k * kEmitRow + threadIdx.x] = f[k];
     }
+    __syncthreads();
+    float4* out = reinterpret_cast<float4*>(a.onodes + a.off[l] + base);
+    for (uint32_t e = threadIdx.x; e < nslots * 8u; e += kEmitNodes)
+        out[e] = s_f[(e & 7u) * kEmitRow + (e >> 3)];
 }
-void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s) {
-    if (!a.onodes) return;
-    hipLaunchKernelGGL(k_clear_stale, dim3(64, a.layers), dim3(256), 0, s, a, counts_prev);
+void launch_emit_onodes(const TreeArgs& a, const uint32_t* counts_prev, hipStream_t s) {
+    if (!a.onodes || !a.layers) return;
+    uint64_t maxcap = 0;
+    for (uint32_t l = 0; l < a.layers; l++) maxcap = a.cap[l] > maxcap ? a.cap[l] : maxcap;
+    const dim3 grid((uint32_t)((maxcap + kEmitNodes - 1) / kEmitNodes), a.layers);
+    hipLaunchKernelGGL(k_emit_onodes, grid, dim3(kEmitNodes), 0, s, a, counts_prev);
 }
 
 // System of every position from the offsets (binary search over nsys + 1 offsets).

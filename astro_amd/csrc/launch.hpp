@@ -100,7 +100,8 @@ void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_c
                        hipStream_t s);
 void launch_fill_sys_of(uint32_t* sys_of, const uint32_t* d_off, uint32_t nsys, uint32_t n,
                         hipStream_t s);
-void launch_clear_stale(const TreeArgs& a, uint32_t* counts_prev, hipStream_t s);
+// The OctreeNode array from the compact nodes (and emptied stale slots) after a fused build.
+void launch_emit_onodes(const TreeArgs& a, const uint32_t* counts_prev, hipStream_t s);
 
 // Reference-shaped stage kernels (Octree.metal live kernels).
 void launch_mark_uniques(const uint64_t* sorted, uint32_t* flags, uint32_t count,
