@@ -424,3 +424,60 @@ def test_lighting_stage_and_step(astro, orc, small_ctx):
                                  return_col=True)
     assert np.array_equal(c2["color"], c["color"])
     assert np.array_equal(p2["position"], p["position"])
+
+
+EDGE_CASES = ["one", "two", "all_equal", "pairs_equal", "massless_half", "b1_tree", "line"]
+
+
+@pytest.mark.parametrize("kind", EDGE_CASES)
+def test_edge_cases_bitexact(astro, orc, small_ctx, kind):
+    # degenerate inputs through the fused build, traversal and 3 steps, against the oracle:
+    # one and two particles, every particle on one point (one leaf), coincident pairs
+    # (key collisions), massless particles, the smallest tree (B = 1), a 1-D line
+    say("edge case", kind)
+    rng = np.random.default_rng(17)
+    bits = 21
+    if kind == "one":
+        xyz, m = [[0.5, -0.25, 2.0]], [1.0]
+    elif kind == "two":
+        xyz, m = [[0, 0, 0], [1e-3, 0, 0]], [1.0, 2.0]
+    elif kind == "all_equal":
+        xyz, m = np.full((300, 3), 0.125), rng.uniform(0.5, 1.5, 300)
+    elif kind == "pairs_equal":
+        base = rng.normal(0, 1, (400, 3))
+        xyz, m = np.repeat(base, 2, axis=0), rng.uniform(0.5, 1.5, 800)
+    elif kind == "massless_half":
+        xyz, m = rng.normal(0, 1, (1000, 3)), np.where(np.arange(1000) % 2 == 0, 0.0, 1e-3)
+    elif kind == "b1_tree":
+        xyz, m, bits = rng.uniform(-1, 1, (2000, 3)), np.full(2000, 5e-4), 1
+    else:
+        t = np.linspace(-4, 4, 1500)
+        xyz, m = np.stack([t, np.zeros_like(t), np.zeros_like(t)], 1), np.full(1500, 1e-3)
+    n = len(xyz)
+    pos = np.zeros(n, POSITION_MASS)
+    pos["position"] = np.asarray(xyz, np.float32)
+    pos["mass"] = np.asarray(m, np.float32)
+    vel = np.zeros(n, astro.VELOCITY_RADIUS)
+    vel["velocity"] = rng.normal(0, 0.1, (n, 3)).astype(np.float32)
+    col = np.zeros(n, COLOR_TYPE)
+    P, Po = astro.params_physics(bits), orc.params_physics(bits)
+    ctx = small_ctx
+    ctx.set_params(P)
+    ctx.set_emit_octree_nodes(True)
+    ctx.upload(pos, vel, col)
+    ctx.build_tree()
+    ctx.compute_forces()
+    ctx.check()
+    t = orc.build_tree(pos, Po)
+    assert ctx.octree_nodes().tobytes() == t["nodes"].tobytes()
+    want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
+    assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
+    ctx.upload(pos, vel, col)
+    ctx.step(1e-3, 3)
+    ctx.check()
+    p2, v2, _ = ctx.download()
+    p, v = pos, vel
+    for _ in range(3):
+        p, v, _, _ = orc.step(p, v, col, Po, 1e-3, threads=8)
+    assert np.array_equal(p2["position"], p["position"])
+    assert np.array_equal(v2["velocity"], v["velocity"])
