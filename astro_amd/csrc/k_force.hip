@@ -47,7 +47,8 @@ __device__ inline void accumulate(Acc3& F, float s, float dx, float dy, float dz
 
 // Quadrupole term of an accepted internal node with second moments S (xx, yy, zz, xy, xz, yz)
 // about its centre of mass, d = COM - target, D = r2 * r:
-//   a_Q = u * q,  u = 3G / (D r2),  q = w d - (S d + T/2 d),  w = 2.5 (d.S.d) / r2,  T = tr S.
+//   a_Q = u * q,  u = 3G v,  q = w d - (S d + T/2 d),  w = 2.5 (d.S.d) (v D),  T = tr S,
+// with v = 1 / (D r2) the one division (v D is 1/r2 to within a rounding).
 // The trace term is what Plummer softening adds to the expansion of  -G m / sqrt(|d|^2 + eps2).
 struct QuadTerm {
     float u, q0, q1, q2;
@@ -62,9 +63,10 @@ __device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, f
     const float dSd = __builtin_fmaf(dz, Sd2, __builtin_fmaf(dy, Sd1, dx * Sd0));
     const float T = (Sxx + Syy) + Szz;
     const float ht = 0.5f * T;
-    const float w = (2.5f * dSd) / r2;
+    const float v = 1.0f / (D * r2);
+    const float w = (2.5f * dSd) * (v * D);
     QuadTerm t;
-    t.u = (3.0f * G) / (D * r2);
+    t.u = (3.0f * G) * v;
     t.q0 = __builtin_fmaf(w, dx, -__builtin_fmaf(ht, dx, Sd0));
     t.q1 = __builtin_fmaf(w, dy, -__builtin_fmaf(ht, dy, Sd1));
     t.q2 = __builtin_fmaf(w, dz, -__builtin_fmaf(ht, dz, Sd2));
@@ -186,6 +188,13 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
         const float4* np = reinterpret_cast<const float4*>(tn_base + (size_t)(n << 5));
         const float4 cm = np[0];
         const uint2 li = reinterpret_cast<const uint2*>(np)[2];
+        // quadrupole mode: the moments are loaded with the node, not after the opening test
+        float4 qm0, qm1;
+        if (QUAD) {
+            const float4* qp = a.qmom + 2 * (size_t)n;
+            qm0 = qp[0];
+            qm1 = qp[1];
+        }
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
         if (!FAST) r2 = fmaxf(r2, r2f);
@@ -202,8 +211,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
         const float D = r2 * rr;
         const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
         if (QUAD && interact && !leaf) {
-            const float4* qp = a.qmom + 2 * (size_t)n;
-            const QuadTerm qt = quad_term(qp[0], qp[1], dx, dy, dz, r2, D, G);
+            const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, D, G);
             accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
             if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
         } else if (KAHAN) {
