@@ -43,8 +43,9 @@ static void compute_layers(const std::vector<uint32_t>& sizes, uint32_t bits, ui
 
 static uint64_t scratch_need(uint32_t nmax, const uint64_t* cap, uint32_t layers) {
     uint64_t sort = radix_work_words(nmax);                          // one digit pass
-    uint64_t lay = layer_tiles(nmax) + layer_supers(nmax);           // leaf layer
-    for (uint32_t l = 1; l < layers; l++) lay += layer_tiles(cap[l - 1]) + layer_supers(cap[l - 1]);
+    uint64_t lay = layer_tiles(nmax) + layer_super_words(nmax);      // leaf layer
+    for (uint32_t l = 1; l < layers; l++)
+        lay += layer_tiles(cap[l - 1]) + layer_super_words(cap[l - 1]);
     uint64_t scan = scan_tiles(nmax) + 1;                            // astro_scan
     return std::max(std::max(sort, lay), scan);
 }
@@ -912,7 +913,7 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     uint64_t nsup = 0, ntile = 0;
     for (uint32_t l = 0; l < c->layers; l++) {
         const uint64_t cand = l == 0 ? c->nt : c->cap[l - 1];
-        nsup += layer_supers(cand);
+        nsup += layer_super_words(cand);
         ntile += layer_tiles(cand);
     }
     if (nsup + ntile > c->scratch_words)
@@ -928,7 +929,7 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     uint32_t* p = c->scratch;
     for (uint32_t l = 0; l < c->layers; l++) {
         sup[l] = p;
-        p += layer_supers(l == 0 ? c->nt : c->cap[l - 1]);
+        p += layer_super_words(l == 0 ? c->nt : c->cap[l - 1]);
     }
     for (uint32_t l = 0; l < c->layers; l++) {
         tc[l] = p;

@@ -358,6 +358,10 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
 constexpr uint32_t kLayerItems = 1;
 constexpr uint32_t kLayerTile = 256 * kLayerItems;
 constexpr uint32_t kSuper = 64;
+// Supertile sums sit one per 128-byte line: device-scope atomics to one line are serialized
+// (about 10 ns each on MI355X), and every block of a layer adds to some supertile sum, so sums
+// packed into one line would queue thousands of atomics behind each other.
+constexpr uint32_t kSuperStride = 32;
 
 __device__ inline uint32_t layer_candidates(const TreeArgs& a, uint32_t l) {
     return l == 0 ? a.nt : a.counts[l - 1];
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(256) void k_layer_count(TreeArgs a, uint32_t l,
     block_exclusive_scan<256>(cnt, s_tmp, &total);
     if (threadIdx.x == 0) {
         tile_cnt[blockIdx.x] = total;
-        if (total) atomicAdd(&super[blockIdx.x / kSuper], total);
+        if (total) atomicAdd(&super[(blockIdx.x / kSuper) * kSuperStride], total);
     }
 }
 
@@ -406,7 +410,7 @@ __device__ inline void count_next_heads(uint32_t l, uint32_t first, uint32_t tot
         const uint32_t t = t0 + threadIdx.x;
         if (c) {
             atomicAdd(&next_cnt[t], c);
-            atomicAdd(&next_super[t / kSuper], c);
+            atomicAdd(&next_super[(t / kSuper) * kSuperStride], c);
         }
     }
 }
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(256) void k_leaf_emit(TreeArgs a, const uint32_t* _
     if (base >= n) return;
     const uint32_t sup = tile / kSuper;
     uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t];
+    for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t * kSuperStride];
     if (threadIdx.x < tile - sup * kSuper) part += tile_cnt[sup * kSuper + threadIdx.x];
     const uint32_t c0 = (uint32_t)base + threadIdx.x * kLayerItems;
     uint32_t hmask = 0, cnt = 0;
@@ -484,7 +488,7 @@ __global__ __launch_bounds__(256) void k_layer_emit_int(TreeArgs a, uint32_t l,
     const uint32_t n = a.counts[l - 1];
     const uint32_t sup = tile / kSuper;
     uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t];
+    for (uint32_t t = threadIdx.x; t < sup; t += 256) part += super[t * kSuperStride];
     if (threadIdx.x < tile - sup * kSuper) part += tile_cnt[sup * kSuper + threadIdx.x];
     uint32_t lv[NL];
 #pragma unroll
@@ -586,8 +590,8 @@ __global__ __launch_bounds__(256) void k_layer_emit_int(TreeArgs a, uint32_t l,
 uint32_t layer_tiles(uint64_t candidates) {
     return (uint32_t)((candidates + kLayerTile - 1) / kLayerTile);
 }
-uint32_t layer_supers(uint64_t candidates) {
-    return (layer_tiles(candidates) + kSuper - 1) / kSuper;
+uint64_t layer_super_words(uint64_t candidates) {
+    return (uint64_t)((layer_tiles(candidates) + kSuper - 1) / kSuper) * kSuperStride;
 }
 
 void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s) {

@@ -91,9 +91,9 @@ void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const 
 // Layers: the leaf layer's heads are counted per tile by launch_layer_count; the emit of
 // layer l writes its nodes and adds the heads of layer l+1 to next_cnt / next_super (zeroed;
 // null for the top layer).  Per layer: layer_tiles(candidate capacity) tile counts and
-// layer_supers(...) supertile sums.
+// layer_super_words(...) words of supertile sums (spread one per 128-byte line).
 uint32_t layer_tiles(uint64_t candidates);
-uint32_t layer_supers(uint64_t candidates);
+uint64_t layer_super_words(uint64_t candidates);
 void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s);
 void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_cnt,
                        const uint32_t* super, uint32_t* next_cnt, uint32_t* next_super,
