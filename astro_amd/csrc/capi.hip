@@ -66,8 +66,6 @@ static int refresh_layers(astro_ctx* c) {
         return fail(c, ASTRO_ERR_CAPACITY, "tree needs more nodes than the context holds");
     invalidate_graph(c);
     c->onodes_dirty = true;  // stale-tail clearing assumes an unchanged layer table
-    c->tree_valid = false;
-    c->onodes_behind = false;
     return ASTRO_OK;
 }
 
@@ -130,7 +128,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
                     c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
-                    c->leaf_of_sorted, c->cubes, c->minmax, c->err_word, c->stats, c->scratch,
+                    c->leaf_of_sorted, c->cubes, c->minmax, c->cube_counter, c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -189,6 +187,10 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         return false;
     };
     if (bad(hipSetDevice(c->device))) return ASTRO_ERR_HIP;
+    int cus = 0;
+    if (bad(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)))
+        return ASTRO_ERR_HIP;
+    c->num_cus = cus > 0 ? (uint32_t)cus : 1u;
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) ||
         bad(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking)) ||
         bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) ||
@@ -225,7 +227,9 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->layer_counts_prev, kMaxLayers);
     e = e ? e : dalloc(c->device_bytes, &c->leaf_of_sorted, NT);
     e = e ? e : dalloc(c->device_bytes, &c->cubes, c->cfg.max_systems);
-    e = e ? e : dalloc(c->device_bytes, &c->minmax, 6ull * c->cfg.max_systems);
+    e = e ? e : dalloc(c->device_bytes, &c->minmax,
+                       std::max(6ull * c->cfg.max_systems, 8ull * cube_single_blocks(NP)));
+    e = e ? e : dalloc(c->device_bytes, &c->cube_counter, 1);
     e = e ? e : dalloc(c->device_bytes, &c->err_word, 1);
     e = e ? e : dalloc(c->device_bytes, &c->stats, 4);
     e = e ? e : dalloc(c->device_bytes, &c->d_sys_off, c->cfg.max_systems + 1ull);
@@ -239,6 +243,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     if (bad(dalloc(c->device_bytes, &c->scratch, c->scratch_words))) return ASTRO_ERR_NOMEM;
     hipStream_t s = c->stream;
     launch_fill_u32(c->err_word, 0, 1, s);
+    launch_fill_u32(c->cube_counter, 0, 1, s);
     launch_fill_u32(c->layer_counts, 0, kMaxLayers, s);
     launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
     launch_fill_u32(c->sys_of, 0, NP, s);
@@ -646,9 +651,12 @@ static int need_particles(astro_ctx* c) {
 static void enqueue_cube(astro_ctx* c, uint32_t count) {
     hipStream_t s = c->stream;
     if (c->params.quantizer != ASTRO_QUANT_BBOX_POW2) return;
-    const bool seg = c->nsys > 1;
+    if (c->nsys <= 1) {
+        launch_cube_single(c->pos, count, c->minmax, c->cube_counter, c->cubes, s);
+        return;
+    }
     launch_bbox_init(c->minmax, c->nsys, s);
-    launch_bbox(c->pos, count, seg ? c->sys_of : nullptr, c->minmax, s);
+    launch_bbox(c->pos, count, c->sys_of, c->minmax, s);
     launch_cube_finalize(c->minmax, c->cubes, c->nsys, s);
 }
 
@@ -886,10 +894,11 @@ static void join_aux(astro_ctx* c) {
     c->join_pending = false;
 }
 
-// The OctreeNode array of the tree in tnodes (and the emptied stale slots), on stream es.
+// The OctreeNode array of the tree in tnodes (and the emptied stale slots), on stream es:
+// beside a traversal with one block per CU, alone with four.
 static void enqueue_emit(astro_ctx* c, hipStream_t es) {
     TreeArgs a = tree_args(c);
-    launch_emit_onodes(a, c->layer_counts_prev, es);
+    launch_emit_onodes(a, c->layer_counts_prev, es == c->aux ? c->num_cus : 4 * c->num_cus, es);
     launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, es);
 }
 
@@ -941,11 +950,7 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
         launch_layer_emit(a, l, tc[l], sup[l], top ? nullptr : tc[l + 1],
                           top ? nullptr : sup[l + 1], s);
     }
-    c->tree_valid = true;
-    if (a.onodes) {
-        if (emit) enqueue_emit(c, s);
-        c->onodes_behind = !emit;
-    }
+    if (a.onodes && emit) enqueue_emit(c, s);
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -978,22 +983,22 @@ extern "C" int astro_compute_forces(astro_ctx* c) {
     return enqueue_forces(c, nullptr);
 }
 
-// One step, pipelined: the OctreeNode array of the previous step's tree is written on the
-// second stream while this step computes its bounding cube, keys and sort (which touch neither
-// the tree nor the node array); the layers wait for it.  The array of the last tree is written
-// by finish_emit at the end of the call.  ev: optional stage events (ASTRO_STAGE_COUNT + 1).
+// One step.  The OctreeNode array of the step's tree is written on the second stream beside
+// the traversal and the update; the next step's layers, or finish_emit at the end of the call,
+// wait for it.  ev: optional stage events (ASTRO_STAGE_COUNT + 1).
 static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     int rc;
     const bool emit = c->emit && c->onodes;
     if (ev) hipEventRecord(ev[0], c->stream);
-    if (emit && c->onodes_behind && c->tree_valid)
-        if ((rc = fork_emit(c))) return rc;
     if ((rc = enqueue_bbox_morton(c))) return rc;
     if (ev) hipEventRecord(ev[1], c->stream);
     if ((rc = enqueue_sort(c))) return rc;
     if (ev) hipEventRecord(ev[2], c->stream);
     join_aux(c);
     if ((rc = enqueue_layers(c, false))) return rc;
+    // the OctreeNode array of this tree is written on the second stream, beside the traversal
+    // and the update (which read the tree but never write it); the next build joins it first
+    if (emit && (rc = fork_emit(c))) return rc;
     if (ev) hipEventRecord(ev[3], c->stream);
     if ((rc = enqueue_forces(c, nullptr))) return rc;
     if (ev) hipEventRecord(ev[4], c->stream);
@@ -1005,13 +1010,9 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     return ASTRO_OK;
 }
 
-// Brings the OctreeNode array up to the last tree after pipelined steps.
+// The main stream waits for the last step's OctreeNode emission.
 static int finish_emit(astro_ctx* c) {
     join_aux(c);
-    if (c->emit && c->onodes && c->onodes_behind && c->tree_valid) {
-        enqueue_emit(c, c->stream);
-        c->onodes_behind = false;
-    }
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -1030,15 +1031,8 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
             if (int rc = enqueue_step(c, dt)) return rc;
         return finish_emit(c);
     }
-    // the captured step starts with the emission of the previous tree: make sure there is one
-    uint32_t k = 0;
-    if (nsteps > 0 && c->emit && !c->tree_valid) {
-        if (int rc = enqueue_step(c, dt)) return rc;
-        k = 1;
-    }
-    if (k < nsteps && (!c->graph_valid || c->graph_dt != dt)) {
+    if (nsteps && (!c->graph_valid || c->graph_dt != dt)) {
         invalidate_graph(c);
-        c->onodes_behind = c->emit;  // the captured step always emits the previous tree
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_step(c, dt);
@@ -1055,8 +1049,7 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
         c->graph_valid = true;
         c->graph_dt = dt;
     }
-    for (; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-    if (nsteps) c->onodes_behind = c->emit;
+    for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
     return finish_emit(c);
 }
 

@@ -20,11 +20,8 @@ struct astro_ctx {
     // second stream: the OctreeNode emission of a step overlaps that step's traversal
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    uint32_t num_cus = 1;              // compute units of the device (emission grid)
     bool join_pending = false;
-    // tnodes / layer_counts hold a tree built on the current layer table
-    bool tree_valid = false;
-    // the OctreeNode array lags the compact tree (a pipelined step emits it one step late)
-    bool onodes_behind = false;
     std::string err;
 
     uint32_t ns = 0, nt = 0, nsys = 1;
@@ -52,7 +49,8 @@ struct astro_ctx {
     uint64_t node_capacity = 0;
     uint32_t *layer_counts = nullptr, *layer_counts_prev = nullptr, *leaf_of_sorted = nullptr;
     float4* cubes = nullptr;
-    uint32_t* minmax = nullptr;
+    uint32_t* minmax = nullptr;        // per-system ordered min/max keys, or per-tile partials
+    uint32_t* cube_counter = nullptr;  // tiles done in the one-system cube kernel (kept zero)
     uint32_t* err_word = nullptr;
     unsigned long long* stats = nullptr;
     uint32_t* d_sys_off = nullptr;     // [max_systems + 1] system offsets on the device

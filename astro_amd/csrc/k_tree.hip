@@ -86,17 +86,15 @@ void launch_bbox(const float4* pos, uint32_t n, const uint32_t* sys_of, uint32_t
                        sys_of, minmax);
 }
 
-// cube = {lo.x, lo.y, lo.z, side}: side = smallest power of two >= the largest extent.
-__global__ void k_cube_finalize(const uint32_t* mm, float4* cubes, uint32_t nsys) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys) return;
+// cube = {lo.x, lo.y, lo.z, side} from the ordered min/max keys k[0..2] / k[3..5]:
+// side = smallest power of two >= the largest extent; an empty set gives {0, 0, 0, 1}.
+__device__ inline float4 cube_from_keys(const uint32_t* k) {
     float lo[3], hi[3];
     bool empty = false;
     for (int a = 0; a < 3; a++) {
-        uint32_t kl = mm[s * 6 + a], kh = mm[s * 6 + 3 + a];
-        if (kl == 0xFFFFFFFFu || kh == 0u) empty = true;
-        lo[a] = ord2f(kl);
-        hi[a] = ord2f(kh);
+        if (k[a] == 0xFFFFFFFFu || k[3 + a] == 0u) empty = true;
+        lo[a] = ord2f(k[a]);
+        hi[a] = ord2f(k[3 + a]);
     }
     if (empty)
         for (int a = 0; a < 3; a++) lo[a] = hi[a] = 0.0f;
@@ -111,11 +109,100 @@ __global__ void k_cube_finalize(const uint32_t* mm, float4* cubes, uint32_t nsys
         float m = frexpf(ext, &e);
         side = ldexpf(1.0f, m == 0.5f ? e - 1 : e);
     }
-    cubes[s] = make_float4(lo[0], lo[1], lo[2], side);
+    return make_float4(lo[0], lo[1], lo[2], side);
+}
+
+__global__ void k_cube_finalize(const uint32_t* mm, float4* cubes, uint32_t nsys) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    uint32_t k[6];
+    for (int a = 0; a < 6; a++) k[a] = mm[s * 6 + a];
+    cubes[s] = cube_from_keys(k);
 }
 void launch_cube_finalize(const uint32_t* minmax, float4* cubes, uint32_t nsys, hipStream_t s) {
     hipLaunchKernelGGL(k_cube_finalize, dim3((nsys + 63) / 64), dim3(64), 0, s, minmax, cubes,
                        nsys);
+}
+
+__device__ inline uint32_t wave_umin(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+__device__ inline uint32_t wave_umax(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+// Min (k[0..2]) / max (k[3..5]) of the block's ordered keys into s_red[6][4]; result in k of
+// thread 0.
+__device__ inline void block_minmax(uint32_t* k, uint32_t (*s_red)[4]) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int a = 0; a < 3; a++) {
+        k[a] = wave_umin(k[a]);
+        k[3 + a] = wave_umax(k[3 + a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 6; a++) s_red[a][wave] = k[a];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 6; a++)
+            for (int w = 1; w < 4; w++)
+                k[a] = a < 3 ? min(k[a], s_red[a][w]) : max(k[a], s_red[a][w]);
+}
+
+// One system's cube in one launch: each block reduces a tile of kBoxTile positions to ordered
+// min/max keys (the same total order as the atomics of k_bbox) and stores them in part; the
+// last block to finish (counted on *counter, zero before the launch and zeroed again by that
+// block) reduces the partials and writes cubes[0].  Plain stores and one atomic per block
+// replace the 6 same-line atomics per tile of k_bbox.
+__global__ __launch_bounds__(256) void k_cube_single(const float4* __restrict__ pos, uint32_t n,
+                                                     uint32_t* part, uint32_t* counter,
+                                                     float4* cubes) {
+    __shared__ uint32_t s_red[6][4];
+    __shared__ uint32_t s_last;
+    const uint64_t base = (uint64_t)blockIdx.x * kBoxTile;
+    const uint64_t end = base + kBoxTile < n ? base + kBoxTile : n;
+    uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+#pragma unroll 4
+    for (uint64_t e = base + threadIdx.x; e < end; e += 256) {
+        const float4 p = pos[e];
+        const uint32_t x = f2ord(p.x), y = f2ord(p.y), z = f2ord(p.z);
+        k[0] = min(k[0], x);
+        k[1] = min(k[1], y);
+        k[2] = min(k[2], z);
+        k[3] = max(k[3], x);
+        k[4] = max(k[4], y);
+        k[5] = max(k[5], z);
+    }
+    block_minmax(k, s_red);
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 6; a++) part[blockIdx.x * 8 + a] = k[a];
+        const uint32_t done =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = done + 1 == gridDim.x ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int a = 0; a < 6; a++) k[a] = a < 3 ? 0xFFFFFFFFu : 0u;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256)
+        for (int a = 0; a < 6; a++) {
+            const uint32_t v = part[b * 8 + a];
+            k[a] = a < 3 ? min(k[a], v) : max(k[a], v);
+        }
+    __syncthreads();  // s_red is reused
+    block_minmax(k, s_red);
+    if (threadIdx.x == 0) {
+        cubes[0] = cube_from_keys(k);
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+uint32_t cube_single_blocks(uint32_t n) { return n ? (n + kBoxTile - 1) / kBoxTile : 1u; }
+void launch_cube_single(const float4* pos, uint32_t n, uint32_t* part, uint32_t* counter,
+                        float4* cubes, hipStream_t s) {
+    hipLaunchKernelGGL(k_cube_single, dim3(cube_single_blocks(n)), dim3(256), 0, s, pos, n, part,
+                       counter, cubes);
 }
 
 // ------------------------------------------------------------------------------- Morton
@@ -621,22 +708,22 @@ void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_c
 }
 
 // The OctreeNode array (the reference layout) from the compact nodes, after all layers are
-// built.  Block (tile, layer) takes 256 slots of one layer: each thread assembles one node in
-// registers and stores it into LDS field-major (rows padded by one float4, so neither the
-// per-node stores nor the copy-out conflict on banks); the block then writes its slots as one
-// contiguous run, 1 KB per wave store.  Slots that held a node in the previous build but not in
-// this one are emptied (Octree.metal:253-269 applied to the stale tail of each layer only);
-// slots never used stay empty.
+// built.  A tile is 256 slots of one layer: each thread assembles one node in registers and
+// stores it into LDS field-major (rows padded by one float4, so neither the per-node stores nor
+// the copy-out conflict on banks); the block then writes the tile as one contiguous run, 1 KB
+// per wave store, with non-temporal stores so the 128 B per node stream past the L2 instead of
+// evicting the compact nodes that a concurrent traversal is reading.  Slots that held a node
+// in the previous build but not in this one are emptied (Octree.metal:253-269 applied to the
+// stale tail of each layer only); slots never used stay empty.
 constexpr uint32_t kEmitNodes = 256;
 constexpr uint32_t kEmitRow = kEmitNodes + 1;
-__global__ __launch_bounds__(kEmitNodes) void k_emit_onodes(TreeArgs a,
-                                                           const uint32_t* counts_prev) {
-    __shared__ float4 s_f[8 * kEmitRow];
-    const uint32_t l = blockIdx.y;
+__device__ __forceinline__ void emit_tile(const TreeArgs& a, const uint32_t* counts_prev,
+                                          uint32_t l, uint32_t tile, float4* s_f) {
     const uint32_t cnt = a.counts[l], prev = counts_prev[l];
-    const uint32_t end = cnt > prev ? cnt : prev;
-    const uint32_t base = blockIdx.x * kEmitNodes;
-    if (base >= end) return;
+    // a count beyond the layer's capacity has raised ERR_CAPACITY; never write past it
+    const uint32_t end = min(cnt > prev ? cnt : prev, (uint32_t)a.cap[l]);
+    const uint32_t base = tile * kEmitNodes;
+    if (base >= end) return;  // uniform over the block
     const uint32_t nslots = min(kEmitNodes, end - base);
     const uint32_t i = base + threadIdx.x;
     const float inv = __uint_as_float(ASTRO_INVALID_CHILD);
@@ -677,16 +764,35 @@ __global__ __launch_bounds__(kEmitNodes) void k_emit_onodes(TreeArgs a,
         for (int k = 0; k < 8; k++) s_f[k * kEmitRow + threadIdx.x] = f[k];
     }
     __syncthreads();
-    float4* out = reinterpret_cast<float4*>(a.onodes + a.off[l] + base);
-    for (uint32_t e = threadIdx.x; e < nslots * 8u; e += kEmitNodes)
-        out[e] = s_f[(e & 7u) * kEmitRow + (e >> 3)];
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v* out = reinterpret_cast<f4v*>(a.onodes + a.off[l] + base);
+    for (uint32_t e = threadIdx.x; e < nslots * 8u; e += kEmitNodes) {
+        const float4 v = s_f[(e & 7u) * kEmitRow + (e >> 3)];
+        const f4v w = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(w, out + e);
+    }
+    __syncthreads();  // s_f is reused by the next tile
 }
-void launch_emit_onodes(const TreeArgs& a, const uint32_t* counts_prev, hipStream_t s) {
+// Persistent blocks: the capacity tiles of all layers, layer by layer, dealt round-robin to
+// the gridDim.x blocks (tiles past a layer's live slots end at once).  A grid of about one
+// block per CU leaves most of each CU to the traversal this kernel runs beside.
+__global__ __launch_bounds__(kEmitNodes) void k_emit_onodes(TreeArgs a,
+                                                           const uint32_t* counts_prev) {
+    __shared__ float4 s_f[8 * kEmitRow];
+    const uint32_t G = gridDim.x;
+    uint32_t w0 = 0;  // tiles of the layers below l
+    for (uint32_t l = 0; l < a.layers; l++) {
+        const uint32_t tiles = (uint32_t)((a.cap[l] + kEmitNodes - 1) / kEmitNodes);
+        for (uint32_t t = (blockIdx.x + G - w0 % G) % G; t < tiles; t += G)
+            emit_tile(a, counts_prev, l, t, s_f);
+        w0 += tiles;
+    }
+}
+void launch_emit_onodes(const TreeArgs& a, const uint32_t* counts_prev, uint32_t blocks,
+                        hipStream_t s) {
     if (!a.onodes || !a.layers) return;
-    uint64_t maxcap = 0;
-    for (uint32_t l = 0; l < a.layers; l++) maxcap = a.cap[l] > maxcap ? a.cap[l] : maxcap;
-    const dim3 grid((uint32_t)((maxcap + kEmitNodes - 1) / kEmitNodes), a.layers);
-    hipLaunchKernelGGL(k_emit_onodes, grid, dim3(kEmitNodes), 0, s, a, counts_prev);
+    hipLaunchKernelGGL(k_emit_onodes, dim3(blocks ? blocks : 1u), dim3(kEmitNodes), 0, s, a,
+                       counts_prev);
 }
 
 // System of every position from the offsets (binary search over nsys + 1 offsets).

@@ -85,6 +85,11 @@ void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s);
 void launch_bbox(const float4* pos, uint32_t n, const uint32_t* sys_of, uint32_t* minmax,
                  hipStream_t s);
 void launch_cube_finalize(const uint32_t* minmax, float4* cubes, uint32_t nsys, hipStream_t s);
+// One system in one launch: part holds 8 * cube_single_blocks(n) words, *counter is zero
+// before the launch and is left zero.
+uint32_t cube_single_blocks(uint32_t n);
+void launch_cube_single(const float4* pos, uint32_t n, uint32_t* part, uint32_t* counter,
+                        float4* cubes, hipStream_t s);
 void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const float4* cubes,
                    const astro_params& p, uint64_t* codes, uint64_t* codes2, uint32_t* idx,
                    hipStream_t s);
@@ -100,8 +105,10 @@ void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_c
                        hipStream_t s);
 void launch_fill_sys_of(uint32_t* sys_of, const uint32_t* d_off, uint32_t nsys, uint32_t n,
                         hipStream_t s);
-// The OctreeNode array from the compact nodes (and emptied stale slots) after a fused build.
-void launch_emit_onodes(const TreeArgs& a, const uint32_t* counts_prev, hipStream_t s);
+// The OctreeNode array from the compact nodes (and emptied stale slots) after a fused build,
+// by `blocks` persistent blocks.
+void launch_emit_onodes(const TreeArgs& a, const uint32_t* counts_prev, uint32_t blocks,
+                        hipStream_t s);
 
 // Reference-shaped stage kernels (Octree.metal live kernels).
 void launch_mark_uniques(const uint64_t* sorted, uint32_t* flags, uint32_t count,
