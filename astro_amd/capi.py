@@ -114,6 +114,7 @@ SIGNATURES = [
     ("astro_direct_forces_fp64", _I, [_P, _P, _U32, _P]),
     ("astro_particle_count", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     ("astro_check_division", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
+    ("astro_check_open_test", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
     ("astro_snapshot_write", _I, [_P, ctypes.c_char_p, ctypes.c_double]),
     ("astro_snapshot_read", _I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
     ("astro_snapshot_info", _I, [ctypes.c_char_p, _P]),
@@ -476,6 +477,12 @@ class Context:
         """Mismatches of the traversal's fast division against IEEE division on n pairs."""
         m = ctypes.c_uint64()
         self._chk(self.lib.astro_check_division(self.h, n, seed, ctypes.byref(m)))
+        return m.value
+
+    def check_open_test(self, n, seed=1):
+        """Disagreements of the squared opening test with the square-root form on n cases."""
+        m = ctypes.c_uint64()
+        self._chk(self.lib.astro_check_open_test(self.h, n, seed, ctypes.byref(m)))
         return m.value
 
     def device_bytes(self):

@@ -82,4 +82,26 @@ ASTRO_HD inline float accept_rmin(float side0, const AcceptRule& a) {
     return r;
 }
 
+// Smallest float x with RN(sqrt(x)) >= T, for T = 0, +inf or T >= 2^-62 finite.  RN(sqrt(x))
+// reaches T exactly when sqrt(x) exceeds the midpoint m of T and its float predecessor (the
+// square root of a float never equals m: m has 25 significant bits ending in a 1, so m^2 would
+// need more than 48), i.e. when x > m^2, i.e. x >= RU(m^2); m^2 is exact in double.  Scaling
+// T by 2^l scales the result by 4^l exactly while both stay normal, so for a traversal with
+// eps2 >= 2^-96 the test  RN(sqrt(r2)) >= rmin0 * 2^l  is  r2 >= accept_r2min(rmin0) * 4^l.
+// Returns 0 for T = 0 and +inf for T = +inf or when RU(m^2) exceeds the float range.
+ASTRO_HD inline float accept_r2min(float T) {
+    if (T == 0.0f) return 0.0f;
+    if (!(T < INFINITY)) return INFINITY;
+    const float pred = f32_from_bits(bits_from_f32(T) - 1u);
+    const double m = ((double)pred + (double)T) * 0.5;
+    const double m2 = m * m;
+    float c = (float)m2;
+    if ((double)c < m2) c = f32_from_bits(bits_from_f32(c) + 1u);
+    return c;
+}
+// The r2 form of the opening test applies to rmin0 (accept_r2min's domain).
+ASTRO_HD inline bool accept_r2_ok(float rmin0) {
+    return rmin0 == 0.0f || !(rmin0 < INFINITY) || rmin0 >= 0x1p-62f;
+}
+
 }  // namespace astro

@@ -1260,6 +1260,23 @@ extern "C" int astro_check_division(astro_ctx* c, uint64_t n, uint64_t seed, uin
     return ASTRO_OK;
 }
 
+// Checks the traversal's r2 opening test against RN(sqrt(r2)) >= rmin0 2^l on n pseudo-random
+// cases near the threshold; *mismatches receives the number of disagreements (0 expected).
+extern "C" int astro_check_open_test(astro_ctx* c, uint64_t n, uint64_t seed,
+                                     uint64_t* mismatches) {
+    if (!c || !mismatches) return ASTRO_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(c->stats);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(d), 0ull, 1, c->stream);
+    if (check_open_test(n, seed, d, c->stream))
+        return fail(c, ASTRO_ERR_HIP, "opening-test check launch");
+    unsigned long long h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *mismatches = h;
+    return ASTRO_OK;
+}
+
 // ------------------------------------------------------------------------- snapshots
 namespace {
 struct FileCloser {

@@ -157,11 +157,17 @@ struct Visits {
 // One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
 // potential.  QUAD: accepted internal nodes add their quadrupole term.  FAST: a.fast holds (no
 // floor or cut-off can bind, square root without denormal scaling; exact for finite positions).
-// FDIV: every G M / (r2 r) is in div_rn_mid's domain.
+// FDIV: every G M / (r2 r) is in div_rn_mid's domain; with FAST and QUAD it also means every
+// lane's rmin0 is in accept_r2min's domain, and nodes are then opened by r2 against
+// r2min0 4^l, so that the square root, the division and the quadrupole term are computed in a
+// branch that waves without an interacting lane skip.  (For monopoles the branch costs more
+// than the square root and division it skips, so they stay straight-line.)
 template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV>
 __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uint32_t* s_stack,
                                          uint32_t tid, float4 p, uint32_t self, uint32_t sys,
-                                         float rmin0, Acc3& F, double& phi, Visits& st) {
+                                         float rmin0, float r2min0, Acc3& F, double& phi,
+                                         Visits& st) {
+    constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
     const uint32_t node_total = (uint32_t)a.node_total;  // node indices are 32-bit
@@ -198,47 +204,78 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
         if (!FAST) r2 = fmaxf(r2, r2f);
-        const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
         const uint32_t layer = tn_layer(li.y);
         const bool leaf = layer == 0u;
-        const bool far = rr >= ldexpf(rmin0, (int)layer);
-        const bool hit = !bad & (n != self) & (FAST || r2 > cut);
-        const bool acc = !bad & far;
-        const bool interact = leaf ? hit : acc;
-        const bool open = !bad & !leaf & !far;
-        // the term is computed for every lane and kept only where it interacts
         const float gm = G * cm.w;
-        const float D = r2 * rr;
-        const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
-        if (QUAD && interact && !leaf) {
-            const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, D, G);
-            accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
-            if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
-        } else if (KAHAN) {
-            Acc3 T = F;
-            accumulate<true>(T, s, dx, dy, dz);
-            F.x = interact ? T.x : F.x;
-            F.y = interact ? T.y : F.y;
-            F.z = interact ? T.z : F.z;
-            F.cx = interact ? T.cx : F.cx;
-            F.cy = interact ? T.cy : F.cy;
-            F.cz = interact ? T.cz : F.cz;
+        bool open;
+        if constexpr (R2) {
+            // RN(sqrt(r2)) >= rmin0 2^l  <=>  r2 >= r2min0 4^l (accept.hpp): the square root
+            // and the division are only computed where some lane interacts.  The empty asm
+            // keeps the mass in the node's one wide load: sunk into the branch it would be a
+            // second memory round trip on the loop's critical path.
+            __asm__ volatile("" : : "v"(gm));
+            const bool far = r2 >= ldexpf(r2min0, 2 * (int)layer);
+            const bool interact = !bad & (leaf ? n != self : far);
+            open = !bad & !leaf & !far;
+            if (interact) {
+                const float rr = sqrt_rn_normal(r2);
+                const float D = r2 * rr;
+                const float s = div_rn_mid(gm, D);
+                if (QUAD && !leaf) {
+                    const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, D, G);
+                    accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
+                    if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+                } else {
+                    accumulate<KAHAN>(F, s, dx, dy, dz);
+                    if (MODE == 2) phi -= (double)gm / (double)rr;
+                }
+            }
+            if (MODE == 1) {
+                st.visits += !bad;
+                st.leafs += leaf && interact;
+                st.monos += !leaf && interact;
+                st.opens += open;
+            }
         } else {
-            // F never holds -0 (it starts at +0, and x + (-x) rounds to +0), so the zero term
-            // fma(0, d, F) leaves it unchanged for finite d; the selects take lane masks
-            const float s_hit = hit ? s : 0.0f, s_acc = acc ? s : 0.0f;
-            const float sk = leaf ? s_hit : s_acc;
-            F.x = __builtin_fmaf(sk, dx, F.x);
-            F.y = __builtin_fmaf(sk, dy, F.y);
-            F.z = __builtin_fmaf(sk, dz, F.z);
+            const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
+            const bool far = rr >= ldexpf(rmin0, (int)layer);
+            const bool hit = !bad & (n != self) & (FAST || r2 > cut);
+            const bool acc = !bad & far;
+            const bool interact = leaf ? hit : acc;
+            open = !bad & !leaf & !far;
+            // the term is computed for every lane and kept only where it interacts
+            const float D = r2 * rr;
+            const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
+            if (QUAD && interact && !leaf) {
+                const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, D, G);
+                accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
+                if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+            } else if (KAHAN) {
+                Acc3 T = F;
+                accumulate<true>(T, s, dx, dy, dz);
+                F.x = interact ? T.x : F.x;
+                F.y = interact ? T.y : F.y;
+                F.z = interact ? T.z : F.z;
+                F.cx = interact ? T.cx : F.cx;
+                F.cy = interact ? T.cy : F.cy;
+                F.cz = interact ? T.cz : F.cz;
+            } else {
+                // F never holds -0 (it starts at +0, and x + (-x) rounds to +0), so the zero
+                // term fma(0, d, F) leaves it unchanged for finite d; the selects take lane masks
+                const float s_hit = hit ? s : 0.0f, s_acc = acc ? s : 0.0f;
+                const float sk = leaf ? s_hit : s_acc;
+                F.x = __builtin_fmaf(sk, dx, F.x);
+                F.y = __builtin_fmaf(sk, dy, F.y);
+                F.z = __builtin_fmaf(sk, dz, F.z);
+            }
+            if (MODE == 1) {
+                st.visits += !bad;
+                st.leafs += leaf && interact;
+                st.monos += !leaf && interact;
+                st.opens += open;
+            }
+            if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         }
-        if (MODE == 1) {
-            st.visits += !bad;
-            st.leafs += leaf && interact;
-            st.monos += !leaf && interact;
-            st.opens += open;
-        }
-        if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         const bool push = open & (top_cnt > 0u);
         const bool overflow = push & (sp >= depth);
         if (overflow) atomicOr(a.err, ERR_STACK);
@@ -274,12 +311,19 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
     // +-div_lmax (then |d| <= 2 div_lmax per axis; masses are bounded on the host)
+    // a node at layer l is accepted iff r >= rmin0 * 2^l (accept.hpp)
+    float rmin0 = 0.0f;
     bool fdiv_lane = true;
     if (active) {
-        const float L = a.div_lmax;
         const float4 cu = a.cubes[sys];
+        float side0;
+        if (a.p.side_rule == ASTRO_SIDE_SHIFTED) side0 = 2097152.0f;  // 2^21
+        else side0 = ldexpf(cu.w, -(int)a.p.bits_per_axis);
+        rmin0 = accept_rmin(side0, a.acc);
+        const float L = a.div_lmax;
         fdiv_lane = L > 0.0f && fabsf(cu.x) <= L && fabsf(cu.y) <= L && fabsf(cu.z) <= L &&
                     fabsf(cu.x + cu.w) <= L && fabsf(cu.y + cu.w) <= L && fabsf(cu.z + cu.w) <= L;
+        if (FAST && QUAD) fdiv_lane = fdiv_lane && accept_r2_ok(rmin0);
     }
     const bool fdiv = __builtin_amdgcn_readfirstlane(__all(fdiv_lane ? 1 : 0)) != 0;
 
@@ -290,19 +334,15 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
         const uint32_t type = a.type[pidx];
         const uint32_t self =
             (type != 2u && j < a.nt) ? (uint32_t)(a.off0 + a.leaf_of_sorted[j]) : 0xFFFFFFFFu;
-        float side0;
-        if (a.p.side_rule == ASTRO_SIDE_SHIFTED) side0 = 2097152.0f;  // 2^21
-        else side0 = ldexpf(a.cubes[sys].w, -(int)a.p.bits_per_axis);
-        // a node at layer l is accepted iff r >= rmin0 * 2^l (accept.hpp)
-        const float rmin0 = accept_rmin(side0, a.acc);
         Acc3 F;
         double phi = 0.0;
         if (fdiv)
             traverse<NT, KAHAN, MODE, QUAD, FAST, true>(a, depth, s_stack, tid, p, self, sys,
-                                                        rmin0, F, phi, st);
+                                                        rmin0, QUAD ? accept_r2min(rmin0) : 0.0f,
+                                                        F, phi, st);
         else
             traverse<NT, KAHAN, MODE, QUAD, FAST, false>(a, depth, s_stack, tid, p, self, sys,
-                                                         rmin0, F, phi, st);
+                                                         rmin0, 0.0f, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
@@ -344,6 +384,45 @@ __global__ void k_check_division(uint64_t n, uint64_t seed, unsigned long long* 
 }
 int check_division(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s) {
     hipLaunchKernelGGL(k_check_division, dim3(2048), dim3(256), 0, s, n, seed, d_mismatch);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Self-check of the r2 form of the opening test: for T = rmin0 drawn with exponents in
+// [-62, 40] (one in eight a power of two) and a layer l in [0, 21], compares
+// RN(sqrt(x)) >= T 2^l  with  x >= accept_r2min(T) 4^l  on x within 8 ulps of the threshold
+// (three in four draws) or drawn from the binades around it, x >= 2^-96.
+__global__ void k_check_open_test(uint64_t n, uint64_t seed, unsigned long long* mismatch) {
+    unsigned long long bad = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t w[2];
+        for (int k = 0; k < 2; k++) {  // SplitMix64
+            uint64_t z = seed + (2 * i + k + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            w[k] = z ^ (z >> 31);
+        }
+        const int e = (int)((w[0] >> 32) % 103u) - 62;
+        const uint32_t mant = ((w[0] >> 29) & 7u) == 0u ? 0u : ((uint32_t)w[0] & 0x7FFFFFu);
+        const float T = __uint_as_float(((uint32_t)(e + 127) << 23) | mant);
+        const int l = (int)((w[1] >> 40) % 22u);
+        const float Tl = ldexpf(T, l), Cl = ldexpf(accept_r2min(T), 2 * l);
+        float x;
+        if (((w[1] >> 62) & 3u) != 0u) {
+            x = __uint_as_float(__float_as_uint(Cl) + (uint32_t)((w[1] >> 8) & 15u) - 8u);
+        } else {
+            const int ex = 2 * (e + l) + (int)((w[1] >> 24) % 5u) - 2;
+            x = __uint_as_float(((uint32_t)(ex + 127) << 23) | ((uint32_t)w[1] & 0x7FFFFFu));
+        }
+        if (!(x >= 0x1p-96f) || !(x < INFINITY)) continue;
+        const bool by_sqrt = sqrt_rn_normal(x) >= Tl, by_r2 = x >= Cl;
+        bad += (by_sqrt != by_r2) || (by_sqrt != (sqrtf(x) >= Tl));
+    }
+    bad = wave_sum_u64(bad);
+    if ((threadIdx.x & 63u) == 0u && bad) atomicAdd(mismatch, bad);
+}
+int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_open_test, dim3(2048), dim3(256), 0, s, n, seed, d_mismatch);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

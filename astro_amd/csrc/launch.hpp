@@ -161,6 +161,8 @@ struct ForceArgs {
 };
 // Self-check of div_rn_mid against the IEEE division on n pseudo-random pairs of its domain.
 int check_division(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s);
+// Self-check of the r2 form of the opening test against the square-root form.
+int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s);
 // Fills acc and fast from p.
 void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);

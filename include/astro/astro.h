@@ -193,6 +193,13 @@ int astro_direct_forces_fp64(astro_ctx* ctx, const uint32_t* targets, uint32_t n
  * *mismatches receives the count of differing results (0 expected).  No reference
  * counterpart.  Synchronous. */
 int astro_check_division(astro_ctx* ctx, uint64_t n, uint64_t seed, uint64_t* mismatches);
+/* Self-check of the traversal's opening test in squared form (used with quadrupoles when the
+ * fast division holds and eps2 >= 2^-96): a node at layer l opens unless
+ * RN(sqrt(r2)) >= rmin0 2^l, which
+ * is evaluated as r2 >= c0 4^l with c0 the smallest float whose rounded square root reaches
+ * rmin0.  Compares both forms on n pseudo-random cases near the threshold; *mismatches
+ * receives the count of disagreements (0 expected).  No reference counterpart.  Synchronous. */
+int astro_check_open_test(astro_ctx* ctx, uint64_t n, uint64_t seed, uint64_t* mismatches);
 /* Device memory the context allocated at creation. */
 int astro_device_bytes(const astro_ctx* ctx, uint64_t* bytes);
 

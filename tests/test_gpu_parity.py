@@ -381,6 +381,13 @@ def test_fast_division_matches_ieee(astro, small_ctx):
     assert small_ctx.check_division(1 << 26, seed=12345) == 0
 
 
+def test_squared_opening_test_matches_sqrt(astro, small_ctx):
+    # with quadrupoles and the fast division the traversal opens a node unless r2 >= c0 4^l,
+    # in place of RN(sqrt(r2)) >= rmin0 2^l; 2^24 cases within 8 ulps of the threshold or near
+    say("squared opening test self-check")
+    assert small_ctx.check_open_test(1 << 24, seed=777) == 0
+
+
 @pytest.mark.parametrize("scale", [1.0, 3.0e4])
 def test_forces_bitexact_both_division_paths(astro, orc, small_ctx, scale):
     # scale 1: cubes inside +-8192, masses bounded: the fast division; scale 3e4: cubes
