@@ -4,8 +4,8 @@
 // Morton order (sorted position j), so the 64 lanes of a wavefront walk nearly the same part
 // of the tree; blocks are mapped so each XCD (and its L2) takes one contiguous stretch of the
 // Morton curve.  The stack holds (first child, remaining count) runs, one per open level,
-// so its depth is bounded by the layer count (<= 22) instead of 7 per level; it lives in LDS
-// in a [depth][thread] layout.  Visit order equals the reference's LIFO order (children
+// so its depth is bounded by the layer count less two (20 at 21 bits per axis) instead of 7
+// per level; it lives in LDS in a [depth][thread] layout.  Visit order equals the reference's LIFO order (children
 // pushed 0..7, popped 7..0), so per-target sums are bit-identical to the oracle.
 //
 // k_force_reference: the reference kernel shape (BarnesHut.metal:22-101) over the 128-B
@@ -429,7 +429,9 @@ int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, h
 template <int NT, bool KAHAN, bool QUAD, bool FAST>
 static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t blocks = (a.ns + NT - 1) / NT;
-    const uint32_t depth = a.p.bits_per_axis + 2u;  // open levels <= layers - 1
+    // an entry is pushed only when a node of layers 1 .. layers-2 opens with siblings left (the
+    // root has none, leaves never open): at most layers - 2 = bits - 1 entries
+    const uint32_t depth = a.p.bits_per_axis > 2u ? a.p.bits_per_axis - 1u : 1u;
     const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
     if (mode == 0)
         hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
