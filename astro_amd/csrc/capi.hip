@@ -132,10 +132,6 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
-    if (c->aux) hipStreamSynchronize(c->aux);
-    if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
-    if (c->aux) hipStreamDestroy(c->aux);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return ASTRO_OK;
@@ -191,11 +187,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     if (bad(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device)))
         return ASTRO_ERR_HIP;
     c->num_cus = cus > 0 ? (uint32_t)cus : 1u;
-    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) ||
-        bad(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking)) ||
-        bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) ||
-        bad(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)))
-        return ASTRO_ERR_HIP;
+    if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(c->device_bytes, &c->pos, NP);
     e = e ? e : dalloc(c->device_bytes, &c->vel, NP);
@@ -887,30 +879,11 @@ static int enqueue_sort(astro_ctx* c) {
                       c->nt, c->dev_consts + 1, plan, c->nsys > 1 ? c->sys_of : nullptr);
 }
 
-// The main stream waits for a deferred OctreeNode emission.
-static void join_aux(astro_ctx* c) {
-    if (!c->join_pending) return;
-    hipStreamWaitEvent(c->stream, c->ev_join, 0);
-    c->join_pending = false;
-}
-
-// The OctreeNode array of the tree in tnodes (and the emptied stale slots), on stream es:
-// beside a traversal with one block per CU, alone with four.
+// The OctreeNode array of the tree in tnodes (and the emptied stale slots).
 static void enqueue_emit(astro_ctx* c, hipStream_t es) {
     TreeArgs a = tree_args(c);
-    launch_emit_onodes(a, c->layer_counts_prev, es == c->aux ? c->num_cus : 4 * c->num_cus, es);
+    launch_emit_onodes(a, c->layer_counts_prev, 4 * c->num_cus, es);
     launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, es);
-}
-
-// Starts the emission of the current tree on the second stream; the main stream carries on
-// (join_aux before anything rewrites the tree or its counts).
-static int fork_emit(astro_ctx* c) {
-    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
-    enqueue_emit(c, c->aux);
-    HIPCHK(c, hipEventRecord(c->ev_join, c->aux));
-    c->join_pending = true;
-    return ASTRO_OK;
 }
 
 // emit: write the OctreeNode array right after the layers (on the main stream); false leaves
@@ -983,36 +956,32 @@ extern "C" int astro_compute_forces(astro_ctx* c) {
     return enqueue_forces(c, nullptr);
 }
 
-// One step.  The OctreeNode array of the step's tree is written on the second stream beside
-// the traversal and the update; the next step's layers, or finish_emit at the end of the call,
-// wait for it.  ev: optional stage events (ASTRO_STAGE_COUNT + 1).
+// A stage boundary of a timed step.  While the stream is being captured the record becomes an
+// external event node of the graph, which times the replay.
+static void record_stage(astro_ctx* c, hipEvent_t e) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    hipStreamIsCapturing(c->stream, &st);
+    hipEventRecordWithFlags(e, c->stream,
+                            st == hipStreamCaptureStatusActive ? hipEventRecordExternal : 0u);
+}
+
+// One step.  ev: optional stage events (ASTRO_STAGE_COUNT + 1).
 static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     int rc;
     const bool emit = c->emit && c->onodes;
-    if (ev) hipEventRecord(ev[0], c->stream);
+    if (ev) record_stage(c, ev[0]);
     if ((rc = enqueue_bbox_morton(c))) return rc;
-    if (ev) hipEventRecord(ev[1], c->stream);
+    if (ev) record_stage(c, ev[1]);
     if ((rc = enqueue_sort(c))) return rc;
-    if (ev) hipEventRecord(ev[2], c->stream);
-    join_aux(c);
-    if ((rc = enqueue_layers(c, false))) return rc;
-    // the OctreeNode array of this tree is written on the second stream, beside the traversal
-    // and the update (which read the tree but never write it); the next build joins it first
-    if (emit && (rc = fork_emit(c))) return rc;
-    if (ev) hipEventRecord(ev[3], c->stream);
+    if (ev) record_stage(c, ev[2]);
+    if ((rc = enqueue_layers(c, emit))) return rc;
+    if (ev) record_stage(c, ev[3]);
     if ((rc = enqueue_forces(c, nullptr))) return rc;
-    if (ev) hipEventRecord(ev[4], c->stream);
+    if (ev) record_stage(c, ev[4]);
     // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
     if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
     launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
-    if (ev) hipEventRecord(ev[5], c->stream);
-    LAUNCHCHK(c);
-    return ASTRO_OK;
-}
-
-// The main stream waits for the last step's OctreeNode emission.
-static int finish_emit(astro_ctx* c) {
-    join_aux(c);
+    if (ev) record_stage(c, ev[5]);
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -1029,14 +998,13 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
     if (!c->use_graph) {
         for (uint32_t k = 0; k < nsteps; k++)
             if (int rc = enqueue_step(c, dt)) return rc;
-        return finish_emit(c);
+        return ASTRO_OK;
     }
     if (nsteps && (!c->graph_valid || c->graph_dt != dt)) {
         invalidate_graph(c);
         hipGraph_t g = nullptr;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_step(c, dt);
-        join_aux(c);
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) {
             if (g) hipGraphDestroy(g);
@@ -1050,7 +1018,7 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
         c->graph_dt = dt;
     }
     for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-    return finish_emit(c);
+    return ASTRO_OK;
 }
 
 extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* stage_ms) {
@@ -1066,9 +1034,23 @@ extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* 
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
         c->onodes_dirty = false;
     }
-    for (uint32_t k = 0; k < nsteps && !rc; k++) rc = enqueue_step(c, dt, &ev[(size_t)k * ne]);
-    if (!rc) rc = finish_emit(c);
+    hipGraphExec_t x = nullptr;
+    if (c->use_graph && nsteps) {
+        // the timed steps captured as one graph: astro_step's kernels, replayed the same way,
+        // with timing nodes at the stage boundaries
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t k = 0; k < nsteps && !rc; k++) rc = enqueue_step(c, dt, &ev[(size_t)k * ne]);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (!rc && e == hipSuccess) e = hipGraphLaunch(x, c->stream);
+        if (!rc && e != hipSuccess) rc = fail(c, ASTRO_ERR_HIP, "timed step graph failed");
+    } else {
+        for (uint32_t k = 0; k < nsteps && !rc; k++) rc = enqueue_step(c, dt, &ev[(size_t)k * ne]);
+    }
     hipError_t se = hipStreamSynchronize(c->stream);
+    if (x) hipGraphExecDestroy(x);
     for (uint32_t s = 0; s < ASTRO_STAGE_COUNT; s++) stage_ms[s] = 0.0f;
     if (!rc && se == hipSuccess) {
         for (uint32_t k = 0; k < nsteps; k++) {

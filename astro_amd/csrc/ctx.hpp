@@ -17,11 +17,7 @@ struct astro_ctx {
     astro_params params{};
     int device = 0;
     hipStream_t stream = nullptr;
-    // second stream: the OctreeNode emission of a step overlaps that step's traversal
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t num_cus = 1;              // compute units of the device (emission grid)
-    bool join_pending = false;
     std::string err;
 
     uint32_t ns = 0, nt = 0, nsys = 1;

@@ -711,8 +711,8 @@ void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_c
 // built.  A tile is 256 slots of one layer: each thread assembles one node in registers and
 // stores it into LDS field-major (rows padded by one float4, so neither the per-node stores nor
 // the copy-out conflict on banks); the block then writes the tile as one contiguous run, 1 KB
-// per wave store, with non-temporal stores so the 128 B per node stream past the L2 instead of
-// evicting the compact nodes that a concurrent traversal is reading.  Slots that held a node
+// per wave store, with non-temporal stores so the 128 B per node stream past the caches instead
+// of evicting the compact nodes that the traversal reads next.  Slots that held a node
 // in the previous build but not in this one are emptied (Octree.metal:253-269 applied to the
 // stale tail of each layer only); slots never used stay empty.
 constexpr uint32_t kEmitNodes = 256;
@@ -774,8 +774,7 @@ __device__ __forceinline__ void emit_tile(const TreeArgs& a, const uint32_t* cou
     __syncthreads();  // s_f is reused by the next tile
 }
 // Persistent blocks: the capacity tiles of all layers, layer by layer, dealt round-robin to
-// the gridDim.x blocks (tiles past a layer's live slots end at once).  A grid of about one
-// block per CU leaves most of each CU to the traversal this kernel runs beside.
+// the gridDim.x blocks (tiles past a layer's live slots end at once).
 __global__ __launch_bounds__(kEmitNodes) void k_emit_onodes(TreeArgs a,
                                                            const uint32_t* counts_prev) {
     __shared__ float4 s_f[8 * kEmitRow];
