@@ -918,11 +918,13 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
         p += layer_tiles(l == 0 ? c->nt : c->cap[l - 1]);
     }
     launch_layer_count(a, tc[0], sup[0], s);
-    for (uint32_t l = 0; l < c->layers; l++) {
-        const bool top = l + 1 == c->layers;
-        launch_layer_emit(a, l, tc[l], sup[l], top ? nullptr : tc[l + 1],
-                          top ? nullptr : sup[l + 1], s);
+    const uint32_t l0 = layers_top_first(a);  // layers l0.. in one single-block launch
+    for (uint32_t l = 0; l < l0; l++) {
+        const bool last = l + 1 == l0;
+        launch_layer_emit(a, l, tc[l], sup[l], last ? nullptr : tc[l + 1],
+                          last ? nullptr : sup[l + 1], s);
     }
+    launch_layers_top(a, l0, s);
     if (a.onodes && emit) enqueue_emit(c, s);
     LAUNCHCHK(c);
     return ASTRO_OK;

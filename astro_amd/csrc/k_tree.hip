@@ -553,6 +553,78 @@ __global__ __launch_bounds__(256) void k_leaf_emit(TreeArgs a, const uint32_t* _
         count_next_heads(0, prefix, total, pos, lev, k, s_cnt, next_cnt, next_super);
 }
 
+// One parent of layer l >= 1: candidate c (a head at level >= l) of layer l - 1 starts it and
+// it takes index p.  u0 is candidate c's node and lv[0..7] the level bytes of candidates c ..
+// c + 7 (read as 0 past the layer's capacity); n is the candidate count.  The children are c
+// and the candidates after it up to the next head (at most 8); they are aggregated serially in
+// order (Octree.metal:157-172).  Returns the parent's head level.
+template <bool QUAD>
+__device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, uint32_t c,
+                                                uint32_t p, const uint32_t* lv, const TNode& u0,
+                                                uint32_t n) {
+    const uint64_t cb = a.off[l - 1];
+    const bool fp64 = l >= a.fp64_from;
+    uint32_t kc = 1;
+#pragma unroll
+    for (uint32_t j = 1; j < 8; j++) kc += (kc == j && c + j < n && lv[j] < l) ? 1u : 0u;
+    const TNode* ch = a.tnodes + cb + c;
+    float cm[4];
+    if (fp64) {
+        Agg64 g;
+        g.add(u0.x, u0.y, u0.z, u0.m);
+        for (uint32_t j = 1; j < kc; j++) {
+            const TNode v = ch[j];
+            g.add(v.x, v.y, v.z, v.m);
+        }
+        g.fin(cm);
+    } else {
+        Agg32 g;
+        g.add(u0.x, u0.y, u0.z, u0.m);
+        for (uint32_t j = 1; j < kc; j++) {
+            const TNode v = ch[j];
+            g.add(v.x, v.y, v.z, v.m);
+        }
+        g.fin(cm);
+    }
+    const uint32_t level = lv[0];
+    const uint64_t nd = a.off[l] + p;
+    TNode t;
+    t.x = cm[0];
+    t.y = cm[1];
+    t.z = cm[2];
+    t.m = cm[3];
+    t.first = (uint32_t)(cb + c);
+    t.info = tn_info(l, kc, level);
+    t.key = u0.key;
+    a.tnodes[nd] = t;
+    a.levels[nd] = (uint8_t)level;
+    if (QUAD) {
+        float S[6];
+        const float4* cq = a.qmom + 2 * (cb + c);
+        if (fp64) {
+            Mom64 q;
+            for (uint32_t j = 0; j < kc; j++) {
+                const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
+                const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                const TNode v = ch[j];
+                q.add(v.x, v.y, v.z, v.m, cm, Sc);
+            }
+            q.fin(S);
+        } else {
+            Mom32 q;
+            for (uint32_t j = 0; j < kc; j++) {
+                const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
+                const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
+                const TNode v = ch[j];
+                q.add(v.x, v.y, v.z, v.m, cm, Sc);
+            }
+            q.fin(S);
+        }
+        store_moments(a.qmom, nd, S);
+    }
+    return level;
+}
+
 // Layers l >= 1.  Every global load whose address depends only on the block and thread
 // position (the candidate count, the prefix sums, the candidates' level bytes with the seven
 // after them, each candidate node) is issued up front, so a block
@@ -598,80 +670,54 @@ __global__ __launch_bounds__(256) void k_layer_emit_int(TreeArgs a, uint32_t l,
         a.counts[l] = prefix + total;  // the last tile knows the layer's size
         if (prefix + total > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
     }
-    const bool fp64 = l >= a.fp64_from;
     uint32_t pos[NI], lev[NI], k = 0;
 #pragma unroll
     for (uint32_t i = 0; i < NI; i++) {
         if (!(hmask & (1u << i))) continue;
         if (p < a.cap[l]) {
-            const uint32_t c = c0 + i;
-            // children c, c+1, ... up to the next head (level >= l), at most 8
-            uint32_t kc = 1;
-#pragma unroll
-            for (uint32_t j = 1; j < 8; j++)
-                kc += (kc == j && c + j < n && lv[i + j] < l) ? 1u : 0u;
-            const TNode* ch = a.tnodes + cb + c;
-            float cm[4];
-            if (fp64) {
-                Agg64 g;
-                g.add(u0[i].x, u0[i].y, u0[i].z, u0[i].m);
-                for (uint32_t j = 1; j < kc; j++) {
-                    const TNode v = ch[j];
-                    g.add(v.x, v.y, v.z, v.m);
-                }
-                g.fin(cm);
-            } else {
-                Agg32 g;
-                g.add(u0[i].x, u0[i].y, u0[i].z, u0[i].m);
-                for (uint32_t j = 1; j < kc; j++) {
-                    const TNode v = ch[j];
-                    g.add(v.x, v.y, v.z, v.m);
-                }
-                g.fin(cm);
-            }
-            const uint32_t level = lv[i];
-            const uint64_t nd = a.off[l] + p;
-            TNode t;
-            t.x = cm[0];
-            t.y = cm[1];
-            t.z = cm[2];
-            t.m = cm[3];
-            t.first = (uint32_t)(cb + c);
-            t.info = tn_info(l, kc, level);
-            t.key = u0[i].key;
-            a.tnodes[nd] = t;
-            a.levels[nd] = (uint8_t)level;
-            float S[6];
-            if (QUAD) {
-                const float4* cq = a.qmom + 2 * (cb + c);
-                if (fp64) {
-                    Mom64 q;
-                    for (uint32_t j = 0; j < kc; j++) {
-                        const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
-                        const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                        const TNode v = ch[j];
-                        q.add(v.x, v.y, v.z, v.m, cm, Sc);
-                    }
-                    q.fin(S);
-                } else {
-                    Mom32 q;
-                    for (uint32_t j = 0; j < kc; j++) {
-                        const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
-                        const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                        const TNode v = ch[j];
-                        q.add(v.x, v.y, v.z, v.m, cm, Sc);
-                    }
-                    q.fin(S);
-                }
-                store_moments(a.qmom, nd, S);
-            }
             pos[k] = p;
-            lev[k++] = level;
+            lev[k++] = emit_parent<QUAD>(a, l, c0 + i, p, &lv[i], u0[i], n);
         }
         p++;
     }
     if (next_cnt)
         count_next_heads(l, prefix, total, pos, lev, k, s_cnt, next_cnt, next_super);
+}
+
+// The top layers in one launch of one block: for small layers the per-layer launches cost
+// more than the work.  Layer l's candidates (layer l - 1's nodes) are taken 1,024 at a time;
+// a block scan of their head flags, carried across the chunks, gives each parent's index.
+// Layer l's nodes are complete (and visible to the block) before layer l + 1 reads them.
+constexpr uint32_t kTopThreads = 1024;
+constexpr uint64_t kTopCandidates = 32768;  // largest candidate capacity of a layer built here
+template <bool QUAD>
+__global__ __launch_bounds__(kTopThreads) void k_layers_top(TreeArgs a, uint32_t l0) {
+    __shared__ uint32_t s_tmp[kTopThreads / 64 + 1];
+    uint32_t n = a.counts[l0 - 1];
+    for (uint32_t l = l0; l < a.layers; l++) {
+        const uint64_t cb = a.off[l - 1];
+        const uint32_t capc = (uint32_t)a.cap[l - 1];
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < n; base += kTopThreads) {
+            const uint32_t c = base + threadIdx.x;
+            uint32_t lv[8];
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) lv[j] = c + j < capc ? a.levels[cb + c + j] : 0u;
+            const TNode u0 = a.tnodes[cb + (c < capc ? c : 0u)];
+            const bool head = c < n && lv[0] >= l;
+            uint32_t total;
+            const uint32_t p = carry + block_exclusive_scan<kTopThreads>(head ? 1u : 0u, s_tmp,
+                                                                        &total);
+            if (head && p < a.cap[l]) emit_parent<QUAD>(a, l, c, p, lv, u0, n);
+            carry += total;
+        }
+        if (threadIdx.x == 0) {
+            a.counts[l] = carry;
+            if (carry > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
+        }
+        n = carry;
+        __syncthreads();  // layer l is written before layer l + 1 reads it
+    }
 }
 
 uint32_t layer_tiles(uint64_t candidates) {
@@ -685,6 +731,19 @@ void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, 
     if (!a.nt) return;
     hipLaunchKernelGGL(k_layer_count, dim3(layer_tiles(a.nt)), dim3(256), 0, s, a, 0u, tile_cnt,
                        super);
+}
+
+uint32_t layers_top_first(const TreeArgs& a) {
+    uint32_t l0 = a.layers;
+    while (l0 > 1 && a.cap[l0 - 2] <= kTopCandidates) l0--;
+    return l0;
+}
+void launch_layers_top(const TreeArgs& a, uint32_t l0, hipStream_t s) {
+    if (l0 >= a.layers || l0 == 0) return;
+    if (a.quad)
+        hipLaunchKernelGGL(k_layers_top<true>, dim3(1), dim3(kTopThreads), 0, s, a, l0);
+    else
+        hipLaunchKernelGGL(k_layers_top<false>, dim3(1), dim3(kTopThreads), 0, s, a, l0);
 }
 
 void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_cnt,

@@ -100,6 +100,11 @@ void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const 
 uint32_t layer_tiles(uint64_t candidates);
 uint64_t layer_super_words(uint64_t candidates);
 void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s);
+// The top layers with at most kTopCandidates candidates each in one single-block launch:
+// layers_top_first(a) is the first of them (a.layers if none; layer 0 never), and the layer
+// before it is emitted without next-layer counts.
+uint32_t layers_top_first(const TreeArgs& a);
+void launch_layers_top(const TreeArgs& a, uint32_t l0, hipStream_t s);
 void launch_layer_emit(const TreeArgs& a, uint32_t layer, const uint32_t* tile_cnt,
                        const uint32_t* super, uint32_t* next_cnt, uint32_t* next_super,
                        hipStream_t s);
