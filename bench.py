@@ -447,7 +447,10 @@ def main():
     for s, ms in stage_ms.items():
         per = ms / K
         e = {"ms": round(per, 4)}
-        if s in sb:
+        if s == "update" and per < 0.003:
+            # no lighting pass: the integrator runs in the traversal kernel's epilogue
+            e["fused_into"] = "force"
+        elif s in sb:
             e["gbs"] = round(sb[s] / (per * 1e-3) / 1e9, 1)
             e["hbm_frac"] = round(e["gbs"] / PEAK_HBM_GBS, 4)
         stages[s] = e
