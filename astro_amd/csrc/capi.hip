@@ -128,7 +128,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
                     c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
-                    c->leaf_of_sorted, c->cubes, c->minmax, c->cube_counter, c->err_word, c->stats, c->scratch,
+                    c->leaf_of_sorted, c->cubes, c->minmax, c->cube_counter, c->plevel, c->fused_buf, c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -222,6 +222,8 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->minmax,
                        std::max(6ull * c->cfg.max_systems, 8ull * cube_single_blocks(NP)));
     e = e ? e : dalloc(c->device_bytes, &c->cube_counter, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->plevel, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->fused_buf, fused_buf_words(NP));
     e = e ? e : dalloc(c->device_bytes, &c->err_word, 1);
     e = e ? e : dalloc(c->device_bytes, &c->stats, 4);
     e = e ? e : dalloc(c->device_bytes, &c->d_sys_off, c->cfg.max_systems + 1ull);
@@ -917,9 +919,16 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
         tc[l] = p;
         p += layer_tiles(l == 0 ? c->nt : c->cap[l - 1]);
     }
-    launch_layer_count(a, tc[0], sup[0], s);
     const uint32_t l0 = layers_top_first(a);  // layers l0.. in one single-block launch
-    for (uint32_t l = 0; l < l0; l++) {
+    // layers 0 .. lf-1 (where nodes hold a few particles each) in the fused launches; the
+    // reference quirk and the second moments keep the per-layer kernels
+    uint32_t lf = 0;
+    if (!a.quad && !a.quirk_last_leaf && c->layers > 10)
+        lf = std::min(std::min(l0, c->layers - 10), kFusedMaxLayers);
+    if (lf < 2) lf = 0;
+    if (lf) launch_fused_layers(a, lf, c->plevel, c->fused_buf, s);
+    if (lf < l0) launch_layer_count(a, lf, tc[lf], sup[lf], s);
+    for (uint32_t l = lf; l < l0; l++) {
         const bool last = l + 1 == l0;
         launch_layer_emit(a, l, tc[l], sup[l], last ? nullptr : tc[l + 1],
                           last ? nullptr : sup[l + 1], s);

@@ -720,6 +720,270 @@ __global__ __launch_bounds__(kTopThreads) void k_layers_top(TreeArgs a, uint32_t
     }
 }
 
+// ----------------------------------------------------------------- fused lower layers
+// The lower layers (0 .. lf-1, where nearly every node holds a handful of particles) in three
+// launches instead of one per layer.  A node of layer l starts at the sorted position j of its
+// first particle, and j starts nodes exactly at the layers l <= h(j), its head level (the
+// level byte the layer kernels carry; 0xFF for a particle whose key equals its predecessor's).
+//  * k_fused_heads: h(j) for every position, and per tile of kFusedTile positions the number
+//    of nodes of each layer < lf starting in it;
+//  * k_fused_scan: per layer, the exclusive scan of those counts over the tiles (each tile's
+//    first node index) and the layer's size;
+//  * k_fused_build: one block per tile builds every layer of its positions in LDS, leaves from
+//    the particles and each layer from the one below, and writes the nodes that start in its
+//    tile.  It reads past its tile up to the next head at layer lf-1 (at most kFusedHalo
+//    positions), so the nodes it owns are complete; a node that runs past that halo is left to
+//  * k_fused_fix: one block, layer by layer, builds those nodes from the finished layer below.
+// Aggregation order and arithmetic are those of the per-layer kernels (emit_leaf, emit_parent),
+// so the nodes are bit-identical.
+constexpr uint32_t kFusedTile = 1024;
+constexpr uint32_t kFusedHalo = 1024;
+constexpr uint32_t kFusedRange = kFusedTile + kFusedHalo;
+constexpr uint32_t kFusedThreads = 1024;
+constexpr uint8_t kNoHead = 0xFF;
+
+uint32_t fused_tiles(uint32_t nt) { return (nt + kFusedTile - 1) / kFusedTile; }
+
+__global__ __launch_bounds__(kFusedThreads) void k_fused_heads(TreeArgs a, uint8_t* plevel,
+                                                               uint32_t* fcnt, uint32_t lf,
+                                                               uint32_t tiles) {
+    __shared__ uint32_t s_hist[kMaxLayers];
+    if (threadIdx.x < kMaxLayers) s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t j = blockIdx.x * kFusedTile + threadIdx.x;
+    if (j < a.nt) {
+        uint32_t h = kNoHead;
+        if (j == 0 || (a.segmented && a.sys_of[j] != a.sys_of[j - 1])) {
+            h = a.bits;
+        } else {
+            const uint64_t k = a.sorted_keys[j], kp = a.sorted_keys[j - 1];
+            if (k != kp) h = head_level(k, kp, a.bits);
+        }
+        plevel[j] = (uint8_t)h;
+        if (h != kNoHead) atomicAdd(&s_hist[h < lf - 1 ? h : lf - 1], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < lf) {
+        uint32_t c = 0;
+        for (uint32_t b = threadIdx.x; b < lf; b++) c += s_hist[b];
+        fcnt[threadIdx.x * tiles + blockIdx.x] = c;
+    }
+}
+
+// Block l: exclusive scan of layer l's per-tile node counts, and the layer's size.
+__global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const uint32_t* fcnt,
+                                                              uint32_t* fbase, uint32_t tiles) {
+    __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
+    const uint32_t l = blockIdx.x;
+    const uint32_t* in = fcnt + (uint64_t)l * tiles;
+    uint32_t* out = fbase + (uint64_t)l * tiles;
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < tiles; t0 += kFusedThreads) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t v = t < tiles ? in[t] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kFusedThreads>(v, s_tmp, &total);
+        if (t < tiles) out[t] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        a.counts[l] = carry;
+        if (carry > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
+    }
+}
+
+__device__ inline uint32_t block_min_u32(uint32_t v, uint32_t* tmp) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    if ((threadIdx.x & 63u) == 0u) tmp[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t m = tmp[0];
+    for (uint32_t w = 1; w < blockDim.x / 64u; w++) m = min(m, tmp[w]);
+    __syncthreads();
+    return m;
+}
+
+template <typename AGG>
+__device__ inline void agg_particles(const TreeArgs& a, uint32_t j0, uint32_t j1, float* cm) {
+    AGG g;
+    for (uint32_t i = j0; i < j1; i++) {
+        const float4 x = a.pos[a.sorted_idx[i]];
+        g.add(x.x, x.y, x.z, x.w);
+    }
+    g.fin(cm);
+}
+template <typename AGG>
+__device__ inline void agg_nodes(const float4* v, uint32_t c0, uint32_t c1, float* cm) {
+    AGG g;
+    for (uint32_t i = c0; i < c1; i++) g.add(v[i].x, v[i].y, v[i].z, v[i].w);
+    g.fin(cm);
+}
+
+__global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
+                                                               const uint8_t* __restrict__ plevel,
+                                                               const uint32_t* __restrict__ fcnt,
+                                                               const uint32_t* __restrict__ fbase,
+                                                               uint32_t* ffix, uint32_t lf,
+                                                               uint32_t tiles) {
+    __shared__ float4 s_val[kFusedRange];     // the current layer's nodes (centre, mass)
+    __shared__ uint16_t s_rank[kFusedRange];  // per position: nodes of the layer before it
+    __shared__ uint16_t s_pos[kFusedRange];   // per node of the layer: its start position
+    __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const uint32_t a0 = t * kFusedTile;
+    const uint32_t b0 = min(a0 + kFusedTile, a.nt);
+    const uint32_t own = b0 - a0;  // positions [0, own) are this tile's
+    if (tid < lf) ffix[tid * tiles + t] = 0xFFFFFFFFu;
+    // the range ends at the next head at layer lf-1 after the tile (every node of the lower
+    // layers ends there), or at the halo's end
+    uint32_t cand = kFusedHalo;
+    {
+        const uint32_t j = b0 + tid;
+        if (tid < kFusedHalo && j < a.nt) {
+            const uint32_t h = plevel[j];
+            if (h != kNoHead && h >= lf - 1) cand = tid;
+        }
+    }
+    const uint32_t m = block_min_u32(cand, s_tmp);
+    uint32_t e0;
+    bool truncated = false;
+    uint32_t h_end = kNoHead;
+    if (m < kFusedHalo) {
+        e0 = b0 + m;
+    } else if (b0 + kFusedHalo >= a.nt) {
+        e0 = a.nt;
+    } else {
+        e0 = b0 + kFusedHalo;
+        truncated = true;
+        h_end = plevel[e0];
+    }
+    const uint32_t np = e0 - a0;
+    const uint32_t q0 = tid, q1 = tid + kFusedThreads;
+    const uint32_t h0 = q0 < np ? plevel[a0 + q0] : kNoHead;
+    const uint32_t h1 = q1 < np ? plevel[a0 + q1] : kNoHead;
+    uint32_t tot_prev = 0;
+    for (uint32_t l = 0; l < lf; l++) {
+        const bool f0 = h0 != kNoHead && h0 >= l, f1 = h1 != kNoHead && h1 >= l;
+        uint32_t t0, t1;
+        const uint32_t r0 = block_exclusive_scan<kFusedThreads>(f0 ? 1u : 0u, s_tmp, &t0);
+        const uint32_t r1 = t0 + block_exclusive_scan<kFusedThreads>(f1 ? 1u : 0u, s_tmp, &t1);
+        const uint32_t tot = t0 + t1;
+        if (f0) s_pos[r0] = (uint16_t)q0;
+        if (f1) s_pos[r1] = (uint16_t)q1;
+        if (l == 0) {  // the leaf of every position of the tile
+            const uint32_t base0 = fbase[t];
+            if (q0 < own) a.leaf_of_sorted[a0 + q0] = base0 + r0 + (f0 ? 1u : 0u) - 1u;
+            if (q1 < own) a.leaf_of_sorted[a0 + q1] = base0 + r1 + (f1 ? 1u : 0u) - 1u;
+        }
+        __syncthreads();
+        // the last node ends at the range's end, which starts a node of this layer unless the
+        // range was cut short
+        const bool end_ok = !truncated || (h_end != kNoHead && h_end >= l);
+        const bool fp64 = l >= a.fp64_from;
+        float cm[2][4];
+        uint32_t st[2], c0[2], c1[2];
+        bool done[2];
+#pragma unroll
+        for (uint32_t i = 0; i < 2; i++) {
+            const uint32_t k = tid + i * kFusedThreads;
+            done[i] = false;
+            if (k >= tot) continue;
+            st[i] = s_pos[k];
+            const uint32_t nx = k + 1 < tot ? s_pos[k + 1] : np;
+            done[i] = k + 1 < tot || end_ok;
+            if (l == 0) {
+                c0[i] = a0 + st[i];
+                c1[i] = a0 + nx;
+                if (fp64) agg_particles<Agg64>(a, c0[i], c1[i], cm[i]);
+                else agg_particles<Agg32>(a, c0[i], c1[i], cm[i]);
+            } else {
+                c0[i] = s_rank[st[i]];
+                c1[i] = nx < np ? s_rank[nx] : tot_prev;
+                if (fp64) agg_nodes<Agg64>(s_val, c0[i], c1[i], cm[i]);
+                else agg_nodes<Agg32>(s_val, c0[i], c1[i], cm[i]);
+            }
+        }
+        __syncthreads();  // the layer below is read; overwrite it
+        const uint32_t base = fbase[(uint64_t)l * tiles + t];
+        const uint32_t base_prev = l ? fbase[(uint64_t)(l - 1) * tiles + t] : 0u;
+#pragma unroll
+        for (uint32_t i = 0; i < 2; i++) {
+            const uint32_t k = tid + i * kFusedThreads;
+            if (k >= tot) continue;
+            s_val[k] = make_float4(cm[i][0], cm[i][1], cm[i][2], cm[i][3]);
+            if (st[i] >= own) continue;  // a halo node: owned by a later tile
+            const uint32_t gi = base + k;
+            if (gi >= a.cap[l]) continue;  // ERR_CAPACITY is raised by the scan
+            if (!done[i]) {  // runs past the halo: k_fused_fix builds it
+                ffix[l * tiles + t] = l == 0 ? c0[i] : base_prev + c0[i];
+                continue;
+            }
+            const uint32_t h = plevel[a0 + st[i]];
+            const uint64_t nd = a.off[l] + gi;
+            TNode tn;
+            tn.x = cm[i][0];
+            tn.y = cm[i][1];
+            tn.z = cm[i][2];
+            tn.m = cm[i][3];
+            tn.first = l == 0 ? ASTRO_INVALID_CHILD : (uint32_t)(a.off[l - 1] + base_prev + c0[i]);
+            tn.info = tn_info(l, l == 0 ? 0u : c1[i] - c0[i], h);
+            tn.key = a.sorted_keys[a0 + st[i]];
+            a.tnodes[nd] = tn;
+            a.levels[nd] = (uint8_t)h;
+        }
+        // per position, the nodes of this layer before it (the next layer's child ranks)
+        if (q0 < np) s_rank[q0] = (uint16_t)r0;
+        if (q1 < np) s_rank[q1] = (uint16_t)r1;
+        tot_prev = tot;
+        __syncthreads();
+    }
+}
+
+// The nodes k_fused_build left (they run past a halo), layer by layer: each tile's last node
+// of the layer, from the layer below, which is complete by then.
+__global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const uint32_t* fcnt,
+                                                             const uint32_t* fbase,
+                                                             const uint32_t* ffix, uint32_t lf,
+                                                             uint32_t tiles) {
+    for (uint32_t l = 0; l < lf; l++) {
+        for (uint32_t t = threadIdx.x; t < tiles; t += kFusedThreads) {
+            const uint32_t v = ffix[(uint64_t)l * tiles + t];
+            if (v == 0xFFFFFFFFu) continue;
+            const uint64_t lt = (uint64_t)l * tiles + t;
+            const uint32_t p = fbase[lt] + fcnt[lt] - 1u;
+            if (p >= a.cap[l]) continue;
+            if (l == 0) {
+                if (a.fp64_from == 0) emit_leaf<Agg64, false>(a, v, p);
+                else emit_leaf<Agg32, false>(a, v, p);
+            } else {
+                const uint64_t cb = a.off[l - 1];
+                const uint32_t capc = (uint32_t)a.cap[l - 1];
+                uint32_t lv[8];
+                for (uint32_t j = 0; j < 8; j++) lv[j] = v + j < capc ? a.levels[cb + v + j] : 0u;
+                const TNode u0 = a.tnodes[cb + v];
+                emit_parent<false>(a, l, v, p, lv, u0, a.counts[l - 1]);
+            }
+        }
+        __syncthreads();  // layer l is complete before layer l + 1 reads it
+    }
+}
+
+void launch_fused_layers(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
+                         hipStream_t s) {
+    const uint32_t tiles = fused_tiles(a.nt);
+    if (!a.nt || lf == 0) return;
+    uint32_t* fcnt = fbuf;
+    uint32_t* fbase = fbuf + (uint64_t)lf * tiles;
+    uint32_t* ffix = fbuf + 2ull * lf * tiles;
+    hipLaunchKernelGGL(k_fused_heads, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt, lf,
+                       tiles);
+    hipLaunchKernelGGL(k_fused_scan, dim3(lf), dim3(kFusedThreads), 0, s, a, fcnt, fbase, tiles);
+    hipLaunchKernelGGL(k_fused_build, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt,
+                       fbase, ffix, lf, tiles);
+    hipLaunchKernelGGL(k_fused_fix, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase, ffix, lf,
+                       tiles);
+}
+
 uint32_t layer_tiles(uint64_t candidates) {
     return (uint32_t)((candidates + kLayerTile - 1) / kLayerTile);
 }
@@ -727,10 +991,12 @@ uint64_t layer_super_words(uint64_t candidates) {
     return (uint64_t)((layer_tiles(candidates) + kSuper - 1) / kSuper) * kSuperStride;
 }
 
-void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s) {
-    if (!a.nt) return;
-    hipLaunchKernelGGL(k_layer_count, dim3(layer_tiles(a.nt)), dim3(256), 0, s, a, 0u, tile_cnt,
-                       super);
+void launch_layer_count(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, uint32_t* super,
+                        hipStream_t s) {
+    const uint64_t cand_cap = layer == 0 ? a.nt : a.cap[layer - 1];
+    if (!cand_cap) return;
+    hipLaunchKernelGGL(k_layer_count, dim3(layer_tiles(cand_cap)), dim3(256), 0, s, a, layer,
+                       tile_cnt, super);
 }
 
 uint32_t layers_top_first(const TreeArgs& a) {

@@ -99,7 +99,15 @@ void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const 
 // layer_super_words(...) words of supertile sums (spread one per 128-byte line).
 uint32_t layer_tiles(uint64_t candidates);
 uint64_t layer_super_words(uint64_t candidates);
-void launch_layer_count(const TreeArgs& a, uint32_t* tile_cnt, uint32_t* super, hipStream_t s);
+void launch_layer_count(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, uint32_t* super,
+                        hipStream_t s);
+// Layers 0 .. lf-1 (lf <= kFusedMaxLayers) in four launches (monopole, no reference quirk):
+// plevel holds nt bytes and fbuf fused_buf_words(nt) words, both scratch.
+constexpr uint32_t kFusedMaxLayers = 16;
+uint32_t fused_tiles(uint32_t nt);
+inline uint64_t fused_buf_words(uint32_t nt) { return 3ull * kFusedMaxLayers * fused_tiles(nt); }
+void launch_fused_layers(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
+                         hipStream_t s);
 // The top layers with at most kTopCandidates candidates each in one single-block launch:
 // layers_top_first(a) is the first of them (a.layers if none; layer 0 never), and the layer
 // before it is emitted without next-layer counts.

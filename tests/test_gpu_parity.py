@@ -433,14 +433,18 @@ def test_lighting_stage_and_step(astro, orc, small_ctx):
     assert np.array_equal(p2["position"], p["position"])
 
 
-EDGE_CASES = ["one", "two", "all_equal", "pairs_equal", "massless_half", "b1_tree", "line"]
+EDGE_CASES = ["one", "two", "all_equal", "pairs_equal", "massless_half", "b1_tree", "line",
+              "equal_run_3000", "dense_cluster"]
 
 
 @pytest.mark.parametrize("kind", EDGE_CASES)
 def test_edge_cases_bitexact(astro, orc, small_ctx, kind):
     # degenerate inputs through the fused build, traversal and 3 steps, against the oracle:
     # one and two particles, every particle on one point (one leaf), coincident pairs
-    # (key collisions), massless particles, the smallest tree (B = 1), a 1-D line
+    # (key collisions), massless particles, the smallest tree (B = 1), a 1-D line; a leaf of
+    # 3,000 equal keys and a cluster of 5,000 particles inside one node of layer 11, so that
+    # nodes of the lower layers run past a 1,024-position tile and its halo (the build's
+    # leftover pass)
     say("edge case", kind)
     rng = np.random.default_rng(17)
     bits = 21
@@ -457,6 +461,13 @@ def test_edge_cases_bitexact(astro, orc, small_ctx, kind):
         xyz, m = rng.normal(0, 1, (1000, 3)), np.where(np.arange(1000) % 2 == 0, 0.0, 1e-3)
     elif kind == "b1_tree":
         xyz, m, bits = rng.uniform(-1, 1, (2000, 3)), np.full(2000, 5e-4), 1
+    elif kind == "equal_run_3000":
+        xyz = np.concatenate([np.full((3000, 3), 0.3), rng.uniform(-5, 5, (2000, 3))])
+        m = rng.uniform(0.5, 1.5, 5000) * 1e-3
+    elif kind == "dense_cluster":
+        xyz = np.concatenate([0.37 + rng.uniform(0, 1e-3, (5000, 3)),
+                              rng.uniform(-5, 5, (3000, 3))])
+        m = rng.uniform(0.5, 1.5, 8000) * 1e-3
     else:
         t = np.linspace(-4, 4, 1500)
         xyz, m = np.stack([t, np.zeros_like(t), np.zeros_like(t)], 1), np.full(1500, 1e-3)
