@@ -487,11 +487,11 @@ __device__ inline void count_next_heads(uint32_t l, uint32_t first, uint32_t tot
                                         uint32_t* s_cnt, uint32_t* next_cnt,
                                         uint32_t* next_super) {
     if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
+    lds_barrier();  // the node stores before it stay in flight
     const uint32_t t0 = first / kLayerTile;
     for (uint32_t i = 0; i < k; i++)
         if (lev[i] >= l + 1) atomicAdd(&s_cnt[pos[i] / kLayerTile - t0], 1u);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x < 3 && total) {
         const uint32_t c = s_cnt[threadIdx.x];
         const uint32_t t = t0 + threadIdx.x;
@@ -744,11 +744,37 @@ constexpr uint8_t kNoHead = 0xFF;
 
 uint32_t fused_tiles(uint32_t nt) { return (nt + kFusedTile - 1) / kFusedTile; }
 
+// block_exclusive_scan with LDS-only barriers (tmp: NT / 64 + 1 words).  Without the closing
+// barrier the caller must pass another barrier before tmp is written again.
+template <int NT, bool CLOSE = true>
+__device__ inline uint32_t lds_block_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) tmp[wave] = inc;
+    lds_barrier();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < NW; w++) {
+            const uint32_t t = tmp[w];
+            tmp[w] = acc;
+            acc += t;
+        }
+        tmp[NW] = acc;
+    }
+    lds_barrier();
+    const uint32_t r = tmp[wave] + inc - v;
+    *total = tmp[NW];
+    if (CLOSE) lds_barrier();
+    return r;
+}
+
 __global__ __launch_bounds__(kFusedThreads) void k_fused_heads(TreeArgs a, uint8_t* plevel,
-                                                               uint32_t* fcnt, uint32_t lf,
-                                                               uint32_t tiles) {
+                                                               uint32_t* fcnt, uint32_t* ffix,
+                                                               uint32_t lf, uint32_t tiles) {
     __shared__ uint32_t s_hist[kMaxLayers];
     if (threadIdx.x < kMaxLayers) s_hist[threadIdx.x] = 0;
+    if (threadIdx.x < lf) ffix[threadIdx.x * tiles + blockIdx.x] = 0xFFFFFFFFu;  // no leftover
     __syncthreads();
     const uint32_t j = blockIdx.x * kFusedTile + threadIdx.x;
     if (j < a.nt) {
@@ -796,10 +822,10 @@ __device__ inline uint32_t block_min_u32(uint32_t v, uint32_t* tmp) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
     if ((threadIdx.x & 63u) == 0u) tmp[threadIdx.x >> 6] = v;
-    __syncthreads();
+    lds_barrier();
     uint32_t m = tmp[0];
     for (uint32_t w = 1; w < blockDim.x / 64u; w++) m = min(m, tmp[w]);
-    __syncthreads();
+    lds_barrier();
     return m;
 }
 
@@ -833,7 +859,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
     const uint32_t a0 = t * kFusedTile;
     const uint32_t b0 = min(a0 + kFusedTile, a.nt);
     const uint32_t own = b0 - a0;  // positions [0, own) are this tile's
-    if (tid < lf) ffix[tid * tiles + t] = 0xFFFFFFFFu;
     // the range ends at the next head at layer lf-1 after the tile (every node of the lower
     // layers ends there), or at the halo's end
     uint32_t cand = kFusedHalo;
@@ -858,16 +883,17 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
         h_end = plevel[e0];
     }
     const uint32_t np = e0 - a0;
-    const uint32_t q0 = tid, q1 = tid + kFusedThreads;
+    const uint32_t q0 = 2 * tid, q1 = 2 * tid + 1;  // two adjacent positions per thread
     const uint32_t h0 = q0 < np ? plevel[a0 + q0] : kNoHead;
     const uint32_t h1 = q1 < np ? plevel[a0 + q1] : kNoHead;
     uint32_t tot_prev = 0;
     for (uint32_t l = 0; l < lf; l++) {
         const bool f0 = h0 != kNoHead && h0 >= l, f1 = h1 != kNoHead && h1 >= l;
-        uint32_t t0, t1;
-        const uint32_t r0 = block_exclusive_scan<kFusedThreads>(f0 ? 1u : 0u, s_tmp, &t0);
-        const uint32_t r1 = t0 + block_exclusive_scan<kFusedThreads>(f1 ? 1u : 0u, s_tmp, &t1);
-        const uint32_t tot = t0 + t1;
+        uint32_t tot;
+        // s_tmp is next written after this layer's further barriers
+        const uint32_t r0 = lds_block_scan<kFusedThreads, false>(
+            (f0 ? 1u : 0u) + (f1 ? 1u : 0u), s_tmp, &tot);
+        const uint32_t r1 = r0 + (f0 ? 1u : 0u);
         if (f0) s_pos[r0] = (uint16_t)q0;
         if (f1) s_pos[r1] = (uint16_t)q1;
         if (l == 0) {  // the leaf of every position of the tile
@@ -875,7 +901,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             if (q0 < own) a.leaf_of_sorted[a0 + q0] = base0 + r0 + (f0 ? 1u : 0u) - 1u;
             if (q1 < own) a.leaf_of_sorted[a0 + q1] = base0 + r1 + (f1 ? 1u : 0u) - 1u;
         }
-        __syncthreads();
+        lds_barrier();
         // the last node ends at the range's end, which starts a node of this layer unless the
         // range was cut short
         const bool end_ok = !truncated || (h_end != kNoHead && h_end >= l);
@@ -903,7 +929,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
                 else agg_nodes<Agg32>(s_val, c0[i], c1[i], cm[i]);
             }
         }
-        __syncthreads();  // the layer below is read; overwrite it
+        lds_barrier();  // the layer below is read; overwrite it
         const uint32_t base = fbase[(uint64_t)l * tiles + t];
         const uint32_t base_prev = l ? fbase[(uint64_t)(l - 1) * tiles + t] : 0u;
 #pragma unroll
@@ -935,7 +961,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
         if (q0 < np) s_rank[q0] = (uint16_t)r0;
         if (q1 < np) s_rank[q1] = (uint16_t)r1;
         tot_prev = tot;
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -975,8 +1001,8 @@ void launch_fused_layers(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32
     uint32_t* fcnt = fbuf;
     uint32_t* fbase = fbuf + (uint64_t)lf * tiles;
     uint32_t* ffix = fbuf + 2ull * lf * tiles;
-    hipLaunchKernelGGL(k_fused_heads, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt, lf,
-                       tiles);
+    hipLaunchKernelGGL(k_fused_heads, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt, ffix,
+                       lf, tiles);
     hipLaunchKernelGGL(k_fused_scan, dim3(lf), dim3(kFusedThreads), 0, s, a, fcnt, fbase, tiles);
     hipLaunchKernelGGL(k_fused_build, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt,
                        fbase, ffix, lf, tiles);
