@@ -946,16 +946,9 @@ static int enqueue_build(astro_ctx* c) {
     return enqueue_layers(c, true);
 }
 
-// The traversal; with update_dt non-null the integrator follows in the same kernel.
-static int enqueue_forces(astro_ctx* c, unsigned long long* stats,
-                          const float* update_dt = nullptr) {
+static int enqueue_forces(astro_ctx* c, unsigned long long* stats) {
     ForceArgs a = force_args(c);
     a.stats = stats;
-    if (update_dt) {
-        a.update = 1;
-        a.dt = *update_dt;
-        a.vel = c->vel;
-    }
     launch_force_compact(a, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -994,15 +987,11 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     if (ev) record_stage(c, ev[2]);
     if ((rc = enqueue_layers(c, emit))) return rc;
     if (ev) record_stage(c, ev[3]);
-    // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres.  Without a lighting pass
-    // between them the integrator runs in the traversal kernel (the update stage is then empty).
-    const bool fused = c->params.lighting_stars == 0;
-    if ((rc = enqueue_forces(c, nullptr, fused ? &dt : nullptr))) return rc;
+    if ((rc = enqueue_forces(c, nullptr))) return rc;
     if (ev) record_stage(c, ev[4]);
-    if (!fused) {
-        enqueue_lighting(c, c->params.lighting_stars);
-        launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
-    }
+    // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
+    if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
+    launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
     if (ev) record_stage(c, ev[5]);
     LAUNCHCHK(c);
     return ASTRO_OK;

@@ -154,27 +154,6 @@ struct Visits {
     unsigned long long visits = 0, leafs = 0, monos = 0, opens = 0;
 };
 
-// The integrator of particle i with force f (BarnesHut.metal:169-196, updateSpheres):
-// a = F*scale[type]; v = v*damping + a*dt; x = x + v*dt -- separate roundings, no contraction.
-__device__ __forceinline__ void integrate(float4* pos, float4* vel, float4 f, uint32_t t,
-                                          uint32_t i, float dt, const astro_params& p) {
-    const float sc = t < 3u ? p.type_scale[t] : 1.0f;
-    float4 v = vel[i];
-    float4 x = pos[i];
-    float ax = f.x * sc, ay = f.y * sc, az = f.z * sc;
-    v.x = v.x * p.damping;
-    v.y = v.y * p.damping;
-    v.z = v.z * p.damping;
-    v.x = v.x + ax * dt;
-    v.y = v.y + ay * dt;
-    v.z = v.z + az * dt;
-    x.x = x.x + v.x * dt;
-    x.y = x.y + v.y * dt;
-    x.z = x.z + v.z * dt;
-    vel[i] = v;
-    pos[i] = x;
-}
-
 // One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
 // potential.  QUAD: accepted internal nodes add their quadrupole term.  FAST: a.fast holds (no
 // floor or cut-off can bind, square root without denormal scaling; exact for finite positions).
@@ -365,12 +344,8 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
             traverse<NT, KAHAN, MODE, QUAD, FAST, false>(a, depth, s_stack, tid, p, self, sys,
                                                          rmin0, 0.0f, F, phi, st);
         const float c = a.p.force_clamp;
-        const float4 f = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
-        a.force[pidx] = f;
+        a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
-        // every particle is exactly one target and no thread reads another target's position
-        // (node centres live in the tree), so the update can follow in place
-        if (a.update) integrate(const_cast<float4*>(a.pos), a.vel, f, type, pidx, a.dt, a.p);
     }
     if (MODE == 1) {
         const unsigned long long v = wave_sum_u64(st.visits), lf = wave_sum_u64(st.leafs),
@@ -578,14 +553,31 @@ void launch_force_reference(const ForceArgs& a, hipStream_t s) {
                        depth * 64u * sizeof(uint32_t), s, a, depth);
 }
 
-// The integrator as its own pass (after a lighting pass, or for the stage call).
+// BarnesHut.metal:169-196 (updateSpheres): a = F*scale[type]; v = v*damping + a*dt;
+// x = x + v*dt -- separate roundings, no contraction.
 __global__ __launch_bounds__(256) void k_update(float4* __restrict__ pos, float4* __restrict__ vel,
                                                 const float4* __restrict__ force,
                                                 const uint32_t* __restrict__ type, uint32_t n,
                                                 float dt, astro_params p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    integrate(pos, vel, force[i], type[i], i, dt, p);
+    const uint32_t t = type[i];
+    const float sc = t < 3u ? p.type_scale[t] : 1.0f;
+    const float4 f = force[i];
+    float4 v = vel[i];
+    float4 x = pos[i];
+    float ax = f.x * sc, ay = f.y * sc, az = f.z * sc;
+    v.x = v.x * p.damping;
+    v.y = v.y * p.damping;
+    v.z = v.z * p.damping;
+    v.x = v.x + ax * dt;
+    v.y = v.y + ay * dt;
+    v.z = v.z + az * dt;
+    x.x = x.x + v.x * dt;
+    x.y = x.y + v.y * dt;
+    x.z = x.z + v.z * dt;
+    vel[i] = v;
+    pos[i] = x;
 }
 
 void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,

@@ -167,10 +167,6 @@ struct ForceArgs {
     AcceptRule acc;             // opening test (accept.hpp), from p.theta
     uint32_t fast;              // 1: eps2 >= max(r2_floor, 2^-96, leaf_cutoff_r2+),
                                 //    sqrt(eps2) >= r_floor: no floor or cut-off can bind
-    float4* vel;                // with update: velocities, integrated in place
-    uint32_t update;            // 1: the integrator (k_update's arithmetic) follows the force
-                                //    in the same thread, which then writes pos and vel
-    float dt;                   // with update: the time step
     float div_lmax;             // > 0: when a system's cube lies within [-div_lmax, div_lmax]^3,
                                 //    every G M / (r2 r) of its targets is in the domain of
                                 //    div_rn_mid; 0: always use the full division

@@ -447,11 +447,7 @@ def main():
     for s, ms in stage_ms.items():
         per = ms / K
         e = {"ms": round(per, 4)}
-        if s == "update" and P.lighting_stars == 0:
-            # no lighting pass: the integrator runs in the traversal kernel's epilogue (the
-            # stage time left is the event pair's own)
-            e["fused_into"] = "force"
-        elif s in sb:
+        if s in sb:
             e["gbs"] = round(sb[s] / (per * 1e-3) / 1e9, 1)
             e["hbm_frac"] = round(e["gbs"] / PEAK_HBM_GBS, 4)
         stages[s] = e
