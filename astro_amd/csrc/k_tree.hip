@@ -961,7 +961,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
         if (q0 < np) s_rank[q0] = (uint16_t)r0;
         if (q1 < np) s_rank[q1] = (uint16_t)r1;
         tot_prev = tot;
-        lds_barrier();
+        // no barrier here: the next layer's scan passes two before anything reads s_val,
+        // s_rank or s_pos, and its first LDS write (s_tmp) was last read before this layer's
+        // own later barriers
     }
 }
 
