@@ -156,7 +156,8 @@ int astro_compute_forces(astro_ctx* ctx);
  * astro_params.lighting_stars > 0) + update, nsteps times. */
 int astro_step(astro_ctx* ctx, float dt, uint32_t nsteps);
 /* Same as astro_step with hipEvents around each stage of each step; stage_ms receives the
- * per-stage sums (ASTRO_STAGE_COUNT floats). */
+ * per-stage sums (ASTRO_STAGE_COUNT floats).  With graphs on (the default) the nsteps steps are
+ * captured as one hipGraph whose event nodes time the stages, and replayed once. */
 enum {
     ASTRO_STAGE_BBOX_MORTON = 0,
     ASTRO_STAGE_SORT = 1,
