@@ -42,7 +42,6 @@ __device__ inline uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Inclusive scan across the 64 lanes of a wave.
 // A workgroup barrier that orders LDS only.  __syncthreads also waits for the block's global
 // stores to complete; a kernel that never reads its own stores back lets them drain in the
 // background across this barrier.
@@ -52,6 +51,7 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Inclusive scan across the 64 lanes of a wave.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
