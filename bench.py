@@ -451,9 +451,12 @@ def main():
                    st1["monopole_terms"])
     achieved = FLOP_PER_INTERACTION * inter / (force_ms * 1e-3) / 1e12
     traffic = None
+    valu_frac = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("force_compact_bytes_per_launch")
+        tj = json.load(open(tpath))
+        traffic = tj.get("force_compact_bytes_per_launch")
+        valu_frac = tj.get("valu_issue_frac")
     sb = stage_bytes(n, V)
     stages = {}
     for s, ms in stage_ms.items():
@@ -487,6 +490,10 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "frac_nonpacked": achieved / PEAK_FP32_TFLOPS_NONPACKED,
                      "traffic": traffic, "kernel_ms": force_ms,
+                     "valu_issue_frac": valu_frac,
+                     "valu_issue_what": "SQ_INSTS_VALU x 4 cycles per wave64 instruction over "
+                                        "1024 SIMDs at 2.4 GHz, against the kernel's duration "
+                                        "(rocprofv3, profiles/)",
                      "interactions_per_step": inter, "flop_per_interaction": FLOP_PER_INTERACTION,
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
         "stages": stages,
