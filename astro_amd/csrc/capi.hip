@@ -57,6 +57,7 @@ static void invalidate_graph(astro_ctx* c) {
 }
 
 static int refresh_layers(astro_ctx* c) {
+    c->tree_built = false;  // the layer table may change: build before traversing again
     std::vector<uint32_t> sizes;
     if (c->nsys <= 1) sizes.push_back(c->nt);
     else
@@ -935,6 +936,7 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     }
     launch_layers_top(a, l0, s);
     if (a.onodes && emit) enqueue_emit(c, s);
+    c->tree_built = true;
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -947,6 +949,10 @@ static int enqueue_build(astro_ctx* c) {
 }
 
 static int enqueue_forces(astro_ctx* c, unsigned long long* stats) {
+    // the traversal trusts the tree's child indices (no per-visit bounds check): only a tree
+    // built on the current layer table is traversed
+    if (!c->tree_built)
+        return fail(c, ASTRO_ERR_STATE, "no tree for the current particles: call astro_build_tree");
     ForceArgs a = force_args(c);
     a.stats = stats;
     launch_force_compact(a, c->stream);

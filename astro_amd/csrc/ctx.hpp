@@ -18,6 +18,7 @@ struct astro_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t num_cus = 1;              // compute units of the device (emission grid)
+    bool tree_built = false;           // tnodes hold a tree built on the current layer table
     std::string err;
 
     uint32_t ns = 0, nt = 0, nsys = 1;

@@ -170,10 +170,11 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
     constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
-    const uint32_t node_total = (uint32_t)a.node_total;  // node indices are 32-bit
     const char* tn_base = reinterpret_cast<const char*>(a.tnodes);
-    // The loop leaves only at its head: an overflow or a bad index empties the stack instead
-    // of breaking out of a divergent branch.
+    // The loop leaves only at its head: an overflow empties the stack instead of breaking out
+    // of a divergent branch.  Node indices are not bounds-checked per visit: the host traverses
+    // only a tree built on the current layer table, and the kernel none whose build overflowed
+    // a layer, so every index is the root or a child run of a written node.
     uint32_t top_first = (uint32_t)(a.root_off + sys), top_cnt = 1, sp = 0;
     while (true) {
         if (top_cnt == 0) {
@@ -184,12 +185,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
             top_cnt = (e & 7u) + 1u;
         }
         top_cnt--;
-        uint32_t n = top_first + top_cnt;
-        // a bad index contributes nothing and empties the stack (the loop then ends at its
-        // head); every other path updates F the same way, so F needs no copies
-        const bool bad = n >= node_total;
-        if (bad) atomicOr(a.err, ERR_NODE_INDEX);
-        n = bad ? 0u : n;
+        const uint32_t n = top_first + top_cnt;
         // 32-bit byte offset from a uniform base (n < 2^27 for any context that fits)
         const float4* np = reinterpret_cast<const float4*>(tn_base + (size_t)(n << 5));
         const float4 cm = np[0];
@@ -215,8 +211,8 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
             // second memory round trip on the loop's critical path.
             __asm__ volatile("" : : "v"(gm));
             const bool far = r2 >= ldexpf(r2min0, 2 * (int)layer);
-            const bool interact = !bad & (leaf ? n != self : far);
-            open = !bad & !leaf & !far;
+            const bool interact = leaf ? n != self : far;
+            open = !leaf & !far;
             if (interact) {
                 const float rr = sqrt_rn_normal(r2);
                 const float D = r2 * rr;
@@ -231,7 +227,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
                 }
             }
             if (MODE == 1) {
-                st.visits += !bad;
+                st.visits += 1;
                 st.leafs += leaf && interact;
                 st.monos += !leaf && interact;
                 st.opens += open;
@@ -239,10 +235,10 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
         } else {
             const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
             const bool far = rr >= ldexpf(rmin0, (int)layer);
-            const bool hit = !bad & (n != self) & (FAST || r2 > cut);
-            const bool acc = !bad & far;
+            const bool hit = (n != self) & (FAST || r2 > cut);
+            const bool acc = far;
             const bool interact = leaf ? hit : acc;
-            open = !bad & !leaf & !far;
+            open = !leaf & !far;
             // the term is computed for every lane and kept only where it interacts
             const float D = r2 * rr;
             const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
@@ -269,7 +265,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
                 F.z = __builtin_fmaf(sk, dz, F.z);
             }
             if (MODE == 1) {
-                st.visits += !bad;
+                st.visits += 1;
                 st.leafs += leaf && interact;
                 st.monos += !leaf && interact;
                 st.opens += open;
@@ -280,9 +276,9 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
         const bool overflow = push & (sp >= depth);
         if (overflow) atomicOr(a.err, ERR_STACK);
         if (push & !overflow) s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
-        sp = (bad | overflow) ? 0u : sp + (push ? 1u : 0u);
+        sp = overflow ? 0u : sp + (push ? 1u : 0u);
         top_first = open ? li.x : top_first;
-        top_cnt = (bad | overflow) ? 0u : (open ? tn_nchild(li.y) : top_cnt);
+        top_cnt = overflow ? 0u : (open ? tn_nchild(li.y) : top_cnt);
     }
 }
 
@@ -307,6 +303,12 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
     }
     const uint32_t tid = threadIdx.x;
     const uint32_t j = tile * NT + tid;
+    // a build that overflowed a layer left the tree incomplete: no traversal (the error word,
+    // which astro_check reports and clears, already says so)
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(a.err, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) &
+        ERR_CAPACITY)
+        return;
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within

@@ -294,6 +294,24 @@ def test_two_disks_500k(astro, orc, multipole):
         assert ok == 4096
 
 
+def test_forces_need_a_built_tree(astro):
+    # the traversal trusts the tree's child indices, so the library refuses to traverse before
+    # a tree is built for the uploaded particles (and after a change of the layer table)
+    say("traversal refused without a tree")
+    pos, vel, col = astro.ic_plummer(1000, 3)
+    with astro.Context(1000, bits_per_axis=21) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload(pos, vel, col)
+        with pytest.raises(astro.AstroError):
+            ctx.compute_forces()
+        ctx.build_tree()
+        ctx.compute_forces()
+        ctx.upload(pos[:500], vel[:500], col[:500])
+        with pytest.raises(astro.AstroError):
+            ctx.compute_forces()
+        ctx.check()
+
+
 def test_astro_check_reports_clean(astro, small_ctx):
     say("error word clean")
     small_ctx.check()
