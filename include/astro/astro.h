@@ -148,9 +148,11 @@ int astro_lighting_pass(astro_ctx* ctx, uint32_t num_stars);
 int astro_update_spheres(astro_ctx* ctx, float dt);
 
 /* ---- the fused MI355X path ------------------------------------------------------------ */
-/* Renderer.swift:511-534 in one pass per layer: bbox, Morton, LSD sort, layered build. */
+/* Renderer.swift:511-534: bbox, Morton, LSD sort, layered build (lower layers fused). */
 int astro_build_tree(astro_ctx* ctx);
-/* Renderer.swift:536-538 over the compact tree: forces for every particle. */
+/* Renderer.swift:536-538 over the compact tree: forces for every particle.  Needs a tree built
+ * (astro_build_tree or a step) since the last change of the layer table (an upload of another
+ * particle count or system layout, a new bits_per_axis): ASTRO_ERR_STATE otherwise. */
 int astro_compute_forces(astro_ctx* ctx);
 /* Renderer.swift:495-542 with explicit dt: build + forces (+ the lighting pass when
  * astro_params.lighting_stars > 0) + update, nsteps times. */
@@ -167,7 +169,8 @@ enum {
     ASTRO_STAGE_COUNT = 5
 };
 int astro_step_timed(astro_ctx* ctx, float dt, uint32_t nsteps, float* stage_ms);
-/* Work figures of the current tree and positions (one extra counting traversal). */
+/* Work figures of the current tree and positions (one extra counting traversal; needs a built
+ * tree, as astro_compute_forces). */
 int astro_traversal_stats(astro_ctx* ctx, astro_stats* out);
 /* Traversal block size (64, 128, 256 or 512; default 256). */
 int astro_set_force_block(astro_ctx* ctx, uint32_t block);
