@@ -5,8 +5,9 @@
 // of the tree; blocks are mapped so each XCD (and its L2) takes one contiguous stretch of the
 // Morton curve.  The stack holds (first child, remaining count) runs, one per open level,
 // so its depth is bounded by the layer count less two (20 at 21 bits per axis) instead of 7
-// per level; it lives in LDS in a [depth][thread] layout.  Visit order equals the reference's LIFO order (children
-// pushed 0..7, popped 7..0), so per-target sums are bit-identical to the oracle.
+// per level; it lives in LDS in a [depth][thread] layout.  Visit order equals the reference's
+// LIFO order (children pushed 0..7, popped 7..0), so per-target sums are bit-identical to the
+// oracle.
 //
 // k_force_reference: the reference kernel shape (BarnesHut.metal:22-101) over the 128-B
 // OctreeNode array with children[8], for callers that built the tree with the stage calls.
@@ -163,7 +164,7 @@ struct Visits {
 // branch that waves without an interacting lane skip.  (For monopoles the branch costs more
 // than the square root and division it skips, so they stay straight-line.)
 template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV>
-__device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uint32_t* s_stack,
+__device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t* s_stack,
                                          uint32_t tid, float4 p, uint32_t self, uint32_t sys,
                                          float rmin0, float r2min0, Acc3& F, double& phi,
                                          Visits& st) {
@@ -171,10 +172,10 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
     const char* tn_base = reinterpret_cast<const char*>(a.tnodes);
-    // The loop leaves only at its head: an overflow empties the stack instead of breaking out
-    // of a divergent branch.  Node indices are not bounds-checked per visit: the host traverses
-    // only a tree built on the current layer table, and the kernel none whose build overflowed
-    // a layer, so every index is the root or a child run of a written node.
+    // The loop leaves only at its head.  Neither node indices nor the stack depth are checked
+    // per visit: the host traverses only a tree built on the current layer table, and the
+    // kernel none whose build overflowed a layer, so every index is the root or a child run of
+    // a written node, one layer below its parent.
     uint32_t top_first = (uint32_t)(a.root_off + sys), top_cnt = 1, sp = 0;
     while (true) {
         if (top_cnt == 0) {
@@ -273,19 +274,19 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t depth, uin
             if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         }
         const bool push = open & (top_cnt > 0u);
-        const bool overflow = push & (sp >= depth);
-        if (overflow) atomicOr(a.err, ERR_STACK);
-        if (push & !overflow) s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
-        sp = overflow ? 0u : sp + (push ? 1u : 0u);
+        // at most one entry per layer 1 .. layers-2 is on the stack (a run is pushed only when
+        // one of its nodes opens, and children lie one layer down), so depth never overflows
+        if (push) s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
+        sp += push ? 1u : 0u;
         top_first = open ? li.x : top_first;
-        top_cnt = overflow ? 0u : (open ? tn_nchild(li.y) : top_cnt);
+        top_cnt = open ? tn_nchild(li.y) : top_cnt;
     }
 }
 
 // Targets in Morton order (sorted position j), one per thread; the stack of (first child,
 // remaining count) runs lives in LDS as [depth][NT].
 template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST>
-__global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t depth) {
+__global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     extern __shared__ uint32_t s_stack[];  // [depth][NT]
 
     // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
@@ -339,11 +340,11 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a, uint32_t dept
         Acc3 F;
         double phi = 0.0;
         if (fdiv)
-            traverse<NT, KAHAN, MODE, QUAD, FAST, true>(a, depth, s_stack, tid, p, self, sys,
+            traverse<NT, KAHAN, MODE, QUAD, FAST, true>(a, s_stack, tid, p, self, sys,
                                                         rmin0, QUAD ? accept_r2min(rmin0) : 0.0f,
                                                         F, phi, st);
         else
-            traverse<NT, KAHAN, MODE, QUAD, FAST, false>(a, depth, s_stack, tid, p, self, sys,
+            traverse<NT, KAHAN, MODE, QUAD, FAST, false>(a, s_stack, tid, p, self, sys,
                                                          rmin0, 0.0f, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
@@ -437,13 +438,13 @@ static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
     const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
     if (mode == 0)
         hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
-                           s, a, depth);
+                           s, a);
     else if (mode == 1)
         hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
-                           s, a, depth);
+                           s, a);
     else
         hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
-                           s, a, depth);
+                           s, a);
 }
 
 template <int NT>
