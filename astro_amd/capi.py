@@ -115,6 +115,7 @@ SIGNATURES = [
     ("astro_particle_count", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
     ("astro_check_division", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
     ("astro_check_open_test", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
+    ("astro_check_sqrt", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     ("astro_snapshot_write", _I, [_P, ctypes.c_char_p, ctypes.c_double]),
     ("astro_snapshot_read", _I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
     ("astro_snapshot_info", _I, [ctypes.c_char_p, _P]),
@@ -478,6 +479,12 @@ class Context:
         m = ctypes.c_uint64()
         self._chk(self.lib.astro_check_division(self.h, n, seed, ctypes.byref(m)))
         return m.value
+
+    def check_sqrt(self):
+        """(floats checked, mismatches) of the exhaustive square-root self-check."""
+        n, m = ctypes.c_uint64(), ctypes.c_uint64()
+        self._chk(self.lib.astro_check_sqrt(self.h, ctypes.byref(n), ctypes.byref(m)))
+        return n.value, m.value
 
     def check_open_test(self, n, seed=1):
         """Disagreements of the squared opening test with the square-root form on n cases."""

@@ -1276,6 +1276,24 @@ extern "C" int astro_check_open_test(astro_ctx* c, uint64_t n, uint64_t seed,
     return ASTRO_OK;
 }
 
+// Checks the traversal's correctly rounded square root on every float in [2^-96, FLT_MAX];
+// *checked receives how many, *mismatches the disagreements with the double square root
+// rounded to float (0 expected).  Synchronous.
+extern "C" int astro_check_sqrt(astro_ctx* c, uint64_t* checked, uint64_t* mismatches) {
+    if (!c || !checked || !mismatches) return ASTRO_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(c->stats);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(d), 0ull, 1, c->stream);
+    const uint64_t n = check_sqrt(d, c->stream);
+    if (!n) return fail(c, ASTRO_ERR_HIP, "square-root check launch");
+    unsigned long long h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *checked = n;
+    *mismatches = h;
+    return ASTRO_OK;
+}
+
 // ------------------------------------------------------------------------- snapshots
 namespace {
 struct FileCloser {

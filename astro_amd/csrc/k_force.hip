@@ -429,6 +429,26 @@ int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, h
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Exhaustive self-check of sqrt_rn_normal: every float x in [2^-96, FLT_MAX] against the
+// double square root rounded to float (correctly rounded: 53 >= 2 * 24 + 2 bits make the
+// double rounding innocuous).  The raw hardware estimate alone differs on about 15% of them.
+__global__ void k_check_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatch) {
+    unsigned long long bad = 0;
+    for (uint64_t u = lo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < hi;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const float x = __uint_as_float((uint32_t)u);
+        const float ref = (float)__builtin_sqrt((double)x);
+        bad += __float_as_uint(sqrt_rn_normal(x)) != __float_as_uint(ref);
+    }
+    bad = wave_sum_u64(bad);
+    if ((threadIdx.x & 63u) == 0u && bad) atomicAdd(mismatch, bad);
+}
+uint64_t check_sqrt(unsigned long long* d_mismatch, hipStream_t s) {
+    const uint32_t lo = (127u - 96u) << 23, hi = 0x7F800000u;  // [2^-96, +inf)
+    hipLaunchKernelGGL(k_check_sqrt, dim3(8192), dim3(256), 0, s, lo, hi, d_mismatch);
+    return hipGetLastError() == hipSuccess ? (uint64_t)(hi - lo) : 0u;
+}
+
 template <int NT, bool KAHAN, bool QUAD, bool FAST>
 static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t blocks = (a.ns + NT - 1) / NT;

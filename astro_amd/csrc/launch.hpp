@@ -176,6 +176,9 @@ struct ForceArgs {
 int check_division(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s);
 // Self-check of the r2 form of the opening test against the square-root form.
 int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, hipStream_t s);
+// Exhaustive self-check of the correctly rounded square root; returns the floats checked (0 on
+// a launch error).
+uint64_t check_sqrt(unsigned long long* d_mismatch, hipStream_t s);
 // Fills acc and fast from p.
 void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);

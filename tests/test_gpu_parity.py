@@ -399,6 +399,14 @@ def test_fast_division_matches_ieee(astro, small_ctx):
     assert small_ctx.check_division(1 << 26, seed=12345) == 0
 
 
+def test_square_root_exhaustive(astro, small_ctx):
+    # the fast path's square root (hardware estimate + one-ulp correction) on every float in
+    # [2^-96, FLT_MAX] against the double square root rounded to float
+    say("exhaustive square-root self-check")
+    n, bad = small_ctx.check_sqrt()
+    assert n == (0x7F800000 - ((127 - 96) << 23)) and bad == 0
+
+
 def test_squared_opening_test_matches_sqrt(astro, small_ctx):
     # with quadrupoles and the fast division the traversal opens a node unless r2 >= c0 4^l,
     # in place of RN(sqrt(r2)) >= rmin0 2^l; 2^24 cases within 8 ulps of the threshold or near
