@@ -854,6 +854,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
     __shared__ float4 s_val[kFusedRange];     // the current layer's nodes (centre, mass)
     __shared__ uint16_t s_rank[kFusedRange];  // per position: nodes of the layer before it
     __shared__ uint16_t s_pos[kFusedRange];   // per node of the layer: its start position
+    __shared__ uint64_t s_key[kFusedRange];   // per position: its key (a node's key is its first)
+    __shared__ uint8_t s_h[kFusedRange];      // per position: its head level
     __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t a0 = t * kFusedTile;
@@ -886,6 +888,11 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
     const uint32_t q0 = 2 * tid, q1 = 2 * tid + 1;  // two adjacent positions per thread
     const uint32_t h0 = q0 < np ? plevel[a0 + q0] : kNoHead;
     const uint32_t h1 = q1 < np ? plevel[a0 + q1] : kNoHead;
+    // keys and levels of the range in LDS: the node stores of every layer read them
+    if (q0 < np) s_key[q0] = a.sorted_keys[a0 + q0];
+    if (q1 < np) s_key[q1] = a.sorted_keys[a0 + q1];
+    s_h[q0] = (uint8_t)h0;
+    s_h[q1] = (uint8_t)h1;
     uint32_t tot_prev = 0;
     for (uint32_t l = 0; l < lf; l++) {
         const bool f0 = h0 != kNoHead && h0 >= l, f1 = h1 != kNoHead && h1 >= l;
@@ -944,7 +951,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
                 ffix[l * tiles + t] = l == 0 ? c0[i] : base_prev + c0[i];
                 continue;
             }
-            const uint32_t h = plevel[a0 + st[i]];
+            const uint32_t h = s_h[st[i]];
             const uint64_t nd = a.off[l] + gi;
             TNode tn;
             tn.x = cm[i][0];
@@ -953,7 +960,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             tn.m = cm[i][3];
             tn.first = l == 0 ? ASTRO_INVALID_CHILD : (uint32_t)(a.off[l - 1] + base_prev + c0[i]);
             tn.info = tn_info(l, l == 0 ? 0u : c1[i] - c0[i], h);
-            tn.key = a.sorted_keys[a0 + st[i]];
+            tn.key = s_key[st[i]];
             a.tnodes[nd] = tn;
             a.levels[nd] = (uint8_t)h;
         }
