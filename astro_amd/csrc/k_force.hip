@@ -163,7 +163,9 @@ struct Visits {
 // r2min0 4^l, so that the square root, the division and the quadrupole term are computed in a
 // branch that waves without an interacting lane skip.  (For monopoles the branch costs more
 // than the square root and division it skips, so they stay straight-line.)
-template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV>
+// OFF64: node byte offsets in 64 bits (trees of 2^27 nodes or more); otherwise 32-bit offsets
+// from the uniform base, which the loads take as a scalar base plus a vector offset.
+template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64>
 __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t* s_stack,
                                          uint32_t tid, float4 p, uint32_t self, uint32_t sys,
                                          float rmin0, float r2min0, Acc3& F, double& phi,
@@ -187,8 +189,9 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t* s_stack,
         }
         top_cnt--;
         const uint32_t n = top_first + top_cnt;
-        // 32-bit byte offset from a uniform base (n < 2^27 for any context that fits)
-        const float4* np = reinterpret_cast<const float4*>(tn_base + (size_t)(n << 5));
+        // byte offset from a uniform base: 32-bit while the tree has fewer than 2^27 nodes
+        const float4* np = reinterpret_cast<const float4*>(
+            tn_base + (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
         const float4 cm = np[0];
         const uint2 li = reinterpret_cast<const uint2*>(np)[2];
         // quadrupole mode: the moments are loaded with the node, not after the opening test
@@ -285,7 +288,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t* s_stack,
 
 // Targets in Morton order (sorted position j), one per thread; the stack of (first child,
 // remaining count) runs lives in LDS as [depth][NT].
-template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST>
+template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool OFF64>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     extern __shared__ uint32_t s_stack[];  // [depth][NT]
 
@@ -340,11 +343,11 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         Acc3 F;
         double phi = 0.0;
         if (fdiv)
-            traverse<NT, KAHAN, MODE, QUAD, FAST, true>(a, s_stack, tid, p, self, sys,
+            traverse<NT, KAHAN, MODE, QUAD, FAST, true, OFF64>(a, s_stack, tid, p, self, sys,
                                                         rmin0, QUAD ? accept_r2min(rmin0) : 0.0f,
                                                         F, phi, st);
         else
-            traverse<NT, KAHAN, MODE, QUAD, FAST, false>(a, s_stack, tid, p, self, sys,
+            traverse<NT, KAHAN, MODE, QUAD, FAST, false, OFF64>(a, s_stack, tid, p, self, sys,
                                                          rmin0, 0.0f, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
@@ -456,15 +459,26 @@ static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
     // root has none, leaves never open): at most layers - 2 = bits - 1 entries
     const uint32_t depth = a.p.bits_per_axis > 2u ? a.p.bits_per_axis - 1u : 1u;
     const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
-    if (mode == 0)
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
-                           s, a);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
-                           s, a);
-    else
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST>), dim3(blocks), dim3(NT), lds,
-                           s, a);
+    if (a.node_total >= (1ull << 27)) {  // byte offsets past 4 GiB
+        if (mode == 0)
+            hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, true>), dim3(blocks),
+                               dim3(NT), lds, s, a);
+        else if (mode == 1)
+            hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, true>), dim3(blocks),
+                               dim3(NT), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, true>), dim3(blocks),
+                               dim3(NT), lds, s, a);
+    } else if (mode == 0) {
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, false>), dim3(blocks),
+                           dim3(NT), lds, s, a);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, false>), dim3(blocks),
+                           dim3(NT), lds, s, a);
+    } else {
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, false>), dim3(blocks),
+                           dim3(NT), lds, s, a);
+    }
 }
 
 template <int NT>

@@ -258,6 +258,29 @@ def test_radix_sort_generic(astro, small_ctx, kind):
     assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), idx[:count][o])
 
 
+def test_tree_past_2_27_nodes(astro):
+    # 10.5 M particles at B = 21: the layer table holds more than 2^27 nodes, so node byte
+    # offsets pass 4 GiB and the traversal takes its 64-bit addressing; the forces of 512
+    # random targets must match the FP64 direct sum to the tree's usual accuracy
+    say("tree of more than 2^27 nodes")
+    n = 10_500_000
+    pos, vel, col = astro.ic_plummer(n, 9)
+    with astro.Context(n, bits_per_axis=21, emit_octree_nodes=False) as ctx:
+        P = astro.params_physics(21)
+        ctx.set_params(P)
+        ctx.upload(pos, vel, col)
+        cap, off = ctx.layer_table()
+        assert int(off[-1] + cap[-1]) >= 1 << 27
+        ctx.build_tree()
+        ctx.compute_forces()
+        ctx.check()
+        tg = np.sort(np.random.default_rng(3).choice(n, 512, replace=False))
+        ref = ctx.direct_forces_fp64(tg)
+        d = ctx.forces()[tg, :3].astype(np.float64) - ref
+        rms = float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))
+        assert rms < 2e-2, rms
+
+
 @pytest.mark.parametrize("multipole", [0, 2])
 def test_two_disks_500k(astro, orc, multipole):
     # the 500,000-particle two-disk system at 63-bit keys: whole tree bit-exact, forces on a
