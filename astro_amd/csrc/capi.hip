@@ -129,7 +129,8 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
                     c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
-                    c->leaf_of_sorted, c->cubes, c->minmax, c->cube_counter, c->plevel, c->fused_buf, c->err_word, c->stats, c->scratch,
+                    c->leaf_of_sorted, c->pnext, c->cubes, c->minmax, c->cube_counter, c->plevel, c->fused_buf,
+                    c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -219,6 +220,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->layer_counts, kMaxLayers);
     e = e ? e : dalloc(c->device_bytes, &c->layer_counts_prev, kMaxLayers);
     e = e ? e : dalloc(c->device_bytes, &c->leaf_of_sorted, NT);
+    e = e ? e : dalloc(c->device_bytes, &c->pnext, NT);
     e = e ? e : dalloc(c->device_bytes, &c->cubes, c->cfg.max_systems);
     e = e ? e : dalloc(c->device_bytes, &c->minmax,
                        std::max(6ull * c->cfg.max_systems, 8ull * cube_single_blocks(NP)));
@@ -854,6 +856,7 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.onodes = c->emit ? c->onodes : nullptr;
     a.counts = c->layer_counts;
     a.leaf_of_sorted = c->leaf_of_sorted;
+    a.pnext = c->pnext;
     a.err = c->err_word;
     for (uint32_t l = 0; l < c->layers; l++) {
         a.off[l] = c->off[l];
@@ -927,6 +930,8 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     if (!a.quad && !a.quirk_last_leaf && c->layers > 10)
         lf = std::min(std::min(l0, c->layers - 10), kFusedMaxLayers);
     if (lf < 2) lf = 0;
+    // the next pointers of every layer's nodes, before any node is written
+    launch_heads_next(a, lf, c->plevel, c->fused_buf, s);
     if (lf) launch_fused_layers(a, lf, c->plevel, c->fused_buf, s);
     if (lf < l0) launch_layer_count(a, lf, tc[lf], sup[lf], s);
     for (uint32_t l = lf; l < l0; l++) {

@@ -10,14 +10,24 @@ namespace astro {
 
 // Compact traversal node, 32 B (one half of a 64-B line).  Children of a node are a
 // contiguous run of the layer below (the build writes each layer in ascending key order),
-// so a (first, count) pair replaces the reference's children[8] (Astro/bridge.h:62).
+// so a (last, count) pair replaces the reference's children[8] (Astro/bridge.h:62).
+//
+// `next` threads the tree in the reference's visiting order (BarnesHut.metal:22-101 pushes
+// children 0..7 on a LIFO stack, so the last child is visited first): it is the node visited
+// after this node's subtree, i.e. the previous sibling, or for a first child its parent's
+// next (kNextEnd after the root).  A traversal then needs no stack: an opened node goes to its
+// last child, any other to its next.  Every node that starts at sorted position s has the
+// same next: the node of layer h(s) (the highest layer at which s starts a node) that holds
+// position s - 1, or kNextEnd where s starts a system.
 struct alignas(16) TNode {
     float x, y, z, m;   // centre of mass, total mass
-    uint32_t first;     // absolute index of the first child (leaves: ASTRO_INVALID_CHILD)
+    uint32_t last;      // absolute index of the last child (leaves: ASTRO_INVALID_CHILD)
     uint32_t info;      // layer | nchild << 5 | head level << 9
-    uint64_t key;       // full Morton key of the node's first particle (node key = key >> 3 layer)
+    uint32_t next;      // the node after this subtree in visiting order (kNextEnd: none)
+    uint32_t start;     // sorted position of the node's first particle (key = sorted key >> 3 layer)
 };
 static_assert(sizeof(TNode) == 32, "TNode must be 32 B");
+constexpr uint32_t kNextEnd = 0xFFFFFFFFu;
 
 __host__ __device__ inline uint32_t tn_info(uint32_t layer, uint32_t nchild, uint32_t h) {
     return layer | (nchild << 5) | (h << 9);
@@ -29,7 +39,7 @@ __host__ __device__ inline uint32_t tn_level(uint32_t info) { return (info >> 9)
 // Error word bits raised by kernels (read back by astro_check).
 enum : uint32_t {
     ERR_SPIN = 1u,       // reserved (no kernel waits on another block)
-    ERR_STACK = 2u,      // traversal stack overflow
+    ERR_STACK = 2u,      // stage-path traversal stack overflow
     ERR_CAPACITY = 4u,   // a layer had more nodes than its capacity
     ERR_NODE_INDEX = 8u, // a traversal met a node index outside the tree
 };

@@ -49,6 +49,7 @@ struct astro_ctx {
     uint32_t* minmax = nullptr;        // per-system ordered min/max keys, or per-tile partials
     uint32_t* cube_counter = nullptr;  // tiles done in the one-system cube kernel (kept zero)
     uint8_t* plevel = nullptr;         // [max_particles] head level per sorted position
+    uint32_t* pnext = nullptr;         // [max_tree] next pointer of the nodes starting there
     uint32_t* fused_buf = nullptr;     // per-tile counts, bases and leftovers of the fused layers
     uint32_t* err_word = nullptr;
     unsigned long long* stats = nullptr;

@@ -165,35 +165,30 @@ struct Visits {
 // than the square root and division it skips, so they stay straight-line.)
 // OFF64: node byte offsets in 64 bits (trees of 2^27 nodes or more); otherwise 32-bit offsets
 // from the uniform base, which the loads take as a scalar base plus a vector offset.
-template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64>
-__device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t* s_stack,
-                                         uint32_t tid, float4 p, uint32_t self, uint32_t sys,
-                                         float rmin0, float r2min0, Acc3& F, double& phi,
-                                         Visits& st) {
+//
+// The walk needs no stack: every node holds `next`, the node visited after its subtree in the
+// reference's order (the previous sibling, or for a first child its parent's next), so an
+// opened node goes to its last child and any other node to its next, until the root's next
+// (kNextEnd).  The visiting order, and with it the order of the force sums, is the
+// reference's LIFO traversal (children pushed 0..7, the last popped first).
+template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64>
+__device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t self,
+                                         uint32_t sys, float rmin0, float r2min0, Acc3& F,
+                                         double& phi, Visits& st) {
     constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
     const char* tn_base = reinterpret_cast<const char*>(a.tnodes);
-    // The loop leaves only at its head.  Neither node indices nor the stack depth are checked
-    // per visit: the host traverses only a tree built on the current layer table, and the
-    // kernel none whose build overflowed a layer, so every index is the root or a child run of
-    // a written node, one layer below its parent.
-    uint32_t top_first = (uint32_t)(a.root_off + sys), top_cnt = 1, sp = 0;
-    while (true) {
-        if (top_cnt == 0) {
-            if (sp == 0) break;
-            sp--;
-            const uint32_t e = s_stack[sp * NT + tid];
-            top_first = e >> 3;
-            top_cnt = (e & 7u) + 1u;
-        }
-        top_cnt--;
-        const uint32_t n = top_first + top_cnt;
+    // Node indices are not checked per visit: the host traverses only a tree built on the
+    // current layer table, and the kernel none whose build overflowed a layer, so every index
+    // is the root, a written node's last child, or a written node's next.
+    uint32_t n = (uint32_t)(a.root_off + sys);
+    do {
         // byte offset from a uniform base: 32-bit while the tree has fewer than 2^27 nodes
         const float4* np = reinterpret_cast<const float4*>(
             tn_base + (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
         const float4 cm = np[0];
-        const uint2 li = reinterpret_cast<const uint2*>(np)[2];
+        const uint4 li = reinterpret_cast<const uint4*>(np)[1];  // last, info, next, start
         // quadrupole mode: the moments are loaded with the node, not after the opening test
         float4 qm0, qm1;
         if (QUAD) {
@@ -276,22 +271,13 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, uint32_t* s_stack,
             }
             if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         }
-        const bool push = open & (top_cnt > 0u);
-        // at most one entry per layer 1 .. layers-2 is on the stack (a run is pushed only when
-        // one of its nodes opens, and children lie one layer down), so depth never overflows
-        if (push) s_stack[sp * NT + tid] = (top_first << 3) | (top_cnt - 1u);
-        sp += push ? 1u : 0u;
-        top_first = open ? li.x : top_first;
-        top_cnt = open ? tn_nchild(li.y) : top_cnt;
-    }
+        n = open ? li.x : li.z;
+    } while (n != kNextEnd);
 }
 
-// Targets in Morton order (sorted position j), one per thread; the stack of (first child,
-// remaining count) runs lives in LDS as [depth][NT].
+// Targets in Morton order (sorted position j), one per thread.
 template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool OFF64>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
-    extern __shared__ uint32_t s_stack[];  // [depth][NT]
-
     // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
     // x-th contiguous eighth of the target tiles (best L2 locality); xcd_chunk K deals runs of
     // K consecutive tiles to the XCDs in turn (better balance when work varies along the
@@ -343,12 +329,12 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         Acc3 F;
         double phi = 0.0;
         if (fdiv)
-            traverse<NT, KAHAN, MODE, QUAD, FAST, true, OFF64>(a, s_stack, tid, p, self, sys,
-                                                        rmin0, QUAD ? accept_r2min(rmin0) : 0.0f,
-                                                        F, phi, st);
+            traverse<KAHAN, MODE, QUAD, FAST, true, OFF64>(a, p, self, sys, rmin0,
+                                                           QUAD ? accept_r2min(rmin0) : 0.0f,
+                                                           F, phi, st);
         else
-            traverse<NT, KAHAN, MODE, QUAD, FAST, false, OFF64>(a, s_stack, tid, p, self, sys,
-                                                         rmin0, 0.0f, F, phi, st);
+            traverse<KAHAN, MODE, QUAD, FAST, false, OFF64>(a, p, self, sys, rmin0, 0.0f, F,
+                                                            phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
@@ -455,10 +441,7 @@ uint64_t check_sqrt(unsigned long long* d_mismatch, hipStream_t s) {
 template <int NT, bool KAHAN, bool QUAD, bool FAST>
 static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t blocks = (a.ns + NT - 1) / NT;
-    // an entry is pushed only when a node of layers 1 .. layers-2 opens with siblings left (the
-    // root has none, leaves never open): at most layers - 2 = bits - 1 entries
-    const uint32_t depth = a.p.bits_per_axis > 2u ? a.p.bits_per_axis - 1u : 1u;
-    const size_t lds = (size_t)depth * NT * sizeof(uint32_t);
+    const size_t lds = 0;
     if (a.node_total >= (1ull << 27)) {  // byte offsets past 4 GiB
         if (mode == 0)
             hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, true>), dim3(blocks),

@@ -407,9 +407,10 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     t.y = cm[1];
     t.z = cm[2];
     t.m = cm[3];
-    t.first = ASTRO_INVALID_CHILD;
+    t.last = ASTRO_INVALID_CHILD;
     t.info = tn_info(0, 0, h);
-    t.key = key;
+    t.next = a.pnext[j];
+    t.start = j;
     a.tnodes[n] = t;
     a.levels[n] = (uint8_t)h;
     float S[6];
@@ -593,9 +594,10 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
     t.y = cm[1];
     t.z = cm[2];
     t.m = cm[3];
-    t.first = (uint32_t)(cb + c);
+    t.last = (uint32_t)(cb + c + kc - 1);
     t.info = tn_info(l, kc, level);
-    t.key = u0.key;
+    t.next = a.pnext[u0.start];
+    t.start = u0.start;
     a.tnodes[nd] = t;
     a.levels[nd] = (uint8_t)level;
     if (QUAD) {
@@ -769,9 +771,12 @@ __device__ inline uint32_t lds_block_scan(uint32_t v, uint32_t* tmp, uint32_t* t
     return r;
 }
 
+// Head level of every sorted position (kNoHead inside a leaf's run) and, per tile, the
+// nodes of each layer below the top that start in it (layers 0 .. nl-1, nl = bits).
 __global__ __launch_bounds__(kFusedThreads) void k_fused_heads(TreeArgs a, uint8_t* plevel,
                                                                uint32_t* fcnt, uint32_t* ffix,
-                                                               uint32_t lf, uint32_t tiles) {
+                                                               uint32_t lf, uint32_t nl,
+                                                               uint32_t tiles) {
     __shared__ uint32_t s_hist[kMaxLayers];
     if (threadIdx.x < kMaxLayers) s_hist[threadIdx.x] = 0;
     if (threadIdx.x < lf) ffix[threadIdx.x * tiles + blockIdx.x] = 0xFFFFFFFFu;  // no leftover
@@ -786,19 +791,21 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_heads(TreeArgs a, uint8
             if (k != kp) h = head_level(k, kp, a.bits);
         }
         plevel[j] = (uint8_t)h;
-        if (h != kNoHead) atomicAdd(&s_hist[h < lf - 1 ? h : lf - 1], 1u);
+        if (h != kNoHead) atomicAdd(&s_hist[h < nl - 1 ? h : nl - 1], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < lf) {
+    if (threadIdx.x < nl) {
         uint32_t c = 0;
-        for (uint32_t b = threadIdx.x; b < lf; b++) c += s_hist[b];
+        for (uint32_t b = threadIdx.x; b < nl; b++) c += s_hist[b];
         fcnt[threadIdx.x * tiles + blockIdx.x] = c;
     }
 }
 
-// Block l: exclusive scan of layer l's per-tile node counts, and the layer's size.
+// Block l: exclusive scan of layer l's per-tile node counts; for the fused layers (l < lf)
+// also the layer's size.
 __global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const uint32_t* fcnt,
-                                                              uint32_t* fbase, uint32_t tiles) {
+                                                              uint32_t* fbase, uint32_t lf,
+                                                              uint32_t tiles) {
     __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
     const uint32_t l = blockIdx.x;
     const uint32_t* in = fcnt + (uint64_t)l * tiles;
@@ -812,10 +819,44 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const 
         if (t < tiles) out[t] = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && l < lf) {
         a.counts[l] = carry;
         if (carry > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
     }
+}
+
+// The next pointer of the nodes starting at each head position s (TNode): with H = h(s) below
+// the top, the node of layer H holding s - 1, whose index is off[H] plus the heads at level
+// >= H before s, minus one; kNextEnd where s starts a system.  The heads at level >= H before
+// s in the tile are counted with one ballot per layer: a wave's own count from the ballot of
+// its lane's level, the waves before it from their per-layer totals in LDS.
+__global__ __launch_bounds__(kFusedThreads) void k_next_fill(TreeArgs a,
+                                                             const uint8_t* __restrict__ plevel,
+                                                             const uint32_t* __restrict__ fbase,
+                                                             uint32_t nl, uint32_t tiles) {
+    constexpr uint32_t NW = kFusedThreads / 64;
+    __shared__ uint32_t s_cnt[NW][kMaxLayers];
+    const uint32_t t = blockIdx.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t j = t * kFusedTile + threadIdx.x;
+    const uint32_t h = j < a.nt ? plevel[j] : kNoHead;
+    const int hv = h == kNoHead ? -1 : (int)h;
+    uint64_t mine = 0;
+    for (uint32_t L = 0; L < nl; L++) {
+        const uint64_t m = __ballot(hv >= (int)L);
+        mine = (uint32_t)hv == L ? m : mine;
+        if (lane == 0) s_cnt[wave][L] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (hv < 0 || j >= a.nt) return;
+    if (h >= nl) {  // a system's first position (h = bits)
+        a.pnext[j] = kNextEnd;
+        return;
+    }
+    uint32_t before = lanes_below(mine);
+    for (uint32_t w = 0; w < wave; w++) before += s_cnt[w][h];
+    // the first position of s's system, a head at every layer, precedes s: the index of the
+    // node holding s - 1 is at least off[h]
+    a.pnext[j] = (uint32_t)(a.off[h] + fbase[(uint64_t)h * tiles + t] + before - 1u);
 }
 
 __device__ inline uint32_t block_min_u32(uint32_t v, uint32_t* tmp) {
@@ -854,7 +895,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
     __shared__ float4 s_val[kFusedRange];     // the current layer's nodes (centre, mass)
     __shared__ uint16_t s_rank[kFusedRange];  // per position: nodes of the layer before it
     __shared__ uint16_t s_pos[kFusedRange];   // per node of the layer: its start position
-    __shared__ uint64_t s_key[kFusedRange];   // per position: its key (a node's key is its first)
     __shared__ uint8_t s_h[kFusedRange];      // per position: its head level
     __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
@@ -888,9 +928,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
     const uint32_t q0 = 2 * tid, q1 = 2 * tid + 1;  // two adjacent positions per thread
     const uint32_t h0 = q0 < np ? plevel[a0 + q0] : kNoHead;
     const uint32_t h1 = q1 < np ? plevel[a0 + q1] : kNoHead;
-    // keys and levels of the range in LDS: the node stores of every layer read them
-    if (q0 < np) s_key[q0] = a.sorted_keys[a0 + q0];
-    if (q1 < np) s_key[q1] = a.sorted_keys[a0 + q1];
+    // levels of the range in LDS: the node stores of every layer read them
     s_h[q0] = (uint8_t)h0;
     s_h[q1] = (uint8_t)h1;
     uint32_t tot_prev = 0;
@@ -958,9 +996,10 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             tn.y = cm[i][1];
             tn.z = cm[i][2];
             tn.m = cm[i][3];
-            tn.first = l == 0 ? ASTRO_INVALID_CHILD : (uint32_t)(a.off[l - 1] + base_prev + c0[i]);
+            tn.last = l == 0 ? ASTRO_INVALID_CHILD : (uint32_t)(a.off[l - 1] + base_prev + c1[i] - 1u);
             tn.info = tn_info(l, l == 0 ? 0u : c1[i] - c0[i], h);
-            tn.key = s_key[st[i]];
+            tn.next = a.pnext[a0 + st[i]];
+            tn.start = a0 + st[i];
             a.tnodes[nd] = tn;
             a.levels[nd] = (uint8_t)h;
         }
@@ -1003,16 +1042,36 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const u
     }
 }
 
-void launch_fused_layers(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
+// fbuf: per-tile counts [nl][tiles], their scan [nl][tiles], the fused layers' leftovers
+// [lf][tiles] (nl = bits <= kMaxLayers).
+static uint32_t* fused_fcnt(uint32_t* fbuf, uint32_t) { return fbuf; }
+static uint32_t* fused_fbase(uint32_t* fbuf, uint32_t tiles) {
+    return fbuf + (uint64_t)kMaxLayers * tiles;
+}
+static uint32_t* fused_ffix(uint32_t* fbuf, uint32_t tiles) {
+    return fbuf + 2ull * kMaxLayers * tiles;
+}
+
+void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
+                       hipStream_t s) {
+    const uint32_t tiles = fused_tiles(a.nt);
+    if (!a.nt) return;
+    const uint32_t nl = a.bits;
+    hipLaunchKernelGGL(k_fused_heads, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
+                       fused_fcnt(fbuf, tiles), fused_ffix(fbuf, tiles), lf, nl, tiles);
+    hipLaunchKernelGGL(k_fused_scan, dim3(nl), dim3(kFusedThreads), 0, s, a,
+                       fused_fcnt(fbuf, tiles), fused_fbase(fbuf, tiles), lf, tiles);
+    hipLaunchKernelGGL(k_next_fill, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
+                       fused_fbase(fbuf, tiles), nl, tiles);
+}
+
+void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, uint32_t* fbuf,
                          hipStream_t s) {
     const uint32_t tiles = fused_tiles(a.nt);
     if (!a.nt || lf == 0) return;
-    uint32_t* fcnt = fbuf;
-    uint32_t* fbase = fbuf + (uint64_t)lf * tiles;
-    uint32_t* ffix = fbuf + 2ull * lf * tiles;
-    hipLaunchKernelGGL(k_fused_heads, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt, ffix,
-                       lf, tiles);
-    hipLaunchKernelGGL(k_fused_scan, dim3(lf), dim3(kFusedThreads), 0, s, a, fcnt, fbase, tiles);
+    uint32_t* fcnt = fused_fcnt(fbuf, tiles);
+    uint32_t* fbase = fused_fbase(fbuf, tiles);
+    uint32_t* ffix = fused_ffix(fbuf, tiles);
     hipLaunchKernelGGL(k_fused_build, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt,
                        fbase, ffix, lf, tiles);
     hipLaunchKernelGGL(k_fused_fix, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase, ffix, lf,
@@ -1092,8 +1151,9 @@ __device__ __forceinline__ void emit_tile(const TreeArgs& a, const uint32_t* cou
         if (i < cnt) {  // the fields of write_onode, from registers
             const uint64_t node = a.off[l] + i;
             const TNode t = a.tnodes[node];
-            const uint64_t code = t.key >> (3 * l);
+            const uint64_t code = a.sorted_keys[t.start] >> (3 * l);
             const uint32_t nch = l ? tn_nchild(t.info) : 0u;
+            const uint32_t first = t.last - nch + 1u;  // (unused for leaves)
             f[0] = make_float4(__uint_as_float((uint32_t)code),
                                __uint_as_float((uint32_t)(code >> 32)), 0.0f, 0.0f);
             f[1] = make_float4(t.x, t.y, t.z, 0.0f);
@@ -1105,14 +1165,14 @@ __device__ __forceinline__ void emit_tile(const TreeArgs& a, const uint32_t* cou
                 f[3] = a.qmom[2 * node];
                 f[4] = make_float4(q1.x, q1.y, 0.0f, 0.0f);
             }
-            f[5] = make_float4(0u < nch ? __uint_as_float(t.first) : inv,
-                               1u < nch ? __uint_as_float(t.first + 1) : inv,
-                               2u < nch ? __uint_as_float(t.first + 2) : inv,
-                               3u < nch ? __uint_as_float(t.first + 3) : inv);
-            f[6] = make_float4(4u < nch ? __uint_as_float(t.first + 4) : inv,
-                               5u < nch ? __uint_as_float(t.first + 5) : inv,
-                               6u < nch ? __uint_as_float(t.first + 6) : inv,
-                               7u < nch ? __uint_as_float(t.first + 7) : inv);
+            f[5] = make_float4(0u < nch ? __uint_as_float(first) : inv,
+                               1u < nch ? __uint_as_float(first + 1) : inv,
+                               2u < nch ? __uint_as_float(first + 2) : inv,
+                               3u < nch ? __uint_as_float(first + 3) : inv);
+            f[6] = make_float4(4u < nch ? __uint_as_float(first + 4) : inv,
+                               5u < nch ? __uint_as_float(first + 5) : inv,
+                               6u < nch ? __uint_as_float(first + 6) : inv,
+                               7u < nch ? __uint_as_float(first + 7) : inv);
             f[7] = make_float4(__uint_as_float(l), 0.0f, 0.0f, 0.0f);
         } else {  // stale: an empty node
             for (int k = 0; k < 8; k++) f[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);

@@ -70,6 +70,7 @@ struct TreeArgs {
     OctreeNode* onodes;         // may be null (no emission)
     uint32_t* counts;           // [layers] valid nodes per layer
     uint32_t* leaf_of_sorted;   // [nt]
+    uint32_t* pnext;            // [nt] per head position s: the next of every node starting at s
     uint32_t* err;
     uint64_t off[kMaxLayers];
     uint64_t cap[kMaxLayers];
@@ -101,12 +102,20 @@ uint32_t layer_tiles(uint64_t candidates);
 uint64_t layer_super_words(uint64_t candidates);
 void launch_layer_count(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, uint32_t* super,
                         hipStream_t s);
-// Layers 0 .. lf-1 (lf <= kFusedMaxLayers) in four launches (monopole, no reference quirk):
-// plevel holds nt bytes and fbuf fused_buf_words(nt) words, both scratch.
+// Before any node is written: the head level of every sorted position (plevel), the node
+// count of every layer below the top per 1,024-position tile and its scan over the tiles
+// (fbuf), and from them a.pnext, the next pointer of the nodes starting at each position.
+// Layers 0 .. lf-1 (lf <= kFusedMaxLayers; 0: none) are then built in two launches
+// (monopole, no reference quirk) by launch_fused_layers.  plevel holds nt bytes and fbuf
+// fused_buf_words(nt) words, both scratch.
 constexpr uint32_t kFusedMaxLayers = 16;
 uint32_t fused_tiles(uint32_t nt);
-inline uint64_t fused_buf_words(uint32_t nt) { return 3ull * kFusedMaxLayers * fused_tiles(nt); }
-void launch_fused_layers(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
+inline uint64_t fused_buf_words(uint32_t nt) {
+    return (2ull * kMaxLayers + kFusedMaxLayers) * fused_tiles(nt);
+}
+void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
+                       hipStream_t s);
+void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, uint32_t* fbuf,
                          hipStream_t s);
 // The top layers with at most kTopCandidates candidates each in one single-block launch:
 // layers_top_first(a) is the first of them (a.layers if none; layer 0 never), and the layer
