@@ -3,7 +3,7 @@
 The compute path is libastro.so (hand-written HIP kernels behind a C-ABI, include/astro/astro.h);
 this package holds its ctypes binding and the reference-layout struct mirrors.
 """
-from .structs import (AstroParams, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_NODE,  # noqa
+from .structs import (AstroParams, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_LINK,  # noqa
                       VELOCITY_RADIUS, INVALID_CHILD, INVALID_MORTON_CODE, SNAPSHOT_VERSION,
                       SnapshotHeader, make_particles)
 from .capi import (AstroError, Context, Server, load, device_count, params_compat,  # noqa

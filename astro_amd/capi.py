@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .structs import (AstroParams, SnapshotHeader, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_NODE,
+from .structs import (AstroParams, SnapshotHeader, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_LINK,
                       VELOCITY_RADIUS, ptr)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -20,7 +20,8 @@ OK, ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_STATE, ERR_DEVICE, ERR_NOMEM = 0, -1, -2
 BUF = dict(POSITION_MASS=0, VELOCITY_RADIUS=1, TYPE=2, FORCE=3, MORTON_CODES=4,
            UNSORTED_CODES=5, UNSORTED_INDICES=6, SORTED_CODES=7, SORTED_INDICES=8,
            UNIQUE_INDICES=9, PARENT_COUNT=10, CHILD_COUNT=11, FLAGS=12, SCAN=13,
-           OCTREE_NODES=14, TREE_NODES=15, LAYER_COUNTS=16, LEAF_OF_SORTED=17, CUBES=18)
+           OCTREE_NODES=14, TREE_POS=15, LAYER_COUNTS=16, LEAF_OF_SORTED=17, CUBES=18,
+           TREE_LINKS=19)
 STAGES = ["bbox_morton", "sort", "build", "force", "update"]
 
 
@@ -116,6 +117,7 @@ SIGNATURES = [
     ("astro_check_division", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
     ("astro_check_open_test", _I, [_P, _U64, _U64, ctypes.POINTER(_U64)]),
     ("astro_check_sqrt", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    ("astro_check_reciprocal", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     ("astro_snapshot_write", _I, [_P, ctypes.c_char_p, ctypes.c_double]),
     ("astro_snapshot_read", _I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
     ("astro_snapshot_info", _I, [ctypes.c_char_p, _P]),
@@ -486,6 +488,12 @@ class Context:
         self._chk(self.lib.astro_check_sqrt(self.h, ctypes.byref(n), ctypes.byref(m)))
         return n.value, m.value
 
+    def check_reciprocal(self):
+        """(floats checked, mismatches) of the exhaustive reciprocal self-check."""
+        n, m = ctypes.c_uint64(), ctypes.c_uint64()
+        self._chk(self.lib.astro_check_reciprocal(self.h, ctypes.byref(n), ctypes.byref(m)))
+        return n.value, m.value
+
     def check_open_test(self, n, seed=1):
         """Disagreements of the squared opening test with the square-root form on n cases."""
         m = ctypes.c_uint64()
@@ -503,10 +511,17 @@ class Context:
         total = int(off[-1] + cap[-1]) if len(cap) else 0
         return self.read("OCTREE_NODES", OCTREE_NODE, total)
 
-    def tree_nodes(self):
+    def tree_pos(self):
+        """Centre and mass (float32 x, y, z, m) of every compact node slot."""
         cap, off = self.layer_table()
         total = int(off[-1] + cap[-1]) if len(cap) else 0
-        return self.read("TREE_NODES", TREE_NODE, total)
+        return self.read("TREE_POS", np.float32, 4 * total).reshape(total, 4)
+
+    def tree_links(self):
+        """Links (last child, info, next, start) of every compact node slot."""
+        cap, off = self.layer_table()
+        total = int(off[-1] + cap[-1]) if len(cap) else 0
+        return self.read("TREE_LINKS", TREE_LINK, total)
 
     def layer_counts(self):
         cap, _ = self.layer_table()

@@ -128,7 +128,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->sorted_codes, c->tmp_keys_a, c->tmp_keys_b, c->unsorted_idx,
                     c->sorted_idx, c->tmp_idx_a, c->tmp_idx_b, c->unique_idx, c->flags,
                     c->scan, c->ebuf, c->fbuf, c->parent_count, c->child_count, c->dev_consts,
-                    c->tnodes, c->onodes, c->layer_counts, c->layer_counts_prev,
+                    c->tpos, c->tlink, c->onodes, c->layer_counts, c->layer_counts_prev,
                     c->leaf_of_sorted, c->pnext, c->cubes, c->minmax, c->cube_counter, c->plevel, c->fused_buf,
                     c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
@@ -214,7 +214,8 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->parent_count, 1);
     e = e ? e : dalloc(c->device_bytes, &c->child_count, 1);
     e = e ? e : dalloc(c->device_bytes, &c->dev_consts, 4);
-    e = e ? e : dalloc(c->device_bytes, &c->tnodes, cap_bound);
+    e = e ? e : dalloc(c->device_bytes, &c->tpos, cap_bound);
+    e = e ? e : dalloc(c->device_bytes, &c->tlink, cap_bound);
     e = e ? e : dalloc(c->device_bytes, &c->levels, cap_bound);
     e = e ? e : dalloc(c->device_bytes, &c->onodes, cap_bound);
     e = e ? e : dalloc(c->device_bytes, &c->layer_counts, kMaxLayers);
@@ -495,7 +496,8 @@ extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t*
         case ASTRO_BUF_FLAGS: p = c->flags; b = NP * 4; break;
         case ASTRO_BUF_SCAN: p = c->scan; b = NP * 4; break;
         case ASTRO_BUF_OCTREE_NODES: p = c->onodes; b = c->node_capacity * sizeof(OctreeNode); break;
-        case ASTRO_BUF_TREE_NODES: p = c->tnodes; b = c->node_capacity * sizeof(TNode); break;
+        case ASTRO_BUF_TREE_POS: p = c->tpos; b = c->node_capacity * 16ull; break;
+        case ASTRO_BUF_TREE_LINKS: p = c->tlink; b = c->node_capacity * sizeof(TLink); break;
         case ASTRO_BUF_LAYER_COUNTS: p = c->layer_counts; b = kMaxLayers * 4; break;
         case ASTRO_BUF_LEAF_OF_SORTED: p = c->leaf_of_sorted; b = NT * 4; break;
         case ASTRO_BUF_CUBES: p = c->cubes; b = c->cfg.max_systems * 16ull; break;
@@ -770,7 +772,8 @@ extern "C" int astro_build_internal_nodes(astro_ctx* c, uint32_t count) {
 
 static ForceArgs force_args(astro_ctx* c) {
     ForceArgs a{};
-    a.tnodes = c->tnodes;
+    a.tpos = c->tpos;
+    a.tlink = c->tlink;
     a.qmom = c->qmom;
     a.onodes = c->onodes;
     a.pos = c->pos;
@@ -849,7 +852,8 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.sys_of = c->sys_of;
     a.sorted_keys = c->sorted_codes;
     a.sorted_idx = c->sorted_idx;
-    a.tnodes = c->tnodes;
+    a.tpos = c->tpos;
+    a.tlink = c->tlink;
     a.levels = c->levels;
     a.qmom = c->qmom;
     a.quad = c->params.multipole == 2 ? 1u : 0u;
@@ -885,7 +889,7 @@ static int enqueue_sort(astro_ctx* c) {
                       c->nt, c->dev_consts + 1, plan, c->nsys > 1 ? c->sys_of : nullptr);
 }
 
-// The OctreeNode array of the tree in tnodes (and the emptied stale slots).
+// The OctreeNode array of the compact tree (and the emptied stale slots).
 static void enqueue_emit(astro_ctx* c, hipStream_t es) {
     TreeArgs a = tree_args(c);
     launch_emit_onodes(a, c->layer_counts_prev, 4 * c->num_cus, es);
@@ -1291,6 +1295,24 @@ extern "C" int astro_check_sqrt(astro_ctx* c, uint64_t* checked, uint64_t* misma
     launch_fill_u64(reinterpret_cast<uint64_t*>(d), 0ull, 1, c->stream);
     const uint64_t n = check_sqrt(d, c->stream);
     if (!n) return fail(c, ASTRO_ERR_HIP, "square-root check launch");
+    unsigned long long h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *checked = n;
+    *mismatches = h;
+    return ASTRO_OK;
+}
+
+// Checks the refined reciprocal behind the traversal's fast division on every float in
+// [2^-45, 2^46); *checked receives how many, *mismatches the disagreements with the double
+// reciprocal rounded to float (0 expected).  Synchronous.
+extern "C" int astro_check_reciprocal(astro_ctx* c, uint64_t* checked, uint64_t* mismatches) {
+    if (!c || !checked || !mismatches) return ASTRO_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(c->stats);
+    launch_fill_u64(reinterpret_cast<uint64_t*>(d), 0ull, 1, c->stream);
+    const uint64_t n = check_reciprocal(d, c->stream);
+    if (!n) return fail(c, ASTRO_ERR_HIP, "reciprocal check launch");
     unsigned long long h = 0;
     HIPCHK(c, hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -8,9 +8,12 @@
 
 namespace astro {
 
-// Compact traversal node, 32 B (one half of a 64-B line).  Children of a node are a
-// contiguous run of the layer below (the build writes each layer in ascending key order),
-// so a (last, count) pair replaces the reference's children[8] (Astro/bridge.h:62).
+// Compact traversal node, 32 B, stored as two parallel 16-B arrays: the centre and mass
+// (float4 {x, y, z, m}) and the links (TLink).  Children of a node are a contiguous run of
+// the layer below (the build writes each layer in ascending key order), so a (last, count)
+// pair replaces the reference's children[8] (Astro/bridge.h:62).  Split, a traversal step
+// reads 16 B from each array; the 64 lanes of a wave then touch fewer distinct cache lines
+// per load than with 32-B nodes, and the two loads share one index-to-offset shift.
 //
 // `next` threads the tree in the reference's visiting order (BarnesHut.metal:22-101 pushes
 // children 0..7 on a LIFO stack, so the last child is visited first): it is the node visited
@@ -19,14 +22,18 @@ namespace astro {
 // last child, any other to its next.  Every node that starts at sorted position s has the
 // same next: the node of layer h(s) (the highest layer at which s starts a node) that holds
 // position s - 1, or kNextEnd where s starts a system.
-struct alignas(16) TNode {
-    float x, y, z, m;   // centre of mass, total mass
+struct alignas(16) TLink {
     uint32_t last;      // absolute index of the last child (leaves: ASTRO_INVALID_CHILD)
     uint32_t info;      // layer | nchild << 5 | head level << 9
     uint32_t next;      // the node after this subtree in visiting order (kNextEnd: none)
     uint32_t start;     // sorted position of the node's first particle (key = sorted key >> 3 layer)
 };
-static_assert(sizeof(TNode) == 32, "TNode must be 32 B");
+static_assert(sizeof(TLink) == 16, "TLink must be 16 B");
+// One node in registers (the build's working form).
+struct TNode {
+    float x, y, z, m;   // centre of mass, total mass
+    uint32_t last, info, next, start;
+};
 constexpr uint32_t kNextEnd = 0xFFFFFFFFu;
 
 __host__ __device__ inline uint32_t tn_info(uint32_t layer, uint32_t nchild, uint32_t h) {

@@ -12,13 +12,13 @@
 constexpr uint32_t kDiagChunks = 64;
 
 struct astro_ctx {
-    using TNode = astro::TNode;
+    using TLink = astro::TLink;
     astro_config cfg{};
     astro_params params{};
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t num_cus = 1;              // compute units of the device (emission grid)
-    bool tree_built = false;           // tnodes hold a tree built on the current layer table
+    bool tree_built = false;           // the compact nodes hold a tree built on the current layer table
     std::string err;
 
     uint32_t ns = 0, nt = 0, nsys = 1;
@@ -39,8 +39,9 @@ struct astro_ctx {
     uint32_t *parent_count = nullptr, *child_count = nullptr;
     uint32_t* dev_consts = nullptr;    // [0] = ns, [1] = nt
     // tree
-    TNode* tnodes = nullptr;
-    uint8_t* levels = nullptr;         // head level per node (parallel to tnodes)
+    float4* tpos = nullptr;            // centre and mass per node
+    TLink* tlink = nullptr;            // links per node (parallel to tpos)
+    uint8_t* levels = nullptr;         // head level per node (parallel to tpos)
     OctreeNode* onodes = nullptr;
     float4* qmom = nullptr;            // 2 per node: second moments, allocated on first use
     uint64_t node_capacity = 0;

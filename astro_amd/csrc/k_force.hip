@@ -137,18 +137,25 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
     return v;
 }
 
-// q = RN(n / d) for d and n in [2^-45, 2^45] (n may also be 0): the compiler's IEEE division
-// sequence (refined reciprocal, two remainder corrections) without its operand scaling and
-// special-case fix-up, which change nothing on that domain.
-__device__ inline float div_rn_mid(float n, float d) {
-    float r = __builtin_amdgcn_rcpf(d);
+// RN(1 / d) for d in [2^-45, 2^46): the hardware reciprocal refined by one Newton step.  The
+// raw estimate differs from the correctly rounded reciprocal on about 11% of that range, the
+// refined one on none (k_check_reciprocal checks every float of it).
+__device__ inline float rcp_rn_mid(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
     const float e = __builtin_fmaf(-d, r, 1.0f);
-    r = __builtin_fmaf(e, r, r);
-    float q = n * r;
-    float t = __builtin_fmaf(-d, q, n);
-    q = __builtin_fmaf(t, r, q);
-    t = __builtin_fmaf(-d, q, n);
-    return __builtin_fmaf(t, r, q);
+    return __builtin_fmaf(e, r, r);
+}
+
+// q = RN(n / d) for d and n in [2^-45, 2^45] (n may also be 0), by Markstein's theorem: with
+// y = RN(1/d), q0 = RN(n y) lies within one ulp of n / d, the residual n - d q0 is exact
+// (one fma), and RN(q0 + residual y) = RN(n / d).  This replaces the compiler's IEEE division
+// sequence (operand scaling, two remainder corrections, special-case fix-up), which changes
+// nothing on that domain.
+__device__ inline float div_rn_mid(float n, float d) {
+    const float y = rcp_rn_mid(d);
+    const float q0 = n * y;
+    const float t = __builtin_fmaf(-d, q0, n);
+    return __builtin_fmaf(t, y, q0);
 }
 
 struct Visits {
@@ -163,32 +170,34 @@ struct Visits {
 // r2min0 4^l, so that the square root, the division and the quadrupole term are computed in a
 // branch that waves without an interacting lane skip.  (For monopoles the branch costs more
 // than the square root and division it skips, so they stay straight-line.)
-// OFF64: node byte offsets in 64 bits (trees of 2^27 nodes or more); otherwise 32-bit offsets
+// OFF64: node byte offsets in 64 bits (trees of 2^28 nodes or more); otherwise 32-bit offsets
 // from the uniform base, which the loads take as a scalar base plus a vector offset.
+// G1: G = 1, so the products G M are the node masses.
 //
 // The walk needs no stack: every node holds `next`, the node visited after its subtree in the
 // reference's order (the previous sibling, or for a first child its parent's next), so an
 // opened node goes to its last child and any other node to its next, until the root's next
 // (kNextEnd).  The visiting order, and with it the order of the force sums, is the
 // reference's LIFO traversal (children pushed 0..7, the last popped first).
-template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64>
+template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64, bool G1>
 __device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t self,
                                          uint32_t sys, float rmin0, float r2min0, Acc3& F,
                                          double& phi, Visits& st) {
     constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
-    const char* tn_base = reinterpret_cast<const char*>(a.tnodes);
+    const char* pos_base = reinterpret_cast<const char*>(a.tpos);
+    const char* link_base = reinterpret_cast<const char*>(a.tlink);
     // Node indices are not checked per visit: the host traverses only a tree built on the
     // current layer table, and the kernel none whose build overflowed a layer, so every index
     // is the root, a written node's last child, or a written node's next.
     uint32_t n = (uint32_t)(a.root_off + sys);
-    do {
-        // byte offset from a uniform base: 32-bit while the tree has fewer than 2^27 nodes
-        const float4* np = reinterpret_cast<const float4*>(
-            tn_base + (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
-        const float4 cm = np[0];
-        const uint4 li = reinterpret_cast<const uint4*>(np)[1];  // last, info, next, start
+    while (true) {
+        // one byte offset (16 B per node) into both arrays, from uniform bases: 32-bit while
+        // the tree has fewer than 2^28 nodes
+        const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
+        const float4 cm = *reinterpret_cast<const float4*>(pos_base + o);
+        const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
         // quadrupole mode: the moments are loaded with the node, not after the opening test
         float4 qm0, qm1;
         if (QUAD) {
@@ -199,9 +208,9 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t 
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
         if (!FAST) r2 = fmaxf(r2, r2f);
-        const uint32_t layer = tn_layer(li.y);
+        const uint32_t layer = tn_layer(li.info);
         const bool leaf = layer == 0u;
-        const float gm = G * cm.w;
+        const float gm = G1 ? cm.w : G * cm.w;  // G1: G = 1, so RN(G M) = M
         bool open;
         if constexpr (R2) {
             // RN(sqrt(r2)) >= rmin0 2^l  <=>  r2 >= r2min0 4^l (accept.hpp): the square root
@@ -271,12 +280,13 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t 
             }
             if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         }
-        n = open ? li.x : li.z;
-    } while (n != kNextEnd);
+        n = open ? li.last : li.next;
+        if (n == kNextEnd) break;
+    }
 }
 
 // Targets in Morton order (sorted position j), one per thread.
-template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool OFF64>
+template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool OFF64, bool G1>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
     // x-th contiguous eighth of the target tiles (best L2 locality); xcd_chunk K deals runs of
@@ -329,12 +339,12 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         Acc3 F;
         double phi = 0.0;
         if (fdiv)
-            traverse<KAHAN, MODE, QUAD, FAST, true, OFF64>(a, p, self, sys, rmin0,
+            traverse<KAHAN, MODE, QUAD, FAST, true, OFF64, G1>(a, p, self, sys, rmin0,
                                                            QUAD ? accept_r2min(rmin0) : 0.0f,
                                                            F, phi, st);
         else
-            traverse<KAHAN, MODE, QUAD, FAST, false, OFF64>(a, p, self, sys, rmin0, 0.0f, F,
-                                                            phi, st);
+            traverse<KAHAN, MODE, QUAD, FAST, false, OFF64, G1>(a, p, self, sys, rmin0, 0.0f,
+                                                                F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
@@ -438,30 +448,54 @@ uint64_t check_sqrt(unsigned long long* d_mismatch, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? (uint64_t)(hi - lo) : 0u;
 }
 
+// Exhaustive self-check of rcp_rn_mid, the premise of div_rn_mid: every float d in
+// [2^-45, 2^46) against the double reciprocal rounded to float (correctly rounded: the double
+// division is, and 53 >= 2 * 24 + 2 bits make the second rounding innocuous).
+__global__ void k_check_reciprocal(uint32_t lo, uint32_t hi, unsigned long long* mismatch) {
+    unsigned long long bad = 0;
+    for (uint64_t u = lo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < hi;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const float d = __uint_as_float((uint32_t)u);
+        const float ref = (float)(1.0 / (double)d);
+        bad += __float_as_uint(rcp_rn_mid(d)) != __float_as_uint(ref);
+    }
+    bad = wave_sum_u64(bad);
+    if ((threadIdx.x & 63u) == 0u && bad) atomicAdd(mismatch, bad);
+}
+uint64_t check_reciprocal(unsigned long long* d_mismatch, hipStream_t s) {
+    const uint32_t lo = (127u - 45u) << 23, hi = (127u + 46u) << 23;  // [2^-45, 2^46)
+    hipLaunchKernelGGL(k_check_reciprocal, dim3(8192), dim3(256), 0, s, lo, hi, d_mismatch);
+    return hipGetLastError() == hipSuccess ? (uint64_t)(hi - lo) : 0u;
+}
+
+template <int NT, bool KAHAN, bool QUAD, bool FAST, bool OFF64, bool G1>
+static void launch_force_modes(const ForceArgs& a, int mode, hipStream_t s) {
+    const uint32_t blocks = (a.ns + NT - 1) / NT;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, OFF64, G1>), dim3(blocks),
+                           dim3(NT), 0, s, a);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, OFF64, G1>), dim3(blocks),
+                           dim3(NT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, OFF64, G1>), dim3(blocks),
+                           dim3(NT), 0, s, a);
+}
+
 template <int NT, bool KAHAN, bool QUAD, bool FAST>
 static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
-    const uint32_t blocks = (a.ns + NT - 1) / NT;
-    const size_t lds = 0;
-    if (a.node_total >= (1ull << 27)) {  // byte offsets past 4 GiB
-        if (mode == 0)
-            hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, true>), dim3(blocks),
-                               dim3(NT), lds, s, a);
-        else if (mode == 1)
-            hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, true>), dim3(blocks),
-                               dim3(NT), lds, s, a);
-        else
-            hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, true>), dim3(blocks),
-                               dim3(NT), lds, s, a);
-    } else if (mode == 0) {
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, false>), dim3(blocks),
-                           dim3(NT), lds, s, a);
-    } else if (mode == 1) {
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, false>), dim3(blocks),
-                           dim3(NT), lds, s, a);
-    } else {
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, false>), dim3(blocks),
-                           dim3(NT), lds, s, a);
+    const bool off64 = a.node_total >= (1ull << 28);  // byte offsets past 4 GiB
+    // G = 1 (the physics profile): the products G M are the masses themselves; compiled for
+    // the plain fast paths only
+    if constexpr (FAST && !KAHAN) {
+        if (a.p.G == 1.0f) {
+            off64 ? launch_force_modes<NT, KAHAN, QUAD, FAST, true, true>(a, mode, s)
+                  : launch_force_modes<NT, KAHAN, QUAD, FAST, false, true>(a, mode, s);
+            return;
+        }
     }
+    off64 ? launch_force_modes<NT, KAHAN, QUAD, FAST, true, false>(a, mode, s)
+          : launch_force_modes<NT, KAHAN, QUAD, FAST, false, false>(a, mode, s);
 }
 
 template <int NT>

@@ -266,6 +266,17 @@ void launch_morton(const float4* pos, uint32_t n, const uint32_t* sys_of, const 
 
 // --------------------------------------------------------------------------- aggregation
 // Octree.metal:84-94 / 157-172: mass += m; com = fma(x, m, com); COM = com / mass.
+// A node's two halves (centre and mass; links) from / to their parallel arrays.
+__device__ inline TNode load_node(const TreeArgs& a, uint64_t i) {
+    const float4 p = a.tpos[i];
+    const TLink k = a.tlink[i];
+    return TNode{p.x, p.y, p.z, p.w, k.last, k.info, k.next, k.start};
+}
+__device__ inline void store_node(const TreeArgs& a, uint64_t i, const TNode& t) {
+    a.tpos[i] = make_float4(t.x, t.y, t.z, t.m);
+    a.tlink[i] = TLink{t.last, t.info, t.next, t.start};
+}
+
 struct Agg32 {
     float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     __device__ void add(float x, float y, float z, float w) {
@@ -411,7 +422,7 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     t.info = tn_info(0, 0, h);
     t.next = a.pnext[j];
     t.start = j;
-    a.tnodes[n] = t;
+    store_node(a, n, t);
     a.levels[n] = (uint8_t)h;
     float S[6];
     if (QUAD) {
@@ -568,22 +579,22 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
     uint32_t kc = 1;
 #pragma unroll
     for (uint32_t j = 1; j < 8; j++) kc += (kc == j && c + j < n && lv[j] < l) ? 1u : 0u;
-    const TNode* ch = a.tnodes + cb + c;
+    const float4* ch = a.tpos + cb + c;
     float cm[4];
     if (fp64) {
         Agg64 g;
         g.add(u0.x, u0.y, u0.z, u0.m);
         for (uint32_t j = 1; j < kc; j++) {
-            const TNode v = ch[j];
-            g.add(v.x, v.y, v.z, v.m);
+            const float4 v = ch[j];
+            g.add(v.x, v.y, v.z, v.w);
         }
         g.fin(cm);
     } else {
         Agg32 g;
         g.add(u0.x, u0.y, u0.z, u0.m);
         for (uint32_t j = 1; j < kc; j++) {
-            const TNode v = ch[j];
-            g.add(v.x, v.y, v.z, v.m);
+            const float4 v = ch[j];
+            g.add(v.x, v.y, v.z, v.w);
         }
         g.fin(cm);
     }
@@ -598,7 +609,7 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
     t.info = tn_info(l, kc, level);
     t.next = a.pnext[u0.start];
     t.start = u0.start;
-    a.tnodes[nd] = t;
+    store_node(a, nd, t);
     a.levels[nd] = (uint8_t)level;
     if (QUAD) {
         float S[6];
@@ -608,8 +619,8 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
             for (uint32_t j = 0; j < kc; j++) {
                 const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
                 const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                const TNode v = ch[j];
-                q.add(v.x, v.y, v.z, v.m, cm, Sc);
+                const float4 v = ch[j];
+                q.add(v.x, v.y, v.z, v.w, cm, Sc);
             }
             q.fin(S);
         } else {
@@ -617,8 +628,8 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
             for (uint32_t j = 0; j < kc; j++) {
                 const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
                 const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                const TNode v = ch[j];
-                q.add(v.x, v.y, v.z, v.m, cm, Sc);
+                const float4 v = ch[j];
+                q.add(v.x, v.y, v.z, v.w, cm, Sc);
             }
             q.fin(S);
         }
@@ -656,7 +667,7 @@ __global__ __launch_bounds__(256) void k_layer_emit_int(TreeArgs a, uint32_t l,
     for (uint32_t i = 0; i < NL; i++) lv[i] = c0 + i < capc ? a.levels[cb + c0 + i] : 0u;
     TNode u0[NI];
 #pragma unroll
-    for (uint32_t i = 0; i < NI; i++) u0[i] = a.tnodes[cb + (c0 + i < capc ? c0 + i : 0u)];
+    for (uint32_t i = 0; i < NI; i++) u0[i] = load_node(a, cb + (c0 + i < capc ? c0 + i : 0u));
     if (base >= n) return;
     uint32_t hmask = 0, cnt = 0;
 #pragma unroll
@@ -705,7 +716,7 @@ __global__ __launch_bounds__(kTopThreads) void k_layers_top(TreeArgs a, uint32_t
             uint32_t lv[8];
 #pragma unroll
             for (uint32_t j = 0; j < 8; j++) lv[j] = c + j < capc ? a.levels[cb + c + j] : 0u;
-            const TNode u0 = a.tnodes[cb + (c < capc ? c : 0u)];
+            const TNode u0 = load_node(a, cb + (c < capc ? c : 0u));
             const bool head = c < n && lv[0] >= l;
             uint32_t total;
             const uint32_t p = carry + block_exclusive_scan<kTopThreads>(head ? 1u : 0u, s_tmp,
@@ -1000,7 +1011,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             tn.info = tn_info(l, l == 0 ? 0u : c1[i] - c0[i], h);
             tn.next = a.pnext[a0 + st[i]];
             tn.start = a0 + st[i];
-            a.tnodes[nd] = tn;
+            store_node(a, nd, tn);
             a.levels[nd] = (uint8_t)h;
         }
         // per position, the nodes of this layer before it (the next layer's child ranks)
@@ -1034,7 +1045,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const u
                 const uint32_t capc = (uint32_t)a.cap[l - 1];
                 uint32_t lv[8];
                 for (uint32_t j = 0; j < 8; j++) lv[j] = v + j < capc ? a.levels[cb + v + j] : 0u;
-                const TNode u0 = a.tnodes[cb + v];
+                const TNode u0 = load_node(a, cb + v);
                 emit_parent<false>(a, l, v, p, lv, u0, a.counts[l - 1]);
             }
         }
@@ -1150,7 +1161,7 @@ __device__ __forceinline__ void emit_tile(const TreeArgs& a, const uint32_t* cou
         float4 f[8];
         if (i < cnt) {  // the fields of write_onode, from registers
             const uint64_t node = a.off[l] + i;
-            const TNode t = a.tnodes[node];
+            const TNode t = load_node(a, node);
             const uint64_t code = a.sorted_keys[t.start] >> (3 * l);
             const uint32_t nch = l ? tn_nchild(t.info) : 0u;
             const uint32_t first = t.last - nch + 1u;  // (unused for leaves)

@@ -63,8 +63,9 @@ struct TreeArgs {
     const uint32_t* sys_of;     // system of each particle / sorted position
     const uint64_t* sorted_keys;
     const uint32_t* sorted_idx;
-    TNode* tnodes;
-    uint8_t* levels;            // head level of every node (parallel to tnodes)
+    float4* tpos;               // centre and mass of every node
+    TLink* tlink;               // links of every node (parallel to tpos)
+    uint8_t* levels;            // head level of every node (parallel to tpos)
     float4* qmom;               // second moments, 2 float4 per node (quadrupole mode)
     uint32_t quad;              // 1: compute second moments
     OctreeNode* onodes;         // may be null (no emission)
@@ -153,7 +154,8 @@ void launch_clear_onodes(OctreeNode* nodes, uint64_t count, hipStream_t s);
 
 // ---- k_force.hip --------------------------------------------------------------------------
 struct ForceArgs {
-    const TNode* tnodes;
+    const float4* tpos;         // node centres and masses
+    const TLink* tlink;         // node links
     const float4* qmom;         // second moments per node (quadrupole mode)
     const OctreeNode* onodes;
     const float4* pos;
@@ -188,6 +190,9 @@ int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, h
 // Exhaustive self-check of the correctly rounded square root; returns the floats checked (0 on
 // a launch error).
 uint64_t check_sqrt(unsigned long long* d_mismatch, hipStream_t s);
+// Exhaustive self-check of the refined reciprocal behind the fast division; returns the floats
+// checked (0 on a launch error).
+uint64_t check_reciprocal(unsigned long long* d_mismatch, hipStream_t s);
 // Fills acc and fast from p.
 void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);

@@ -21,12 +21,12 @@ OCTREE_NODE = np.dtype({"names": ["mortonCode", "centerOfMass", "totalMass", "em
                         "formats": ["<u8", ("<f4", 3), "<f4", ("<f4", 4), ("<f4", 3),
                                     ("<u4", 8), "<u4"],
                         "offsets": [0, 16, 32, 48, 64, 80, 112], "itemsize": 128})
-# compact traversal node of libastro (32 B): centre of mass, mass, last child, info word
-# (layer | nchild << 5 | head level << 9), the node after the subtree in visiting order
-# (0xFFFFFFFF: none), sorted position of the node's first particle
-TREE_NODE = np.dtype({"names": ["com", "mass", "last", "info", "next", "start"],
-                      "formats": [("<f4", 3), "<f4", "<u4", "<u4", "<u4", "<u4"],
-                      "offsets": [0, 12, 16, 20, 24, 28], "itemsize": 32})
+# links of a compact traversal node of libastro (16 B, parallel to the float4 centres and
+# masses): last child, info word (layer | nchild << 5 | head level << 9), the node after the
+# subtree in visiting order (0xFFFFFFFF: none), sorted position of the node's first particle
+TREE_LINK = np.dtype({"names": ["last", "info", "next", "start"],
+                      "formats": ["<u4", "<u4", "<u4", "<u4"],
+                      "offsets": [0, 4, 8, 12], "itemsize": 16})
 
 INVALID_MORTON_CODE = 0xFFFFFFFFFFFFFFFF
 INVALID_CHILD = 0xFFFFFFFF

@@ -60,10 +60,12 @@ enum {
     ASTRO_BUF_FLAGS = 12,          /* uint32                                              */
     ASTRO_BUF_SCAN = 13,           /* uint32                                              */
     ASTRO_BUF_OCTREE_NODES = 14,   /* OctreeNode[sum of layer capacities]                 */
-    ASTRO_BUF_TREE_NODES = 15,     /* compact traversal nodes, 32 B each                  */
+    ASTRO_BUF_TREE_POS = 15,       /* float4 {x,y,z,m} per compact traversal node         */
     ASTRO_BUF_LAYER_COUNTS = 16,   /* uint32[layers]: valid nodes per layer, last build   */
     ASTRO_BUF_LEAF_OF_SORTED = 17, /* uint32 leaf index of each sorted source             */
     ASTRO_BUF_CUBES = 18,          /* float4 {lo.x, lo.y, lo.z, side} per system          */
+    ASTRO_BUF_TREE_LINKS = 19,     /* uint32 {last child, info, next, start} per compact  */
+                                   /* node (info = layer | nchild << 5 | head level << 9) */
     ASTRO_BUF_COUNT
 };
 
@@ -209,6 +211,12 @@ int astro_check_open_test(astro_ctx* ctx, uint64_t n, uint64_t seed, uint64_t* m
  * receives the count, *mismatches the disagreements with the double-precision square root
  * rounded to float (0 expected).  No reference counterpart.  Synchronous, about 0.1 s. */
 int astro_check_sqrt(astro_ctx* ctx, uint64_t* checked, uint64_t* mismatches);
+/* Exhaustive self-check of the reciprocal behind the traversal's fast division (the hardware
+ * estimate refined by one Newton step, which the division then uses once, by Markstein's
+ * theorem) on every float in [2^-45, 2^46): *checked receives the count, *mismatches the
+ * disagreements with the double-precision reciprocal rounded to float (0 expected).  No
+ * reference counterpart.  Synchronous. */
+int astro_check_reciprocal(astro_ctx* ctx, uint64_t* checked, uint64_t* mismatches);
 /* Device memory the context allocated at creation. */
 int astro_device_bytes(const astro_ctx* ctx, uint64_t* bytes);
 

@@ -22,7 +22,7 @@ def say(*a):
 
 
 def check_links(ctx, nsys):
-    tn = ctx.tree_nodes()
+    tn = ctx.tree_links()
     cap, off = ctx.layer_table()
     off = [int(v) for v in off]
     cnt = ctx.layer_counts()

@@ -258,19 +258,20 @@ def test_radix_sort_generic(astro, small_ctx, kind):
     assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), idx[:count][o])
 
 
-def test_tree_past_2_27_nodes(astro):
-    # 10.5 M particles at B = 21: the layer table holds more than 2^27 nodes, so node byte
-    # offsets pass 4 GiB and the traversal takes its 64-bit addressing; the forces of 512
-    # random targets must match the FP64 direct sum to the tree's usual accuracy
-    say("tree of more than 2^27 nodes")
-    n = 10_500_000
+def test_tree_past_2_28_nodes(astro):
+    # 21 M particles at B = 21: the layer table holds more than 2^28 nodes, so the byte
+    # offsets of the 16-B node halves pass 4 GiB and the traversal takes its 64-bit
+    # addressing; the forces of 512 random targets must match the FP64 direct sum to the
+    # tree's usual accuracy
+    say("tree of more than 2^28 nodes")
+    n = 21_000_000
     pos, vel, col = astro.ic_plummer(n, 9)
     with astro.Context(n, bits_per_axis=21, emit_octree_nodes=False) as ctx:
         P = astro.params_physics(21)
         ctx.set_params(P)
         ctx.upload(pos, vel, col)
         cap, off = ctx.layer_table()
-        assert int(off[-1] + cap[-1]) >= 1 << 27
+        assert int(off[-1] + cap[-1]) >= 1 << 28
         ctx.build_tree()
         ctx.compute_forces()
         ctx.check()
@@ -428,6 +429,14 @@ def test_square_root_exhaustive(astro, small_ctx):
     say("exhaustive square-root self-check")
     n, bad = small_ctx.check_sqrt()
     assert n == (0x7F800000 - ((127 - 96) << 23)) and bad == 0
+
+
+def test_reciprocal_exhaustive(astro, small_ctx):
+    # the fast division's premise (Markstein): the refined hardware reciprocal is correctly
+    # rounded on every float in [2^-45, 2^46)
+    say("exhaustive reciprocal self-check")
+    n, bad = small_ctx.check_reciprocal()
+    assert n == ((127 + 46) << 23) - ((127 - 45) << 23) and bad == 0
 
 
 def test_squared_opening_test_matches_sqrt(astro, small_ctx):
