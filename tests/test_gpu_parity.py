@@ -258,6 +258,29 @@ def test_radix_sort_generic(astro, small_ctx, kind):
     assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), idx[:count][o])
 
 
+def test_radix_sort_many_tiles(astro):
+    # 3 M keys (2,930 tiles of 1,024) with many repeats (ties keep their input order), twice
+    # in a row with different live counts, against numpy's stable argsort
+    say("radix sort, many tiles")
+    n = 3_000_000
+    rng = np.random.default_rng(21)
+    with astro.Context(n, bits_per_axis=21) as ctx:
+        for rep in range(2):
+            k = (rng.integers(0, 1 << 20, n).astype(np.uint64) << np.uint64(40)) | \
+                rng.integers(0, 4, n).astype(np.uint64)
+            count = n - 1000 * rep
+            idx = rng.permutation(n).astype(np.uint32)
+            ctx.write("UNSORTED_CODES", k)
+            ctx.write("UNSORTED_INDICES", idx)
+            ctx.write("PARENT_COUNT", np.array([count], np.uint32))
+            ctx.radix_sort("UNSORTED_CODES", "UNSORTED_INDICES", "SORTED_CODES", "SORTED_INDICES",
+                           n, "PARENT_COUNT")
+            ctx.check()
+            o = np.argsort(k[:count], kind="stable")
+            assert np.array_equal(ctx.read("SORTED_CODES", np.uint64, count), k[:count][o])
+            assert np.array_equal(ctx.read("SORTED_INDICES", np.uint32, count), idx[:count][o])
+
+
 def test_tree_past_2_28_nodes(astro):
     # 21 M particles at B = 21: the layer table holds more than 2^28 nodes, so the byte
     # offsets of the 16-B node halves pass 4 GiB and the traversal takes its 64-bit
