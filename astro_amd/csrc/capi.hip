@@ -134,6 +134,9 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->beside) hipStreamDestroy(c->beside);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return ASTRO_OK;
@@ -190,6 +193,9 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         return ASTRO_ERR_HIP;
     c->num_cus = cus > 0 ? (uint32_t)cus : 1u;
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
+    if (bad(hipStreamCreateWithFlags(&c->beside, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
+    if (bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming))) return ASTRO_ERR_HIP;
+    if (bad(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming))) return ASTRO_ERR_HIP;
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(c->device_bytes, &c->pos, NP);
     e = e ? e : dalloc(c->device_bytes, &c->vel, NP);
@@ -892,7 +898,9 @@ static int enqueue_sort(astro_ctx* c) {
 // The OctreeNode array of the compact tree (and the emptied stale slots).
 static void enqueue_emit(astro_ctx* c, hipStream_t es) {
     TreeArgs a = tree_args(c);
-    launch_emit_onodes(a, c->layer_counts_prev, 4 * c->num_cus, es);
+    // beside the traversal: one block per CU, so that the traversal keeps most wave slots (four
+    // per CU measured the same)
+    launch_emit_onodes(a, c->layer_counts_prev, (es == c->stream ? 4 : 1) * c->num_cus, es);
     launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, es);
 }
 
@@ -1000,13 +1008,23 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     if (ev) record_stage(c, ev[1]);
     if ((rc = enqueue_sort(c))) return rc;
     if (ev) record_stage(c, ev[2]);
-    if ((rc = enqueue_layers(c, emit))) return rc;
+    // the OctreeNode array only reads the compact tree: written beside the traversal, which
+    // is bound by instruction issue and the vector memory pipeline, not by HBM
+    const bool beside = emit && c->emit_beside;
+    if ((rc = enqueue_layers(c, emit && !beside))) return rc;
+    if (beside) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->beside, c->ev_fork, 0));
+        enqueue_emit(c, c->beside);
+        HIPCHK(c, hipEventRecord(c->ev_join, c->beside));
+    }
     if (ev) record_stage(c, ev[3]);
     if ((rc = enqueue_forces(c, nullptr))) return rc;
     if (ev) record_stage(c, ev[4]);
     // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
     if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
     launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
+    if (beside) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (ev) record_stage(c, ev[5]);
     LAUNCHCHK(c);
     return ASTRO_OK;

@@ -17,6 +17,11 @@ struct astro_ctx {
     astro_params params{};
     int device = 0;
     hipStream_t stream = nullptr;
+    // the OctreeNode array is written on `beside` while the traversal runs on `stream` (both at
+    // the default priority), joined before the step ends
+    hipStream_t beside = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool emit_beside = true;
     uint32_t num_cus = 1;              // compute units of the device (emission grid)
     bool tree_built = false;           // the compact nodes hold a tree built on the current layer table
     std::string err;

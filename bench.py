@@ -457,7 +457,7 @@ def main():
         tj = json.load(open(tpath))
         traffic = tj.get("force_compact_bytes_per_launch")
         valu_frac = tj.get("valu_issue_frac")
-    sb = stage_bytes(n, V)
+    sb = stage_bytes(n, V, emit=False)  # the OctreeNode array is written beside the traversal
     stages = {}
     for s, ms in stage_ms.items():
         per = ms / K
