@@ -896,12 +896,14 @@ static int enqueue_sort(astro_ctx* c) {
 }
 
 // The OctreeNode array of the compact tree (and the emptied stale slots).
-static void enqueue_emit(astro_ctx* c, hipStream_t es) {
+// The layer counts become the next build's previous counts (the stale slots it empties) once
+// the array is written: in line right after it, beside the traversal after the join.
+static void enqueue_emit(astro_ctx* c, hipStream_t es, bool copy_counts = true) {
     TreeArgs a = tree_args(c);
     // beside the traversal: one block per CU, so that the traversal keeps most wave slots (four
     // per CU measured the same)
     launch_emit_onodes(a, c->layer_counts_prev, (es == c->stream ? 4 : 1) * c->num_cus, es);
-    launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, es);
+    if (copy_counts) launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, es);
 }
 
 // emit: write the OctreeNode array right after the layers (on the main stream); false leaves
@@ -1015,7 +1017,7 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     if (beside) {
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->beside, c->ev_fork, 0));
-        enqueue_emit(c, c->beside);
+        enqueue_emit(c, c->beside, false);
         HIPCHK(c, hipEventRecord(c->ev_join, c->beside));
     }
     if (ev) record_stage(c, ev[3]);
@@ -1024,7 +1026,10 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
     if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
     launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
-    if (beside) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (beside) {
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, c->stream);
+    }
     if (ev) record_stage(c, ev[5]);
     LAUNCHCHK(c);
     return ASTRO_OK;
