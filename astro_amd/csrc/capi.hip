@@ -193,9 +193,6 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         return ASTRO_ERR_HIP;
     c->num_cus = cus > 0 ? (uint32_t)cus : 1u;
     if (bad(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
-    if (bad(hipStreamCreateWithFlags(&c->beside, hipStreamNonBlocking))) return ASTRO_ERR_HIP;
-    if (bad(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming))) return ASTRO_ERR_HIP;
-    if (bad(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming))) return ASTRO_ERR_HIP;
     hipError_t e = hipSuccess;
     e = e ? e : dalloc(c->device_bytes, &c->pos, NP);
     e = e ? e : dalloc(c->device_bytes, &c->vel, NP);
@@ -1002,6 +999,17 @@ static void record_stage(astro_ctx* c, hipEvent_t e) {
 }
 
 // One step.  ev: optional stage events (ASTRO_STAGE_COUNT + 1).
+// The second stream and its events, created on the first step that writes the OctreeNode
+// array (outside any capture): contexts that never write it keep one stream, so that many
+// contexts still spread over the device's hardware queues.
+static int ensure_beside(astro_ctx* c) {
+    if (!c->emit || !c->onodes || !c->emit_beside || c->beside) return ASTRO_OK;
+    HIPCHK(c, hipStreamCreateWithFlags(&c->beside, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return ASTRO_OK;
+}
+
 static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     int rc;
     const bool emit = c->emit && c->onodes;
@@ -1012,7 +1020,7 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     if (ev) record_stage(c, ev[2]);
     // the OctreeNode array only reads the compact tree: written beside the traversal, which
     // is bound by instruction issue and the vector memory pipeline, not by HBM
-    const bool beside = emit && c->emit_beside;
+    const bool beside = emit && c->emit_beside && c->beside;
     if ((rc = enqueue_layers(c, emit && !beside))) return rc;
     if (beside) {
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
@@ -1039,6 +1047,7 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
     HIPCHK(c, hipSetDevice(c->device));
+    if (int rc = ensure_beside(c)) return rc;
     if (c->emit && c->onodes_dirty) {  // the one-off full clear, outside any graph
         launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
@@ -1074,6 +1083,7 @@ extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* 
     if (!c || !stage_ms) return ASTRO_ERR_ARG;
     if (int rc = need_particles(c)) return rc;
     HIPCHK(c, hipSetDevice(c->device));
+    if (int r = ensure_beside(c)) return r;
     const uint32_t ne = ASTRO_STAGE_COUNT + 1;
     std::vector<hipEvent_t> ev((size_t)nsteps * ne);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
