@@ -177,7 +177,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     c->node_capacity = cap_bound;
     if (cap_bound >= (1ull << 29)) {
         astro_destroy(c);
-        return ASTRO_ERR_CAPACITY;  // node indices are packed in 29 bits on the stack
+        return ASTRO_ERR_CAPACITY;  // node indices stay far below the 0xFFFFFFFF sentinels
     }
 
     auto bad = [&](hipError_t e) {

@@ -1,11 +1,10 @@
 // k_force.hip -- Barnes-Hut force traversal and integrator for gfx950.
 //
-// k_force_compact: the product traversal over 32-B compact nodes.  Targets are processed in
-// Morton order (sorted position j), so the 64 lanes of a wavefront walk nearly the same part
-// of the tree; blocks are mapped so each XCD (and its L2) takes one contiguous stretch of the
-// Morton curve.  The stack holds (first child, remaining count) runs, one per open level,
-// so its depth is bounded by the layer count less two (20 at 21 bits per axis) instead of 7
-// per level; it lives in LDS in a [depth][thread] layout.  Visit order equals the reference's
+// k_force_compact: the product traversal over the compact nodes (a 16-B centre and mass, and
+// 16-B links).  Targets are processed in Morton order (sorted position j), so the 64 lanes of a
+// wavefront walk nearly the same part of the tree; blocks are mapped so each XCD (and its L2)
+// takes one contiguous stretch of the Morton curve.  The walk needs no stack: an opened node
+// goes to its last child, any other to its `next` link.  Visit order equals the reference's
 // LIFO order (children pushed 0..7, popped 7..0), so per-target sums are bit-identical to the
 // oracle.
 //
