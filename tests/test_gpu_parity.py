@@ -137,6 +137,26 @@ def run_stage_sequence(ctx, nt, layers_cap_off):
     ctx.build_internal_nodes(nt)
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_step_writes_octree_nodes(astro, orc, graph):
+    # a step writes the OctreeNode array on a second stream beside the traversal and joins it:
+    # after two steps the array must equal the oracle's tree of the positions the second step
+    # started from, stale slots of the first step's tree emptied
+    say("octree nodes written by steps, graph", graph)
+    pos, vel, col = astro.ic_plummer(20000, 31)
+    with astro.Context(20000, bits_per_axis=21) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.set_use_graph(graph)
+        ctx.upload(pos, vel, col)
+        ctx.step(1e-3, 1)
+        ctx.check()
+        p1, _, _ = ctx.download()
+        ctx.step(1e-3, 1)
+        ctx.check()
+        t = orc.build_tree(p1, orc.params_physics(21))
+        assert ctx.octree_nodes().tobytes() == t["nodes"].tobytes()
+
+
 @pytest.mark.parametrize("case", CASES[:3], ids=[c[0] for c in CASES[:3]])
 def test_stage_path_matches_oracle(astro, orc, small_ctx, case):
     say("stage path", case[0])
