@@ -21,7 +21,7 @@ BUF = dict(POSITION_MASS=0, VELOCITY_RADIUS=1, TYPE=2, FORCE=3, MORTON_CODES=4,
            UNSORTED_CODES=5, UNSORTED_INDICES=6, SORTED_CODES=7, SORTED_INDICES=8,
            UNIQUE_INDICES=9, PARENT_COUNT=10, CHILD_COUNT=11, FLAGS=12, SCAN=13,
            OCTREE_NODES=14, TREE_POS=15, LAYER_COUNTS=16, LEAF_OF_SORTED=17, CUBES=18,
-           TREE_LINKS=19)
+           TREE_LINKS=19, TREE_MOMENTS=20)
 STAGES = ["bbox_morton", "sort", "build", "force", "update"]
 
 
@@ -516,6 +516,13 @@ class Context:
         cap, off = self.layer_table()
         total = int(off[-1] + cap[-1]) if len(cap) else 0
         return self.read("TREE_POS", np.float32, 4 * total).reshape(total, 4)
+
+    def tree_moments(self):
+        """Second-moment rows of every compact node slot (multipole = 2): uint32 view (n, 8) of
+        {Sxx, Syy, Szz, Sxy}, {Sxz, Syz, last | layer << 27, next} (links below 2^27 nodes)."""
+        cap, off = self.layer_table()
+        total = int(off[-1] + cap[-1]) if len(cap) else 0
+        return self.read("TREE_MOMENTS", np.uint32, 8 * total).reshape(total, 8)
 
     def tree_links(self):
         """Links (last child, info, next, start) of every compact node slot."""

@@ -501,6 +501,7 @@ extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t*
         case ASTRO_BUF_OCTREE_NODES: p = c->onodes; b = c->node_capacity * sizeof(OctreeNode); break;
         case ASTRO_BUF_TREE_POS: p = c->tpos; b = c->node_capacity * 16ull; break;
         case ASTRO_BUF_TREE_LINKS: p = c->tlink; b = c->node_capacity * sizeof(TLink); break;
+        case ASTRO_BUF_TREE_MOMENTS: p = c->qmom; b = c->qmom ? c->node_capacity * 32ull : 0; break;
         case ASTRO_BUF_LAYER_COUNTS: p = c->layer_counts; b = kMaxLayers * 4; break;
         case ASTRO_BUF_LEAF_OF_SORTED: p = c->leaf_of_sorted; b = NT * 4; break;
         case ASTRO_BUF_CUBES: p = c->cubes; b = c->cfg.max_systems * 16ull; break;
@@ -875,6 +876,7 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.fp64_from = fp64_from(c);
     a.quirk_last_leaf = c->params.quirk_last_leaf;
     a.segmented = c->nsys > 1;
+    a.pack = c->total < kLinkPackNodes ? 1u : 0u;
     return a;
 }
 

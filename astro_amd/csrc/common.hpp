@@ -35,6 +35,12 @@ struct TNode {
     uint32_t last, info, next, start;
 };
 constexpr uint32_t kNextEnd = 0xFFFFFFFFu;
+// Quadrupole trees of fewer than kLinkPackNodes nodes also keep a node's links in the two
+// free words of its second moment row (Sxz, Syz, last | layer << 27, next; leaves:
+// kPackedNoChild), so that a quadrupole traversal step reads three 16-B rows, not four.
+constexpr uint64_t kLinkPackNodes = 1ull << 27;
+constexpr uint32_t kPackedIndexMask = 0x07FFFFFFu;
+constexpr uint32_t kPackedNoChild = kPackedIndexMask;
 
 __host__ __device__ inline uint32_t tn_info(uint32_t layer, uint32_t nchild, uint32_t h) {
     return layer | (nchild << 5) | (h << 9);

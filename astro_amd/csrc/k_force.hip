@@ -169,7 +169,7 @@ struct Visits {
 // r2min0 4^l, so that the square root, the division and the quadrupole term are computed in a
 // branch that waves without an interacting lane skip.  (For monopoles the branch costs more
 // than the square root and division it skips, so they stay straight-line.)
-// OFF64: node byte offsets in 64 bits (trees of 2^28 nodes or more); otherwise 32-bit offsets
+// OFF64: node byte offsets in 64 bits (trees of 2^27 nodes or more); otherwise 32-bit offsets
 // from the uniform base, which the loads take as a scalar base plus a vector offset.
 // G1: G = 1, so the products G M are the node masses.
 //
@@ -193,16 +193,28 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t 
     uint32_t n = (uint32_t)(a.root_off + sys);
     while (true) {
         // one byte offset (16 B per node) into both arrays, from uniform bases: 32-bit while
-        // the tree has fewer than 2^28 nodes
+        // the tree has fewer than 2^27 nodes
         const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
         const float4 cm = *reinterpret_cast<const float4*>(pos_base + o);
-        const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
-        // quadrupole mode: the moments are loaded with the node, not after the opening test
+        // quadrupole mode: the moments are loaded with the node, not after the opening test;
+        // below 2^27 nodes their second row also holds the links (last | layer << 27, next)
+        constexpr bool QLINKS = QUAD && !OFF64;
         float4 qm0, qm1;
         if (QUAD) {
-            const float4* qp = a.qmom + 2 * (size_t)n;
+            const float4* qp = reinterpret_cast<const float4*>(
+                reinterpret_cast<const char*>(a.qmom) +
+                (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
             qm0 = qp[0];
             qm1 = qp[1];
+        }
+        TLink li;
+        if (QLINKS) {
+            const uint32_t w = __float_as_uint(qm1.z);
+            li.last = w & kPackedIndexMask;
+            li.info = w >> 27;  // the layer (tn_layer reads the low bits)
+            li.next = __float_as_uint(qm1.w);
+        } else {
+            li = *reinterpret_cast<const TLink*>(link_base + o);
         }
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
@@ -483,7 +495,8 @@ static void launch_force_modes(const ForceArgs& a, int mode, hipStream_t s) {
 
 template <int NT, bool KAHAN, bool QUAD, bool FAST>
 static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
-    const bool off64 = a.node_total >= (1ull << 28);  // byte offsets past 4 GiB
+    // 64-bit offsets from 2^27 nodes (needed from 2^28), where the moment rows stop carrying links
+    const bool off64 = a.node_total >= kLinkPackNodes;
     // G = 1 (the physics profile): the products G M are the masses themselves; compiled for
     // the plain fast paths only
     if constexpr (FAST && !KAHAN) {

@@ -374,9 +374,15 @@ struct Mom64 {
     }
 };
 
-__device__ inline void store_moments(float4* qm, uint64_t n, const float* S) {
-    qm[2 * n] = make_float4(S[0], S[1], S[2], S[3]);
-    qm[2 * n + 1] = make_float4(S[4], S[5], 0.0f, 0.0f);
+// The moments of node n, and in a packed tree (a.pack) its links in the row's free words.
+__device__ inline void store_moments(const TreeArgs& a, uint64_t n, const float* S,
+                                     uint32_t last, uint32_t layer, uint32_t next) {
+    const uint32_t w2 = a.pack ? ((last == ASTRO_INVALID_CHILD ? kPackedNoChild : last) |
+                                  (layer << 27))
+                               : 0u;
+    const uint32_t w3 = a.pack ? next : 0u;
+    a.qmom[2 * n] = make_float4(S[0], S[1], S[2], S[3]);
+    a.qmom[2 * n + 1] = make_float4(S[4], S[5], __uint_as_float(w2), __uint_as_float(w3));
 }
 
 __device__ inline uint32_t head_level(uint64_t a, uint64_t b, uint32_t bits) {
@@ -441,7 +447,7 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
             }
             q.fin(S);
         }
-        store_moments(a.qmom, n, S);
+        store_moments(a, n, S, ASTRO_INVALID_CHILD, 0u, t.next);
     }
     return h;
 }
@@ -633,7 +639,7 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
             }
             q.fin(S);
         }
-        store_moments(a.qmom, nd, S);
+        store_moments(a, nd, S, t.last, l, t.next);
     }
     return level;
 }

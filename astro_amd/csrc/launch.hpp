@@ -81,6 +81,7 @@ struct TreeArgs {
     uint32_t fp64_from;         // first layer aggregated in FP64
     uint32_t quirk_last_leaf;
     uint32_t segmented;         // 1: heads also at system starts
+    uint32_t pack;              // 1: fewer than kLinkPackNodes nodes (links in the moment rows)
 };
 
 void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s);

@@ -66,6 +66,10 @@ enum {
     ASTRO_BUF_CUBES = 18,          /* float4 {lo.x, lo.y, lo.z, side} per system          */
     ASTRO_BUF_TREE_LINKS = 19,     /* uint32 {last child, info, next, start} per compact  */
                                    /* node (info = layer | nchild << 5 | head level << 9) */
+    ASTRO_BUF_TREE_MOMENTS = 20,   /* with multipole = 2: float4 {Sxx,Syy,Szz,Sxy},       */
+                                   /* {Sxz,Syz, last | layer << 27, next} per compact     */
+                                   /* node (the links only below 2^27 nodes; leaves'      */
+                                   /* last 0x07FFFFFF)                                    */
     ASTRO_BUF_COUNT
 };
 

@@ -21,7 +21,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def check_links(ctx, nsys):
+def check_links(ctx, nsys, multipole=0):
     tn = ctx.tree_links()
     cap, off = ctx.layer_table()
     off = [int(v) for v in off]
@@ -53,6 +53,12 @@ def check_links(ctx, nsys):
     got = tn["next"].astype(np.int64)[valid]
     bad = np.nonzero(got != exp[valid])[0]
     assert bad.size == 0, f"{bad.size} wrong next links, first at node {valid[bad[0]]}"
+    if multipole == 2 and off[top] + int(cap[top]) < 1 << 27:
+        # the quadrupole traversal reads the links from the second moment row instead
+        q = ctx.tree_moments()[valid].astype(np.int64)
+        last_v = np.where(last[valid] == END, 0x07FFFFFF, last[valid])
+        assert np.array_equal(q[:, 6], last_v | ((info[valid] & 31) << 27))
+        assert np.array_equal(q[:, 7], tn["next"].astype(np.int64)[valid])
 
 
 def compat_system(n, nt):
@@ -78,7 +84,7 @@ def test_links_one_system(astro, bits, n, multipole):
         ctx.upload(pos, vel, col)
         ctx.build_tree()
         ctx.check()
-        check_links(ctx, 1)
+        check_links(ctx, 1, multipole)
 
 
 def test_links_compat_subset(astro):
@@ -120,4 +126,4 @@ def test_links_batched(astro, multipole):
         ctx.upload_systems(pos, vel, col, off)
         ctx.build_tree()
         ctx.check()
-        check_links(ctx, len(sizes))
+        check_links(ctx, len(sizes), multipole)
