@@ -938,9 +938,9 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     }
     const uint32_t l0 = layers_top_first(a);  // layers l0.. in one single-block launch
     // layers 0 .. lf-1 (where nodes hold a few particles each) in the fused launches; the
-    // reference quirk and the second moments keep the per-layer kernels
+    // reference quirk keeps the per-layer kernels
     uint32_t lf = 0;
-    if (!a.quad && !a.quirk_last_leaf && c->layers > 10)
+    if (!a.quirk_last_leaf && c->layers > 10)
         lf = std::min(std::min(l0, c->layers - 10), kFusedMaxLayers);
     if (lf < 2) lf = 0;
     // the next pointers of every layer's nodes, before any node is written

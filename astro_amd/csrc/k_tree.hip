@@ -902,7 +902,28 @@ __device__ inline void agg_nodes(const float4* v, uint32_t c0, uint32_t c1, floa
     for (uint32_t i = c0; i < c1; i++) g.add(v[i].x, v[i].y, v[i].z, v[i].w);
     g.fin(cm);
 }
+// Second moments about cm, in emit_leaf's and emit_parent's order.
+template <typename MOM>
+__device__ inline void mom_particles(const TreeArgs& a, uint32_t j0, uint32_t j1, const float* cm,
+                                     float* S) {
+    MOM q;
+    for (uint32_t i = j0; i < j1; i++) {
+        const float4 x = a.pos[a.sorted_idx[i]];
+        q.add(x.x, x.y, x.z, x.w, cm, nullptr);
+    }
+    q.fin(S);
+}
+template <typename MOM>
+__device__ inline void mom_nodes(const float4* v, const float* sm, uint32_t c0, uint32_t c1,
+                                 const float* cm, float* S) {
+    MOM q;
+    for (uint32_t i = c0; i < c1; i++) q.add(v[i].x, v[i].y, v[i].z, v[i].w, cm, sm + 6 * i);
+    q.fin(S);
+}
 
+// QUAD: the second moments too, aggregated in the per-layer kernels' order (emit_leaf,
+// emit_parent) from the layer below kept in LDS beside the centres.
+template <bool QUAD>
 __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
                                                                const uint8_t* __restrict__ plevel,
                                                                const uint32_t* __restrict__ fcnt,
@@ -910,6 +931,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
                                                                uint32_t* ffix, uint32_t lf,
                                                                uint32_t tiles) {
     __shared__ float4 s_val[kFusedRange];     // the current layer's nodes (centre, mass)
+    __shared__ float s_mom[QUAD ? 6 * kFusedRange : 1];  // and their second moments
     __shared__ uint16_t s_rank[kFusedRange];  // per position: nodes of the layer before it
     __shared__ uint16_t s_pos[kFusedRange];   // per node of the layer: its start position
     __shared__ uint8_t s_h[kFusedRange];      // per position: its head level
@@ -968,7 +990,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
         // range was cut short
         const bool end_ok = !truncated || (h_end != kNoHead && h_end >= l);
         const bool fp64 = l >= a.fp64_from;
-        float cm[2][4];
+        float cm[2][4], S[2][6];
         uint32_t st[2], c0[2], c1[2];
         bool done[2];
 #pragma unroll
@@ -984,11 +1006,19 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
                 c1[i] = a0 + nx;
                 if (fp64) agg_particles<Agg64>(a, c0[i], c1[i], cm[i]);
                 else agg_particles<Agg32>(a, c0[i], c1[i], cm[i]);
+                if (QUAD) {
+                    if (fp64) mom_particles<Mom64>(a, c0[i], c1[i], cm[i], S[i]);
+                    else mom_particles<Mom32>(a, c0[i], c1[i], cm[i], S[i]);
+                }
             } else {
                 c0[i] = s_rank[st[i]];
                 c1[i] = nx < np ? s_rank[nx] : tot_prev;
                 if (fp64) agg_nodes<Agg64>(s_val, c0[i], c1[i], cm[i]);
                 else agg_nodes<Agg32>(s_val, c0[i], c1[i], cm[i]);
+                if (QUAD) {
+                    if (fp64) mom_nodes<Mom64>(s_val, s_mom, c0[i], c1[i], cm[i], S[i]);
+                    else mom_nodes<Mom32>(s_val, s_mom, c0[i], c1[i], cm[i], S[i]);
+                }
             }
         }
         lds_barrier();  // the layer below is read; overwrite it
@@ -999,6 +1029,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             const uint32_t k = tid + i * kFusedThreads;
             if (k >= tot) continue;
             s_val[k] = make_float4(cm[i][0], cm[i][1], cm[i][2], cm[i][3]);
+            if (QUAD)
+                for (uint32_t e = 0; e < 6; e++) s_mom[6 * k + e] = S[i][e];
             if (st[i] >= own) continue;  // a halo node: owned by a later tile
             const uint32_t gi = base + k;
             if (gi >= a.cap[l]) continue;  // ERR_CAPACITY is raised by the scan
@@ -1019,6 +1051,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             tn.start = a0 + st[i];
             store_node(a, nd, tn);
             a.levels[nd] = (uint8_t)h;
+            if (QUAD) store_moments(a, nd, S[i], tn.last, l, tn.next);
         }
         // per position, the nodes of this layer before it (the next layer's child ranks)
         if (q0 < np) s_rank[q0] = (uint16_t)r0;
@@ -1032,6 +1065,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
 
 // The nodes k_fused_build left (they run past a halo), layer by layer: each tile's last node
 // of the layer, from the layer below, which is complete by then.
+template <bool QUAD>
 __global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const uint32_t* fcnt,
                                                              const uint32_t* fbase,
                                                              const uint32_t* ffix, uint32_t lf,
@@ -1044,15 +1078,15 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const u
             const uint32_t p = fbase[lt] + fcnt[lt] - 1u;
             if (p >= a.cap[l]) continue;
             if (l == 0) {
-                if (a.fp64_from == 0) emit_leaf<Agg64, false>(a, v, p);
-                else emit_leaf<Agg32, false>(a, v, p);
+                if (a.fp64_from == 0) emit_leaf<Agg64, QUAD>(a, v, p);
+                else emit_leaf<Agg32, QUAD>(a, v, p);
             } else {
                 const uint64_t cb = a.off[l - 1];
                 const uint32_t capc = (uint32_t)a.cap[l - 1];
                 uint32_t lv[8];
                 for (uint32_t j = 0; j < 8; j++) lv[j] = v + j < capc ? a.levels[cb + v + j] : 0u;
                 const TNode u0 = load_node(a, cb + v);
-                emit_parent<false>(a, l, v, p, lv, u0, a.counts[l - 1]);
+                emit_parent<QUAD>(a, l, v, p, lv, u0, a.counts[l - 1]);
             }
         }
         __syncthreads();  // layer l is complete before layer l + 1 reads it
@@ -1089,10 +1123,17 @@ void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, 
     uint32_t* fcnt = fused_fcnt(fbuf, tiles);
     uint32_t* fbase = fused_fbase(fbuf, tiles);
     uint32_t* ffix = fused_ffix(fbuf, tiles);
-    hipLaunchKernelGGL(k_fused_build, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel, fcnt,
-                       fbase, ffix, lf, tiles);
-    hipLaunchKernelGGL(k_fused_fix, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase, ffix, lf,
-                       tiles);
+    if (a.quad) {
+        hipLaunchKernelGGL(k_fused_build<true>, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
+                           fcnt, fbase, ffix, lf, tiles);
+        hipLaunchKernelGGL(k_fused_fix<true>, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase,
+                           ffix, lf, tiles);
+    } else {
+        hipLaunchKernelGGL(k_fused_build<false>, dim3(tiles), dim3(kFusedThreads), 0, s, a,
+                           plevel, fcnt, fbase, ffix, lf, tiles);
+        hipLaunchKernelGGL(k_fused_fix<false>, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase,
+                           ffix, lf, tiles);
+    }
 }
 
 uint32_t layer_tiles(uint64_t candidates) {
