@@ -528,3 +528,48 @@ def test_lighting_per_system(orc):
     col["color"][2] = [0, 2, 0, 1]
     out = orc.lighting_pass(pos, col, 1, sys_off=[0, 2, 4])["color"]
     assert np.allclose(out[1], [1, 1, 1, 1]) and np.allclose(out[3], [0, 0.5, 0, 1])
+
+
+@pytest.mark.parametrize("n,seed", [(3000, 5), (800, 9)])
+def test_next_link_formula_matches_lifo_walk(orc, n, seed):
+    # The traversal's `next` link: the node the reference's LIFO walk (BarnesHut.metal:22-101,
+    # children pushed 0..7) visits after a node's subtree.  The library computes it per head
+    # position s as off[h(s)] + #{j < s : h(j) >= h(s)} - 1 (none where h(s) is the top layer),
+    # h = the highest Morton digit in which a key differs from its predecessor's.  Both forms on
+    # the oracle's tree of a seeded system; a slip in h (digit index), the tie rule or the
+    # "- 1" fails here.
+    rng = np.random.default_rng(seed)
+    pos = np.zeros(n, POSITION_MASS)
+    pos["position"] = rng.normal(0.0, 1.0, (n, 3)).astype(np.float32)
+    pos["position"][: n // 10] = pos["position"][0]  # a run of equal keys
+    pos["mass"] = 1.0
+    P = orc.params_physics(21)
+    t = orc.build_tree(pos, P)
+    nodes, off, cnt = t["nodes"], t["off"], t["count"]
+    top = len(off) - 1
+    ch = nodes["children"].astype(np.int64)
+    nxt = np.full(len(nodes), -2, np.int64)
+    nxt[off[top]] = -1
+    for l in range(top, 0, -1):
+        for p in range(off[l], off[l] + cnt[l]):
+            kids = [c for c in ch[p] if c != 0xFFFFFFFF]
+            for i, c in enumerate(kids):
+                nxt[c] = kids[i - 1] if i > 0 else nxt[p]
+    sk = t["sorted_codes"].astype(np.uint64)
+    h = np.full(len(sk), -1, np.int64)
+    h[0] = top
+    for j in range(1, len(sk)):
+        x = int(sk[j]) ^ int(sk[j - 1])
+        if x:
+            h[j] = min((x.bit_length() - 1) // 3, top)
+    f = np.full(len(sk), -3, np.int64)
+    for H in range(top):
+        before = np.cumsum(h >= H) - (h >= H)
+        sel = h == H
+        f[sel] = off[H] + before[sel] - 1
+    f[h == top] = -1
+    for l in range(top + 1):
+        starts = np.nonzero(h >= l)[0]
+        assert len(starts) == cnt[l]
+        idx = np.arange(off[l], off[l] + cnt[l])
+        assert np.array_equal(nxt[idx], f[starts]), l
