@@ -494,6 +494,9 @@ def main():
                      "valu_issue_what": "SQ_INSTS_VALU x 4 cycles per wave64 instruction over "
                                         "1024 SIMDs at 2.4 GHz, against the kernel's duration "
                                         "(rocprofv3, profiles/)",
+                     "limiter": "vector memory pipeline: texture address unit busy 86% of the "
+                                "kernel's cycles with the walk's divergent node loads, beside "
+                                "VALU issue (profiles/force_memory_pipeline_two_disks_500k.md)",
                      "interactions_per_step": inter, "flop_per_interaction": FLOP_PER_INTERACTION,
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
         "stages": stages,
