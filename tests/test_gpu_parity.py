@@ -538,15 +538,16 @@ EDGE_CASES = ["one", "two", "all_equal", "pairs_equal", "massless_half", "b1_tre
               "equal_run_3000", "dense_cluster"]
 
 
+@pytest.mark.parametrize("multipole", [0, 2])
 @pytest.mark.parametrize("kind", EDGE_CASES)
-def test_edge_cases_bitexact(astro, orc, small_ctx, kind):
+def test_edge_cases_bitexact(astro, orc, small_ctx, kind, multipole):
     # degenerate inputs through the fused build, traversal and 3 steps, against the oracle:
     # one and two particles, every particle on one point (one leaf), coincident pairs
     # (key collisions), massless particles, the smallest tree (B = 1), a 1-D line; a leaf of
     # 3,000 equal keys and a cluster of 5,000 particles inside one node of layer 11, so that
     # nodes of the lower layers run past a 1,024-position tile and its halo (the build's
     # leftover pass)
-    say("edge case", kind)
+    say("edge case", kind, "multipole", multipole)
     rng = np.random.default_rng(17)
     bits = 21
     if kind == "one":
@@ -580,15 +581,28 @@ def test_edge_cases_bitexact(astro, orc, small_ctx, kind):
     vel["velocity"] = rng.normal(0, 0.1, (n, 3)).astype(np.float32)
     col = np.zeros(n, COLOR_TYPE)
     P, Po = astro.params_physics(bits), orc.params_physics(bits)
+    P.multipole = Po.multipole = multipole
     ctx = small_ctx
     ctx.set_params(P)
     ctx.set_emit_octree_nodes(True)
     ctx.upload(pos, vel, col)
     ctx.build_tree()
-    ctx.compute_forces()
     ctx.check()
     t = orc.build_tree(pos, Po)
     assert ctx.octree_nodes().tobytes() == t["nodes"].tobytes()
+    if multipole == 2:  # the links the quadrupole traversal reads from the moment rows
+        cap, off = ctx.layer_table()
+        cnt = ctx.layer_counts()
+        valid = np.concatenate([np.arange(int(off[l]), int(off[l]) + int(cnt[l]))
+                                for l in range(len(cap))])
+        tl = ctx.tree_links()
+        q = ctx.tree_moments()[valid].astype(np.int64)
+        last = tl["last"].astype(np.int64)[valid]
+        last = np.where(last == 0xFFFFFFFF, 0x07FFFFFF, last)
+        assert np.array_equal(q[:, 6], last | ((tl["info"].astype(np.int64)[valid] & 31) << 27))
+        assert np.array_equal(q[:, 7], tl["next"].astype(np.int64)[valid])
+    ctx.compute_forces()
+    ctx.check()
     want, _ = orc.barnes_hut(t, pos, col, Po, threads=8)
     assert np.array_equal(ctx.forces()[:, :3], want[:, :3])
     ctx.upload(pos, vel, col)
