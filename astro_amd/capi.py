@@ -109,6 +109,8 @@ SIGNATURES = [
     ("astro_system_count", _I, [_P, ctypes.POINTER(_U32)]),
     ("astro_set_force_block", _I, [_P, _U32]),
     ("astro_set_force_xcd_chunk", _I, [_P, _U32]),
+    ("astro_set_force_top_cache", _I, [_P, _I]),
+    ("astro_set_wide_offsets", _I, [_P, _I]),
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32),
                             ctypes.POINTER(_F)]),
     ("astro_diagnostics", _I, [_P, _P]),
@@ -229,15 +231,29 @@ def ic_batch_sizes(n_systems, n_min, n_max, seed):
     return out
 
 
+# When the OctreeNode array is written (astro.h ASTRO_EMIT_*)
+EMIT_NEVER, EMIT_EVERY_BUILD, EMIT_ON_READ = 0, 1, 2
+
+
+def _emit_mode(m):
+    if m is True:
+        return EMIT_ON_READ
+    if m is False or m is None:
+        return EMIT_NEVER
+    if m not in (EMIT_NEVER, EMIT_EVERY_BUILD, EMIT_ON_READ):
+        raise ValueError(f"emission mode {m!r}")
+    return int(m)
+
+
 class Context:
     """One libastro context: the reference's Renderer + MetalKernelsRadixSort buffers on one
     GPU (Renderer.swift:336-391, RadixSort.swift:40-67)."""
 
     def __init__(self, max_particles, max_tree=0, bits_per_axis=21, max_systems=1,
-                 emit_octree_nodes=True, device=0):
+                 emit_octree_nodes=EMIT_ON_READ, device=0):
         self.lib = load()
         cfg = AstroConfig(max_particles, max_tree, bits_per_axis, max_systems,
-                          1 if emit_octree_nodes else 0, device)
+                          _emit_mode(emit_octree_nodes), device)
         h = ctypes.c_void_p()
         rc = self.lib.astro_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
@@ -279,8 +295,9 @@ class Context:
         self._chk(self.lib.astro_get_params(self.h, ctypes.byref(p)))
         return p
 
-    def set_emit_octree_nodes(self, on):
-        self._chk(self.lib.astro_set_emit_octree_nodes(self.h, 1 if on else 0))
+    def set_emit_octree_nodes(self, mode):
+        """EMIT_NEVER, EMIT_EVERY_BUILD or EMIT_ON_READ (True: on read, False: never)."""
+        self._chk(self.lib.astro_set_emit_octree_nodes(self.h, _emit_mode(mode)))
 
     def set_use_graph(self, on):
         self._chk(self.lib.astro_set_use_graph(self.h, 1 if on else 0))
@@ -436,6 +453,12 @@ class Context:
 
     def set_force_xcd_chunk(self, chunk):
         self._chk(self.lib.astro_set_force_xcd_chunk(self.h, chunk))
+
+    def set_force_top_cache(self, on):
+        self._chk(self.lib.astro_set_force_top_cache(self.h, 1 if on else 0))
+
+    def set_wide_offsets(self, on):
+        self._chk(self.lib.astro_set_wide_offsets(self.h, 1 if on else 0))
 
     def autotune(self, reps=3):
         """Picks the traversal block size and XCD chunk; returns (block, chunk, ms)."""

@@ -153,7 +153,8 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
         c->cfg.max_tree = c->cfg.max_particles;
     if (c->cfg.max_systems == 0) c->cfg.max_systems = 1;
     c->device = cfg->device;
-    c->emit = cfg->emit_octree_nodes != 0;
+    c->emit_mode = cfg->emit_octree_nodes <= ASTRO_EMIT_ON_READ ? cfg->emit_octree_nodes
+                                                                : ASTRO_EMIT_ON_READ;
     astro_params_compat(&c->params);
     c->params.bits_per_axis = std::min<uint32_t>(c->params.bits_per_axis, cfg->bits_per_axis);
     const uint32_t NP = c->cfg.max_particles, NT = c->cfg.max_tree;
@@ -289,9 +290,13 @@ extern "C" int astro_get_params(const astro_ctx* c, astro_params* p) {
     return ASTRO_OK;
 }
 
-extern "C" int astro_set_emit_octree_nodes(astro_ctx* c, int on) {
+extern "C" int astro_set_emit_octree_nodes(astro_ctx* c, int mode) {
     if (!c) return ASTRO_ERR_ARG;
-    c->emit = on != 0;
+    if (mode < ASTRO_EMIT_NEVER || mode > ASTRO_EMIT_ON_READ)
+        return fail(c, ASTRO_ERR_ARG, "emission mode must be 0, 1 or 2");
+    // a pending array of the current tree is written before the mode changes
+    if (int rc = materialize_onodes(c)) return rc;
+    c->emit_mode = (uint32_t)mode;
     invalidate_graph(c);
     return ASTRO_OK;
 }
@@ -498,7 +503,11 @@ extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t*
         case ASTRO_BUF_CHILD_COUNT: p = c->child_count; b = 4; break;
         case ASTRO_BUF_FLAGS: p = c->flags; b = NP * 4; break;
         case ASTRO_BUF_SCAN: p = c->scan; b = NP * 4; break;
-        case ASTRO_BUF_OCTREE_NODES: p = c->onodes; b = c->node_capacity * sizeof(OctreeNode); break;
+        case ASTRO_BUF_OCTREE_NODES:
+            if (int rc = materialize_onodes(c)) return rc;
+            p = c->onodes;
+            b = c->node_capacity * sizeof(OctreeNode);
+            break;
         case ASTRO_BUF_TREE_POS: p = c->tpos; b = c->node_capacity * 16ull; break;
         case ASTRO_BUF_TREE_LINKS: p = c->tlink; b = c->node_capacity * sizeof(TLink); break;
         case ASTRO_BUF_TREE_MOMENTS: p = c->qmom; b = c->qmom ? c->node_capacity * 32ull : 0; break;
@@ -699,7 +708,10 @@ extern "C" int astro_clear_octree_node_buffer(astro_ctx* c, OctreeNode* d_nodes,
                                               uint32_t count) {
     if (!c || !d_nodes) return ASTRO_ERR_ARG;
     launch_clear_onodes(d_nodes, count, c->stream);
-    if (d_nodes == c->onodes) c->onodes_dirty = true;
+    if (d_nodes == c->onodes) {
+        c->onodes_dirty = true;
+        c->onodes_pending = false;
+    }
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -733,6 +745,7 @@ extern "C" int astro_aggregate_leaf_nodes(astro_ctx* c, uint32_t count) {
                           c->onodes + c->off[0], c->unsorted_codes, c->unsorted_idx, count,
                           c->params.quirk_last_leaf, count, fp64_from(c) == 0, c->stream);
     c->onodes_dirty = true;
+    c->onodes_pending = false;
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -748,6 +761,7 @@ extern "C" int astro_aggregate_internal_nodes(astro_ctx* c, uint32_t layer, uint
                            c->unsorted_idx, c->child_count, node_count, layer >= fp64_from(c),
                            c->stream);
     c->onodes_dirty = true;
+    c->onodes_pending = false;
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -790,6 +804,11 @@ static ForceArgs force_args(astro_ctx* c) {
     a.force = c->force;
     a.err = c->err_word;
     a.stats = nullptr;
+    a.counts = c->layer_counts;
+    for (uint32_t l = 0; l < c->layers; l++) a.off[l] = c->off[l];
+    a.layers = c->layers;
+    a.top = c->force_top ? 1u : 0u;
+    a.wide = c->force_wide ? 1u : 0u;
     a.off0 = c->off[0];
     a.root_off = c->off[c->layers - 1];
     a.node_total = c->total;
@@ -823,6 +842,7 @@ extern "C" int astro_barnes_hut(astro_ctx* c) {
     if (c->params.multipole != 0)
         return fail(c, ASTRO_ERR_STATE,
                     "the stage aggregation keeps no second moments: use astro_compute_forces");
+    if (int rc = materialize_onodes(c)) return rc;  // it reads the OctreeNode array
     launch_force_reference(force_args(c), c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -861,7 +881,7 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.levels = c->levels;
     a.qmom = c->qmom;
     a.quad = c->params.multipole == 2 ? 1u : 0u;
-    a.onodes = c->emit ? c->onodes : nullptr;
+    a.onodes = c->emit_mode != ASTRO_EMIT_NEVER ? c->onodes : nullptr;
     a.counts = c->layer_counts;
     a.leaf_of_sorted = c->leaf_of_sorted;
     a.pnext = c->pnext;
@@ -876,7 +896,7 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.fp64_from = fp64_from(c);
     a.quirk_last_leaf = c->params.quirk_last_leaf;
     a.segmented = c->nsys > 1;
-    a.pack = c->total < kLinkPackNodes ? 1u : 0u;
+    a.pack = c->total < kLinkPackNodes && !c->force_wide ? 1u : 0u;
     return a;
 }
 
@@ -919,7 +939,8 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     }
     if (nsup + ntile > c->scratch_words)
         return fail(c, ASTRO_ERR_CAPACITY, "layer scratch too small");
-    if (a.onodes && c->onodes_dirty) {
+    const bool every = c->emit_mode == ASTRO_EMIT_EVERY_BUILD;
+    if (every && c->onodes_dirty) {
         launch_clear_onodes(c->onodes, c->node_capacity, s);
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
         c->onodes_dirty = false;
@@ -953,7 +974,9 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
                           last ? nullptr : sup[l + 1], s);
     }
     launch_layers_top(a, l0, s);
-    if (a.onodes && emit) enqueue_emit(c, s);
+    if (every && emit) enqueue_emit(c, s);
+    // on read: the array of this tree is written when a caller reads it
+    c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
     c->tree_built = true;
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -974,6 +997,23 @@ static int enqueue_forces(astro_ctx* c, unsigned long long* stats) {
     ForceArgs a = force_args(c);
     a.stats = stats;
     launch_force_compact(a, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// On read: writes the OctreeNode array of the last built tree if a build left it pending
+// (stream-ordered before whatever the caller enqueues next).
+int materialize_onodes(astro_ctx* c) {
+    if (!c->onodes_pending) return ASTRO_OK;
+    c->onodes_pending = false;
+    if (!c->tree_built) return ASTRO_OK;  // the layer table changed since: nothing to write
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->onodes_dirty) {
+        launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
+        launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
+        c->onodes_dirty = false;
+    }
+    enqueue_emit(c, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -1005,7 +1045,7 @@ static void record_stage(astro_ctx* c, hipEvent_t e) {
 // array (outside any capture): contexts that never write it keep one stream, so that many
 // contexts still spread over the device's hardware queues.
 static int ensure_beside(astro_ctx* c) {
-    if (!c->emit || !c->onodes || !c->emit_beside || c->beside) return ASTRO_OK;
+    if (c->emit_mode != ASTRO_EMIT_EVERY_BUILD || !c->emit_beside || c->beside) return ASTRO_OK;
     HIPCHK(c, hipStreamCreateWithFlags(&c->beside, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
@@ -1014,7 +1054,7 @@ static int ensure_beside(astro_ctx* c) {
 
 static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     int rc;
-    const bool emit = c->emit && c->onodes;
+    const bool emit = c->emit_mode == ASTRO_EMIT_EVERY_BUILD;
     if (ev) record_stage(c, ev[0]);
     if ((rc = enqueue_bbox_morton(c))) return rc;
     if (ev) record_stage(c, ev[1]);
@@ -1050,7 +1090,8 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
     if (int rc = need_particles(c)) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     if (int rc = ensure_beside(c)) return rc;
-    if (c->emit && c->onodes_dirty) {  // the one-off full clear, outside any graph
+    if (c->emit_mode == ASTRO_EMIT_EVERY_BUILD && c->onodes_dirty) {
+        // the one-off full clear, outside any graph
         launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
         c->onodes_dirty = false;
@@ -1078,6 +1119,8 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
         c->graph_dt = dt;
     }
     for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    // each replay built a tree whose OctreeNode array is left for the next read
+    if (nsteps) c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
     return ASTRO_OK;
 }
 
@@ -1090,7 +1133,7 @@ extern "C" int astro_step_timed(astro_ctx* c, float dt, uint32_t nsteps, float* 
     std::vector<hipEvent_t> ev((size_t)nsteps * ne);
     for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
     int rc = ASTRO_OK;
-    if (c->emit && c->onodes_dirty) {
+    if (c->emit_mode == ASTRO_EMIT_EVERY_BUILD && c->onodes_dirty) {
         launch_clear_onodes(c->onodes, c->node_capacity, c->stream);
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, c->stream);
         c->onodes_dirty = false;
@@ -1153,8 +1196,24 @@ extern "C" int astro_device_bytes(const astro_ctx* c, uint64_t* bytes) {
 
 extern "C" int astro_set_force_block(astro_ctx* c, uint32_t block) {
     if (!c) return ASTRO_ERR_ARG;
-    if (!force_block_supported(block)) return fail(c, ASTRO_ERR_ARG, "block must be 64/128/256/512");
+    if (!force_block_supported(block))
+        return fail(c, ASTRO_ERR_ARG, "block must be 64/128/256/512/1024");
     c->force_block = block;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_force_top_cache(astro_ctx* c, int on) {
+    if (!c) return ASTRO_ERR_ARG;
+    c->force_top = on != 0;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_wide_offsets(astro_ctx* c, int on) {
+    if (!c) return ASTRO_ERR_ARG;
+    c->force_wide = on != 0;
+    c->tree_built = false;  // the moment rows of a packed tree no longer match
     invalidate_graph(c);
     return ASTRO_OK;
 }
@@ -1175,7 +1234,7 @@ extern "C" int astro_autotune(astro_ctx* c, uint32_t reps, uint32_t* chosen_bloc
     if (int rc = need_particles(c)) return rc;
     if (reps == 0) reps = 3;
     if (int rc = enqueue_build(c)) return rc;
-    const uint32_t blocks[4] = {64, 128, 256, 512};
+    const uint32_t blocks[5] = {64, 128, 256, 512, 1024};
     const uint32_t chunks[4] = {0, 8, 32, 128};
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));

@@ -64,10 +64,13 @@ struct astro_ctx {
     double* diag_partial = nullptr;    // [max_systems * kDiagChunks * 9]
     uint32_t force_block = 256;        // traversal block size (autotuned)
     uint32_t force_chunk = 0;          // traversal tile runs per XCD (autotuned; 0: eighths)
+    bool force_top = true;             // traversal reads the top layers from LDS (blocks >= 512)
+    bool force_wide = false;           // 64-bit node offsets and unpacked moment rows on any tree
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation
     uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
     uint64_t scratch_words = 0;
-    bool emit = true;
+    uint32_t emit_mode = ASTRO_EMIT_ON_READ;  // when the OctreeNode array is written
+    bool onodes_pending = false;       // on read: the last build's array is not written yet
     bool onodes_dirty = true;          // stage calls wrote onodes outside the fused build
     // bounds of the uploaded masses, for the traversal's division domain: every mass finite
     // and >= 0, the smallest positive one, the largest system total (conservative)
@@ -115,3 +118,5 @@ void astro_mass_bounds(astro_ctx* c, const float* m, uint32_t n, size_t stride, 
 int astro_refresh_layers(astro_ctx* c);
 int astro_set_counts(astro_ctx* c, uint32_t ns, uint32_t nt);
 int astro_enqueue_step(astro_ctx* c, float dt);
+// On read: writes the pending OctreeNode array of the last build (no-op otherwise).
+int materialize_onodes(astro_ctx* c);

@@ -161,6 +161,44 @@ struct Visits {
     unsigned long long visits = 0, leafs = 0, monos = 0, opens = 0;
 };
 
+// The top of the tree in LDS.  A block copies the nodes of the top layers lt .. L-1 (the most
+// that fit, chosen from the layer counts) into LDS before its walks start, and rewrites their
+// links: a top node is named kTopBit | layer << 24 | slot, with its LDS slot in the low 12
+// bits, so that its layer is known before its data arrives.  Every target walks the top
+// layers, and its 64 lanes read scattered nodes there: from LDS those reads leave the texture
+// address unit, which bounds the walk.
+//   last' of a top node of layer l > lt names its last child (a top node of layer l - 1); a
+//     node of layer lt keeps its child's global index (bit 31 clear); a top leaf (lt = 0) is
+//     never opened.
+//   next' names a top node (the next of a node of layer l lies at its head level, >= l), or
+//     is kNextEnd.
+// A lane leaves the top only by opening a node P of layer lt, and returns to it only from
+// P's first-child chain, whose global next links all equal P's next: so a lane keeps P's
+// next' (`resume`) and follows it wherever a global next points at or above off[lt].
+constexpr uint32_t kTopBit = 0x80000000u;
+constexpr uint32_t kTopSlotMask = 0x00000FFFu;
+// slots per block: 80 KB of LDS per 1,024 threads (two blocks per CU), 24 B per node for
+// monopoles (centre and mass, last', next'), 48 B with the moment rows
+template <int NT, bool QUAD>
+struct TopCap {
+    static constexpr uint32_t kSlots = (QUAD ? 1664u : 3328u) * (uint32_t)NT / 1024u;
+};
+// LDS pointers keep their address space, so that the walk's LDS and global reads stay
+// separate ds_read and global_load instructions (a generic pointer would turn both into flat
+// loads)
+#define ASTRO_LDS __attribute__((address_space(3)))
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+typedef float lds_f2 __attribute__((ext_vector_type(2)));
+typedef uint32_t lds_u2 __attribute__((ext_vector_type(2)));
+struct TopView {
+    const ASTRO_LDS lds_f4* pos;  // [slots] centre and mass
+    const ASTRO_LDS lds_u2* link; // [slots] last', next'
+    const ASTRO_LDS lds_f4* q0;   // [slots] Sxx, Syy, Szz, Sxy (quadrupoles)
+    const ASTRO_LDS lds_f2* q1;   // [slots] Sxz, Syz
+    uint32_t lt;         // first cached layer (L: none)
+    uint32_t top_lo;     // off[lt]: global indices at or above it are top nodes (0xFFFFFFFF: none)
+};
+
 // One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
 // potential.  QUAD: accepted internal nodes add their quadrupole term.  FAST: a.fast holds (no
 // floor or cut-off can bind, square root without denormal scaling; exact for finite positions).
@@ -178,10 +216,11 @@ struct Visits {
 // opened node goes to its last child and any other node to its next, until the root's next
 // (kNextEnd).  The visiting order, and with it the order of the force sums, is the
 // reference's LIFO traversal (children pushed 0..7, the last popped first).
-template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64, bool G1>
-__device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t self,
-                                         uint32_t sys, float rmin0, float r2min0, Acc3& F,
-                                         double& phi, Visits& st) {
+template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64, bool G1,
+          bool TOPC>
+__device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, uint32_t root,
+                                         float4 p, uint32_t self, float rmin0, float r2min0,
+                                         Acc3& F, double& phi, Visits& st) {
     constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
@@ -190,36 +229,67 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t 
     // Node indices are not checked per visit: the host traverses only a tree built on the
     // current layer table, and the kernel none whose build overflowed a layer, so every index
     // is the root, a written node's last child, or a written node's next.
-    uint32_t n = (uint32_t)(a.root_off + sys);
+    uint32_t n = root;
+    uint32_t resume = kNextEnd;  // TOPC: next' of the last node of layer lt this lane opened
     while (true) {
-        // one byte offset (16 B per node) into both arrays, from uniform bases: 32-bit while
-        // the tree has fewer than 2^27 nodes
-        const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
-        const float4 cm = *reinterpret_cast<const float4*>(pos_base + o);
-        // quadrupole mode: the moments are loaded with the node, not after the opening test;
-        // below 2^27 nodes their second row also holds the links (last | layer << 27, next)
-        constexpr bool QLINKS = QUAD && !OFF64;
-        float4 qm0, qm1;
-        if (QUAD) {
-            const float4* qp = reinterpret_cast<const float4*>(
-                reinterpret_cast<const char*>(a.qmom) +
-                (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
-            qm0 = qp[0];
-            qm1 = qp[1];
+        float4 cm, qm0, qm1;
+        uint32_t last, next, layer;
+        const bool top = TOPC && (n & kTopBit);
+        lds_f4 tcm, tq0;
+        lds_f2 tq1;
+        lds_u2 tlk;
+        if (!top) {
+            // one byte offset (16 B per node) into both arrays, from uniform bases: 32-bit
+            // while the tree has fewer than 2^27 nodes
+            const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
+            cm = *reinterpret_cast<const float4*>(pos_base + o);
+            // quadrupole mode: the moments are loaded with the node, not after the opening
+            // test; below 2^27 nodes their second row also holds the links
+            // (last | layer << 27, next)
+            constexpr bool QLINKS = QUAD && !OFF64;
+            if (QUAD) {
+                const float4* qp = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const char*>(a.qmom) +
+                    (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
+                qm0 = qp[0];
+                qm1 = qp[1];
+            }
+            if (QLINKS) {
+                const uint32_t w = __float_as_uint(qm1.z);
+                last = w & kPackedIndexMask;
+                layer = w >> 27;
+                next = __float_as_uint(qm1.w);
+            } else {
+                const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
+                last = li.last;
+                layer = tn_layer(li.info);
+                next = li.next;
+            }
         }
-        TLink li;
-        if (QLINKS) {
-            const uint32_t w = __float_as_uint(qm1.z);
-            li.last = w & kPackedIndexMask;
-            li.info = w >> 27;  // the layer (tn_layer reads the low bits)
-            li.next = __float_as_uint(qm1.w);
-        } else {
-            li = *reinterpret_cast<const TLink*>(link_base + o);
+        if (top) {  // issued after the global loads above; both are waited for together
+            const uint32_t s = n & kTopSlotMask;
+            tcm = tv.pos[s];
+            tlk = tv.link[s];
+            if (QUAD) {
+                tq0 = tv.q0[s];
+                tq1 = tv.q1[s];
+            }
+        }
+        if (top) {
+            cm = make_float4(tcm.x, tcm.y, tcm.z, tcm.w);
+            if (QUAD) {
+                qm0 = make_float4(tq0.x, tq0.y, tq0.z, tq0.w);
+                qm1 = make_float4(tq1.x, tq1.y, 0.0f, 0.0f);
+            }
+            last = tlk.x;
+            next = tlk.y;
+            layer = (n >> 24) & 31u;
+        } else if (TOPC && next >= tv.top_lo) {
+            next = resume;  // back to the top: the next of the opened node of layer lt
         }
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
         if (!FAST) r2 = fmaxf(r2, r2f);
-        const uint32_t layer = tn_layer(li.info);
         const bool leaf = layer == 0u;
         const float gm = G1 ? cm.w : G * cm.w;  // G1: G = 1, so RN(G M) = M
         bool open;
@@ -291,14 +361,83 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, float4 p, uint32_t 
             }
             if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         }
-        n = open ? li.last : li.next;
+        // only top nodes have layer lt (the rest of the tree lies below it)
+        if (TOPC && open && layer == tv.lt) resume = next;
+        n = open ? last : next;
         if (n == kNextEnd) break;
     }
 }
 
-// Targets in Morton order (sorted position j), one per thread.
-template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool OFF64, bool G1>
+// The LDS top of the tree (TOPC): the cached layers lt .. L-1 are the most, from the root
+// down, whose node counts sum to at most `slots`; slots are given out layer by layer from lt
+// up (base[l]).  Thread 0 decides, every thread then copies its share of the nodes with their
+// links rewritten.  Returns the view the walks read.
+template <int NT, bool QUAD>
+__device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, float4* s_q0,
+                            float2* s_q1, uint32_t* s_base, uint32_t* s_meta) {
+    constexpr uint32_t slots = TopCap<NT, QUAD>::kSlots;
+    const uint32_t L = a.layers, tid = threadIdx.x;
+    if (tid == 0) {
+        uint32_t lt = L, K = 0;
+        for (uint32_t l = L; l-- > 0;) {
+            const uint32_t c = a.counts[l];
+            if (K + c > slots) break;
+            K += c;
+            lt = l;
+        }
+        uint32_t b = 0;
+        for (uint32_t l = lt; l < L; l++) {
+            s_base[l] = b;
+            b += a.counts[l];
+        }
+        s_meta[0] = lt;
+        s_meta[1] = K;
+    }
+    __syncthreads();
+    const uint32_t lt = s_meta[0], K = s_meta[1];
+    for (uint32_t s = tid; s < K; s += NT) {
+        uint32_t l = lt;
+        while (l + 1 < L && s >= s_base[l + 1]) l++;
+        const uint32_t n = (uint32_t)a.off[l] + (s - s_base[l]);
+        const TLink k = a.tlink[n];
+        s_pos[s] = a.tpos[n];
+        uint32_t last = k.last, next = k.next;
+        if (l > lt) last = kTopBit | ((l - 1u) << 24) | (last - (uint32_t)a.off[l - 1] + s_base[l - 1]);
+        else if (l == 0u) last = kNextEnd;  // a leaf: never opened
+        if (next != kNextEnd) {
+            const uint32_t h = tn_level(k.info);  // the layer of the next node
+            next = kTopBit | (h << 24) | (next - (uint32_t)a.off[h] + s_base[h]);
+        }
+        s_link[s] = make_uint2(last, next);
+        if (QUAD) {
+            s_q0[s] = a.qmom[2 * (size_t)n];
+            const float4 q = a.qmom[2 * (size_t)n + 1];
+            s_q1[s] = make_float2(q.x, q.y);
+        }
+    }
+    __syncthreads();
+    TopView tv;
+    tv.pos = (const ASTRO_LDS lds_f4*)s_pos;
+    tv.link = (const ASTRO_LDS lds_u2*)s_link;
+    tv.q0 = (const ASTRO_LDS lds_f4*)s_q0;
+    tv.q1 = (const ASTRO_LDS lds_f2*)s_q1;
+    tv.lt = lt;
+    tv.top_lo = lt < L ? (uint32_t)a.off[lt] : 0xFFFFFFFFu;
+    return tv;
+}
+
+// Targets in Morton order (sorted position j), one per thread.  TOPC: the top layers are read
+// from LDS (fill_top); needs 32-bit node offsets.
+template <int NT, bool KAHAN, int MODE, bool QUAD, bool FAST, bool OFF64, bool G1, bool TOPC>
 __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
+    static_assert(!(TOPC && OFF64), "the LDS top needs 32-bit node offsets");
+    constexpr uint32_t kSl = TOPC ? TopCap<NT, QUAD>::kSlots : 1u;
+    __shared__ float4 s_pos[kSl];
+    __shared__ uint2 s_link[kSl];
+    __shared__ float4 s_q0[TOPC && QUAD ? kSl : 1u];
+    __shared__ float2 s_q1[TOPC && QUAD ? kSl : 1u];
+    __shared__ uint32_t s_base[kMaxLayers];
+    __shared__ uint32_t s_meta[2];
     // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
     // x-th contiguous eighth of the target tiles (best L2 locality); xcd_chunk K deals runs of
     // K consecutive tiles to the XCDs in turn (better balance when work varies along the
@@ -320,6 +459,8 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
                                                          __HIP_MEMORY_SCOPE_AGENT)) &
         ERR_CAPACITY)
         return;
+    TopView tv{};
+    if (TOPC) tv = fill_top<NT, QUAD>(a, s_pos, s_link, s_q0, s_q1, s_base, s_meta);
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
@@ -345,17 +486,23 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         const uint32_t pidx = j < a.nt ? a.sorted_idx[j] : j;
         const float4 p = a.pos[pidx];
         const uint32_t type = a.type[pidx];
-        const uint32_t self =
+        uint32_t self =
             (type != 2u && j < a.nt) ? (uint32_t)(a.off0 + a.leaf_of_sorted[j]) : 0xFFFFFFFFu;
+        uint32_t root = (uint32_t)(a.root_off + sys);
+        if (TOPC && tv.lt < a.layers) {
+            // the roots are top nodes; so are the leaves when the whole tree is cached
+            root = kTopBit | ((a.layers - 1u) << 24) | (s_base[a.layers - 1u] + sys);
+            if (tv.lt == 0u && self != 0xFFFFFFFFu)
+                self = kTopBit | (self - (uint32_t)a.off0 + s_base[0]);
+        }
         Acc3 F;
         double phi = 0.0;
         if (fdiv)
-            traverse<KAHAN, MODE, QUAD, FAST, true, OFF64, G1>(a, p, self, sys, rmin0,
-                                                           QUAD ? accept_r2min(rmin0) : 0.0f,
-                                                           F, phi, st);
+            traverse<KAHAN, MODE, QUAD, FAST, true, OFF64, G1, TOPC>(
+                a, tv, root, p, self, rmin0, QUAD ? accept_r2min(rmin0) : 0.0f, F, phi, st);
         else
-            traverse<KAHAN, MODE, QUAD, FAST, false, OFF64, G1>(a, p, self, sys, rmin0, 0.0f,
-                                                                F, phi, st);
+            traverse<KAHAN, MODE, QUAD, FAST, false, OFF64, G1, TOPC>(a, tv, root, p, self, rmin0,
+                                                                      0.0f, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
@@ -479,35 +626,47 @@ uint64_t check_reciprocal(unsigned long long* d_mismatch, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? (uint64_t)(hi - lo) : 0u;
 }
 
-template <int NT, bool KAHAN, bool QUAD, bool FAST, bool OFF64, bool G1>
+template <int NT, bool KAHAN, bool QUAD, bool FAST, bool OFF64, bool G1, bool TOPC>
 static void launch_force_modes(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t blocks = (a.ns + NT - 1) / NT;
     if (mode == 0)
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, OFF64, G1>), dim3(blocks),
-                           dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 0, QUAD, FAST, OFF64, G1, TOPC>),
+                           dim3(blocks), dim3(NT), 0, s, a);
     else if (mode == 1)
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, OFF64, G1>), dim3(blocks),
-                           dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 1, QUAD, FAST, OFF64, G1, TOPC>),
+                           dim3(blocks), dim3(NT), 0, s, a);
     else
-        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, OFF64, G1>), dim3(blocks),
-                           dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_force_compact<NT, KAHAN, 2, QUAD, FAST, OFF64, G1, TOPC>),
+                           dim3(blocks), dim3(NT), 0, s, a);
+}
+
+// 64-bit offsets from 2^27 nodes (needed from 2^28), where the moment rows stop carrying
+// links; `wide` asks for them on any tree (a check of that path on small trees)
+bool force_off64(const ForceArgs& a) { return a.node_total >= kLinkPackNodes || a.wide; }
+
+template <int NT, bool KAHAN, bool QUAD, bool FAST, bool G1>
+static void launch_force_off(const ForceArgs& a, int mode, hipStream_t s) {
+    constexpr bool kTopBlock = NT >= 512;  // the LDS top needs many waves per block
+    if (force_off64(a)) {
+        launch_force_modes<NT, KAHAN, QUAD, FAST, true, G1, false>(a, mode, s);
+    } else if (kTopBlock && a.top) {
+        launch_force_modes<NT, KAHAN, QUAD, FAST, false, G1, kTopBlock>(a, mode, s);
+    } else {
+        launch_force_modes<NT, KAHAN, QUAD, FAST, false, G1, false>(a, mode, s);
+    }
 }
 
 template <int NT, bool KAHAN, bool QUAD, bool FAST>
 static void launch_force_variant(const ForceArgs& a, int mode, hipStream_t s) {
-    // 64-bit offsets from 2^27 nodes (needed from 2^28), where the moment rows stop carrying links
-    const bool off64 = a.node_total >= kLinkPackNodes;
     // G = 1 (the physics profile): the products G M are the masses themselves; compiled for
     // the plain fast paths only
     if constexpr (FAST && !KAHAN) {
         if (a.p.G == 1.0f) {
-            off64 ? launch_force_modes<NT, KAHAN, QUAD, FAST, true, true>(a, mode, s)
-                  : launch_force_modes<NT, KAHAN, QUAD, FAST, false, true>(a, mode, s);
+            launch_force_off<NT, KAHAN, QUAD, FAST, true>(a, mode, s);
             return;
         }
     }
-    off64 ? launch_force_modes<NT, KAHAN, QUAD, FAST, true, false>(a, mode, s)
-          : launch_force_modes<NT, KAHAN, QUAD, FAST, false, false>(a, mode, s);
+    launch_force_off<NT, KAHAN, QUAD, FAST, false>(a, mode, s);
 }
 
 template <int NT>
@@ -526,7 +685,9 @@ static void launch_force_nt(const ForceArgs& a, int mode, hipStream_t s) {
     }
 }
 
-bool force_block_supported(uint32_t nt) { return nt == 64 || nt == 128 || nt == 256 || nt == 512; }
+bool force_block_supported(uint32_t nt) {
+    return nt == 64 || nt == 128 || nt == 256 || nt == 512 || nt == 1024;
+}
 
 void launch_force_compact(const ForceArgs& a, hipStream_t s) {
     if (!a.ns) return;
@@ -535,6 +696,7 @@ void launch_force_compact(const ForceArgs& a, hipStream_t s) {
         case 64: launch_force_nt<64>(a, mode, s); break;
         case 128: launch_force_nt<128>(a, mode, s); break;
         case 512: launch_force_nt<512>(a, mode, s); break;
+        case 1024: launch_force_nt<1024>(a, mode, s); break;
         default: launch_force_nt<256>(a, mode, s); break;
     }
 }

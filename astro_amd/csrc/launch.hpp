@@ -170,11 +170,16 @@ struct ForceArgs {
     uint32_t* err;
     unsigned long long* stats;  // 4 counters, or null
     double* phi;                // per-particle FP64 potential, or null
+    const uint32_t* counts;     // [layers] valid nodes per layer (device)
+    uint64_t off[kMaxLayers];   // layer offsets
+    uint32_t layers;
+    uint32_t top;               // 1: the top layers from LDS (blocks of 512 or 1,024 threads)
+    uint32_t wide;              // 1: 64-bit node offsets on any tree
     uint64_t off0;              // leaf layer offset
     uint64_t root_off;          // top layer offset (root of system s at root_off + s)
     uint64_t node_total;        // nodes in the current layer table (index bound)
     uint32_t ns, nt;
-    uint32_t block;             // traversal block size (64, 128, 256 or 512)
+    uint32_t block;             // traversal block size (64, 128, 256, 512 or 1024)
     uint32_t xcd_chunk;         // tiles per run dealt to the XCDs in turn (0: contiguous eighths)
     AcceptRule acc;             // opening test (accept.hpp), from p.theta
     uint32_t fast;              // 1: eps2 >= max(r2_floor, 2^-96, leaf_cutoff_r2+),
@@ -197,6 +202,8 @@ uint64_t check_reciprocal(unsigned long long* d_mismatch, hipStream_t s);
 // Fills acc and fast from p.
 void set_accept_threshold(ForceArgs& a);
 bool force_block_supported(uint32_t nt);
+// The traversal's addressing: 64-bit node offsets (trees of 2^27 nodes or more, or `wide`).
+bool force_off64(const ForceArgs& a);
 void launch_force_compact(const ForceArgs& a, hipStream_t s);
 void launch_force_reference(const ForceArgs& a, hipStream_t s);
 void launch_diag(const float4* pos, const float4* vel, const double* phi,

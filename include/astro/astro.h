@@ -39,9 +39,18 @@ typedef struct {
     uint32_t max_tree;          /* N_t: tree sources (the first N_t particles); 0 = N_s   */
     uint32_t bits_per_axis;     /* largest B that will be used (tree capacity)           */
     uint32_t max_systems;       /* independent systems per batched step (>= 1)           */
-    uint32_t emit_octree_nodes; /* 1: keep the 128-B OctreeNode array up to date         */
+    uint32_t emit_octree_nodes; /* when the 128-B OctreeNode array is written: one of    */
+                                /* ASTRO_EMIT_NEVER, _EVERY_BUILD, _ON_READ              */
     int32_t device;             /* HIP device ordinal                                    */
 } astro_config;
+
+/* When the OctreeNode array (bridge.h:47-65, the reference's node buffer) is written.  The
+ * traversal reads the compact nodes, so the array is an output for callers only:
+ *   NEVER       not written by the fused build (the stage calls still write it);
+ *   EVERY_BUILD written by every build, in a step beside the traversal (on a second stream);
+ *   ON_READ     written, with the same bytes, when a caller asks for it: astro_get_buffer of
+ *               ASTRO_BUF_OCTREE_NODES or astro_barnes_hut (default). */
+enum { ASTRO_EMIT_NEVER = 0, ASTRO_EMIT_EVERY_BUILD = 1, ASTRO_EMIT_ON_READ = 2 };
 
 /* Buffers reachable through astro_get_buffer (names follow Renderer.swift:56-70). */
 enum {
@@ -90,8 +99,9 @@ void astro_params_compat(astro_params* p);                    /* the reference's
 void astro_params_physics(astro_params* p, uint32_t bits_per_axis);
 int astro_set_params(astro_ctx* ctx, const astro_params* p);
 int astro_get_params(const astro_ctx* ctx, astro_params* p);
-/* 1: the fused build also writes the 128-B OctreeNode array (default: config value). */
-int astro_set_emit_octree_nodes(astro_ctx* ctx, int on);
+/* When the OctreeNode array is written (ASTRO_EMIT_*; default: the config value).  A pending
+ * array of the current tree is written first. */
+int astro_set_emit_octree_nodes(astro_ctx* ctx, int mode);
 /* 1: astro_step replays one captured hipGraph per step (default). */
 int astro_set_use_graph(astro_ctx* ctx, int on);
 
@@ -178,8 +188,17 @@ int astro_step_timed(astro_ctx* ctx, float dt, uint32_t nsteps, float* stage_ms)
 /* Work figures of the current tree and positions (one extra counting traversal; needs a built
  * tree, as astro_compute_forces). */
 int astro_traversal_stats(astro_ctx* ctx, astro_stats* out);
-/* Traversal block size (64, 128, 256 or 512; default 256). */
+/* Traversal block size (64, 128, 256, 512 or 1024; default 256). */
 int astro_set_force_block(astro_ctx* ctx, uint32_t block);
+/* With blocks of 512 or 1,024 threads, each block first copies the top layers of the tree (as
+ * many as fit in its LDS, chosen from the layer counts on the device) into LDS, and the walks
+ * read them there (default on; the results are the same bits either way).  Replaces nothing in
+ * the reference: BarnesHut.metal:22-101 reads every node from device memory. */
+int astro_set_force_top_cache(astro_ctx* ctx, int on);
+/* Check switch: build and traverse every tree as if it held 2^27 nodes or more (64-bit node
+ * offsets, no links in the moment rows), so that path can be compared with the oracle on small
+ * trees.  Takes effect at the next build.  No counterpart in the reference. */
+int astro_set_wide_offsets(astro_ctx* ctx, int on);
 /* How target tiles are dealt to the 8 XCDs: 0 (default) gives each XCD one contiguous eighth
  * of the Morton curve; K > 0 deals runs of K consecutive tiles to the XCDs in turn. */
 int astro_set_force_xcd_chunk(astro_ctx* ctx, uint32_t chunk);

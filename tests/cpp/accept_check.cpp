@@ -12,7 +12,7 @@
 
 int main() {
     std::mt19937_64 rng(12345);
-    const float thetas[] = {0.7f, 0.5f, 0.3f, 1.0f, 0.7000001f, 0.9999999f, 0.25f, 1.7f};
+    const float thetas[] = {0.7f, 0.5f, 0.55f, 0.6f, 0.3f, 1.0f, 0.7000001f, 0.9999999f, 0.25f, 1.7f};
     long bad = 0, near = 0, total = 0;
     for (float th : thetas) {
         const astro::AcceptRule rule = astro::make_accept_rule(th);

@@ -6,6 +6,7 @@ import sys
 
 import numpy as np
 import pytest
+from gpu_util import same_bytes
 
 pytestmark = pytest.mark.gpu
 
@@ -50,8 +51,8 @@ def test_batched_equals_alone_and_oracle(astro, orc, graph, multipole):
             one.check()
             pa, va, _ = one.download()
             sl = slice(off[k], off[k + 1])
-            assert pb[sl].tobytes() == pa.tobytes(), f"system {k} differs from alone"
-            assert vb[sl].tobytes() == va.tobytes()
+            assert same_bytes(pb[sl], pa), f"system {k} differs from alone"
+            assert same_bytes(vb[sl], va)
             po, vo = p, v
             for _ in range(steps):
                 po, vo, _, _ = orc.step(po, vo, c, Po, 1e-3, threads=8)
@@ -105,15 +106,15 @@ def test_server_alone_vs_mix(astro):
     others = [(10 + k, k % 4, *astro.ic_plummer(1000 + 700 * k, 50 + k), 2 + 3 * k)
               for k in range(6)]
     mix, recs = run_server(astro, P, [(1, 2, *a, 6)] + others, 100000, 16)
-    assert alone[1][0].tobytes() == mix[1][0].tobytes()
-    assert alone[1][1].tobytes() == mix[1][1].tobytes()
+    assert same_bytes(alone[1][0], mix[1][0])
+    assert same_bytes(alone[1][1], mix[1][1])
     assert len(recs) == 7 and all(r["t_done"] >= r["t_admit"] >= r["t_submit"] for r in recs)
     # and the plain single-system path agrees
     with astro.Context(3000, bits_per_axis=21) as ctx:
         ctx.set_params(P)
         ctx.upload(*a)
         ctx.step(1e-3, 6)
-        assert ctx.download()[0].tobytes() == alone[1][0].tobytes()
+        assert same_bytes(ctx.download()[0], alone[1][0])
 
 
 def test_server_admission_respects_budget(astro):
@@ -170,12 +171,12 @@ def test_autotune_keeps_results(astro):
         ctx.compute_forces()
         f0 = ctx.forces()
         b, k, ms = ctx.autotune(3)
-        assert b in (64, 128, 256, 512) and k in (0, 8, 32, 128) and ms > 0
-        for blk, chunk in ((64, 0), (128, 8), (512, 32), (256, 1), (256, 4096)):
+        assert b in (64, 128, 256, 512, 1024) and k in (0, 8, 32, 128) and ms > 0
+        for blk, chunk in ((64, 0), (128, 8), (512, 32), (256, 1), (256, 4096), (1024, 0)):
             ctx.set_force_block(blk)
             ctx.set_force_xcd_chunk(chunk)
             ctx.compute_forces()
-            assert ctx.forces().tobytes() == f0.tobytes()
+            assert same_bytes(ctx.forces(), f0)
         with pytest.raises(astro.AstroError):
             ctx.set_force_xcd_chunk(4097)
         ctx.check()
@@ -281,4 +282,4 @@ def test_snapshot_resume_is_bitexact(astro, tmp_path, batched):
         ctx2.check()
         b = ctx2.download()
     for x, y in zip(a, b):
-        assert x.tobytes() == y.tobytes()
+        assert same_bytes(x, y)

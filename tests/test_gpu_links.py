@@ -99,15 +99,19 @@ def test_links_compat_subset(astro):
         check_links(ctx, 1)
 
 
-def test_links_two_disks_500k(astro):
-    say("links two disks 500k")
+@pytest.mark.parametrize("multipole", [0, 2])
+def test_links_two_disks_500k(astro, multipole):
+    # with quadrupoles the links packed into the moment rows come from the fused build too
+    say("links two disks 500k, multipole", multipole)
     pos, vel, col = astro.ic_two_disks(250000, 2024)
     with astro.Context(len(pos), bits_per_axis=21, emit_octree_nodes=False) as ctx:
-        ctx.set_params(astro.params_physics(21))
+        P = astro.params_physics(21)
+        P.multipole = multipole
+        ctx.set_params(P)
         ctx.upload(pos, vel, col)
         ctx.build_tree()
         ctx.check()
-        check_links(ctx, 1)
+        check_links(ctx, 1, multipole)
 
 
 @pytest.mark.parametrize("multipole", [0, 2])

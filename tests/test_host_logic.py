@@ -13,8 +13,8 @@ def test_division_free_opening_test_is_exact(tmp_path):
     subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
                            os.path.join(ROOT, "tests", "cpp", "accept_check.cpp"), "-o", str(exe)])
     bad, near, total, bad2, total2 = map(int, subprocess.check_output([str(exe)]).split())
-    assert total == 1600000 and near == 800000
+    assert total == 2000000 and near == 1000000  # 10 thetas x 200,000
     assert bad == 0
     # the squared form  r2 >= accept_r2min(rmin0) 4^l
-    assert total2 > 1500000
+    assert total2 > 1900000
     assert bad2 == 0
