@@ -110,6 +110,7 @@ SIGNATURES = [
     ("astro_set_force_block", _I, [_P, _U32]),
     ("astro_set_force_xcd_chunk", _I, [_P, _U32]),
     ("astro_set_force_top_cache", _I, [_P, _I]),
+    ("astro_scan_noalloc", _I, [_P, _P, _U32, _P]),
     ("astro_set_wide_offsets", _I, [_P, _I]),
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32),
                             ctypes.POINTER(_F)]),

@@ -142,6 +142,14 @@ int astro_radix_sort(astro_ctx* ctx, const uint64_t* d_keys_in, const uint32_t* 
                      const uint32_t* d_count);
 /* RadixSort.swift:225-279: exclusive scan of len u32 values */
 int astro_scan(astro_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32_t len);
+/* astro_scan without a context and without allocating: the same kernels and results, with
+ * their per-tile sums in a fixed device array of the library (64 KiB per GPU, kept until the
+ * process ends).  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses that forbid it).
+ * Runs on the current device and on `stream` (a hipStream_t; NULL: the null stream);
+ * asynchronous.  len <= 33,554,432 (16,384 tiles of 2,048), else ASTRO_ERR_CAPACITY.  Calls
+ * that overlap on one GPU share the array and race: order them on one stream. */
+int astro_scan_noalloc(const uint32_t* d_in, uint32_t* d_out, uint32_t len, void* stream);
 /* RadixSort.swift:312-349: one stable 1-bit split pass */
 int astro_split_pass(astro_ctx* ctx, const uint64_t* d_keys_in, const uint32_t* d_idx_in,
                      uint64_t* d_keys_out, uint32_t* d_idx_out, uint32_t bit,
