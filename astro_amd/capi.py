@@ -15,7 +15,8 @@ from .structs import (AstroParams, SnapshotHeader, COLOR_TYPE, OCTREE_NODE, POSI
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libastro.so")
 
-OK, ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_STATE, ERR_DEVICE, ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
+OK, ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_STATE, ERR_DEVICE, ERR_NOMEM, ERR_BUSY = \
+    0, -1, -2, -3, -4, -5, -6, -7
 
 BUF = dict(POSITION_MASS=0, VELOCITY_RADIUS=1, TYPE=2, FORCE=3, MORTON_CODES=4,
            UNSORTED_CODES=5, UNSORTED_INDICES=6, SORTED_CODES=7, SORTED_INDICES=8,
@@ -39,20 +40,26 @@ class AstroConfig(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_particles", ctypes.c_uint32), ("max_systems", ctypes.c_uint32),
-                ("hbm_budget", ctypes.c_uint64), ("device", ctypes.c_int32)]
+                ("hbm_budget", ctypes.c_uint64), ("device", ctypes.c_int32),
+                ("max_extent", ctypes.c_float), ("max_wait_s", ctypes.c_double)]
+
+
+REQUEST_DONE, REQUEST_FAILED = 0, 1
 
 
 class Completion(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("n", ctypes.c_uint32), ("steps", ctypes.c_uint32),
                 ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
-                ("t_done", ctypes.c_double)]
+                ("t_done", ctypes.c_double), ("status", ctypes.c_uint32),
+                ("steps_done", ctypes.c_uint32)]
 
 
 class TickInfo(ctypes.Structure):
     _fields_ = [("admitted", ctypes.c_uint32), ("completed", ctypes.c_uint32),
                 ("active", ctypes.c_uint32), ("queued", ctypes.c_uint32),
                 ("particles", ctypes.c_uint32), ("active_after", ctypes.c_uint32),
-                ("step_ms", ctypes.c_double)]
+                ("step_ms", ctypes.c_double), ("failed", ctypes.c_uint32),
+                ("synced", ctypes.c_uint32)]
 
 
 class AstroStats(ctypes.Structure):
@@ -132,6 +139,7 @@ SIGNATURES = [
     ("astro_server_device_bytes", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     ("astro_server_submit", _I, [_P, _U64, _P, _P, _P, _U32, _U32]),
     ("astro_server_tick", _I, [_P, _F, ctypes.POINTER(TickInfo)]),
+    ("astro_server_sync", _I, [_P, ctypes.POINTER(TickInfo)]),
     ("astro_server_poll", _I, [_P, _P, _U32, ctypes.POINTER(_U32)]),
     ("astro_server_fetch", _I, [_P, _U64, _P, _P]),
     ("astro_server_counters", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64),
@@ -562,9 +570,11 @@ class Context:
 class Server:
     """Continuous-batching server (astro_server_* in include/astro/astro.h)."""
 
-    def __init__(self, params, max_particles, max_systems, hbm_budget=0, device=0):
+    def __init__(self, params, max_particles, max_systems, hbm_budget=0, device=0,
+                 max_extent=0.0, max_wait_s=0.0):
         self.lib = load()
-        cfg = ServerConfig(max_particles, max_systems, hbm_budget, device)
+        cfg = ServerConfig(max_particles, max_systems, hbm_budget, device, max_extent,
+                           max_wait_s)
         h = ctypes.c_void_p()
         rc = self.lib.astro_server_create(ctypes.byref(cfg), ctypes.byref(params), ctypes.byref(h))
         if rc != OK:
@@ -608,6 +618,12 @@ class Server:
     def tick(self, dt):
         ti = TickInfo()
         self._chk(self.lib.astro_server_tick(self.h, dt, ctypes.byref(ti)))
+        return {f: getattr(ti, f) for f, _ in TickInfo._fields_}
+
+    def sync(self):
+        """Waits for the device and settles the enqueued ticks."""
+        ti = TickInfo()
+        self._chk(self.lib.astro_server_sync(self.h, ctypes.byref(ti)))
         return {f: getattr(ti, f) for f, _ in TickInfo._fields_}
 
     def poll(self, max_records=4096):

@@ -1075,7 +1075,8 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     if (ev) record_stage(c, ev[4]);
     // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
     if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
-    launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream);
+    launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream,
+                  c->nsys > 1 ? c->sys_of : nullptr, c->guard, c->guard_extent);
     if (beside) {
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
         launch_copy_u32(c->layer_counts, c->layer_counts_prev, kMaxLayers, c->stream);

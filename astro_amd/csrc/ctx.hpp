@@ -64,6 +64,8 @@ struct astro_ctx {
     double* diag_partial = nullptr;    // [max_systems * kDiagChunks * 9]
     uint32_t force_block = 256;        // traversal block size (autotuned)
     uint32_t force_chunk = 0;          // traversal tile runs per XCD (autotuned; 0: eighths)
+    uint32_t* guard = nullptr;         // per-system state flags raised by the integrator (server)
+    float guard_extent = 0.0f;         // |coordinate| bound of the guard (0: finiteness only)
     bool force_top = true;             // traversal reads the top layers from LDS (blocks >= 512)
     bool force_wide = false;           // 64-bit node offsets and unpacked moment rows on any tree
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation

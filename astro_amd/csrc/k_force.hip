@@ -10,6 +10,7 @@
 //
 // k_force_reference: the reference kernel shape (BarnesHut.metal:22-101) over the 128-B
 // OctreeNode array with children[8], for callers that built the tree with the stage calls.
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -782,11 +783,15 @@ void launch_force_reference(const ForceArgs& a, hipStream_t s) {
 }
 
 // BarnesHut.metal:169-196 (updateSpheres): a = F*scale[type]; v = v*damping + a*dt;
-// x = x + v*dt -- separate roundings, no contraction.
+// x = x + v*dt -- separate roundings, no contraction.  guard (optional): flags the system of
+// every particle whose new position or velocity is not finite or whose position leaves
+// [-extent, extent]^3 (the server evicts such systems; the arithmetic is unchanged).
 __global__ __launch_bounds__(256) void k_update(float4* __restrict__ pos, float4* __restrict__ vel,
                                                 const float4* __restrict__ force,
                                                 const uint32_t* __restrict__ type, uint32_t n,
-                                                float dt, astro_params p) {
+                                                float dt, astro_params p,
+                                                const uint32_t* __restrict__ sys_of,
+                                                uint32_t* __restrict__ guard, float extent) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t t = type[i];
@@ -806,13 +811,20 @@ __global__ __launch_bounds__(256) void k_update(float4* __restrict__ pos, float4
     x.z = x.z + v.z * dt;
     vel[i] = v;
     pos[i] = x;
+    if (guard) {
+        // NaN fails every comparison, so a non-finite value also raises the flag
+        const bool ok = fabsf(x.x) <= extent && fabsf(x.y) <= extent && fabsf(x.z) <= extent &&
+                        fabsf(v.x) <= FLT_MAX && fabsf(v.y) <= FLT_MAX && fabsf(v.z) <= FLT_MAX;
+        if (!ok) atomicOr(&guard[sys_of ? sys_of[i] : 0u], 1u);
+    }
 }
 
 void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,
-                   uint32_t n, float dt, const astro_params& p, hipStream_t s) {
+                   uint32_t n, float dt, const astro_params& p, hipStream_t s,
+                   const uint32_t* sys_of, uint32_t* guard, float extent) {
     if (!n) return;
     hipLaunchKernelGGL(k_update, dim3((n + 255) / 256), dim3(256), 0, s, pos, vel, force, type, n,
-                       dt, p);
+                       dt, p, sys_of, guard, extent > 0.0f ? extent : FLT_MAX);
 }
 
 // FP64 diagnostics: per (system, chunk) sums of {m, kinetic, potential (1/2 m phi), m v,

@@ -212,8 +212,13 @@ void launch_diag(const float4* pos, const float4* vel, const double* phi,
 void launch_lighting(const float4* pos, float4* color, const uint32_t* type, uint32_t n,
                      uint32_t nstars, const uint32_t* sys_of, const uint32_t* sys_off,
                      hipStream_t s);
+// guard (optional, one word per system): set to nonzero for the system of every particle
+// whose new state is not finite or whose position leaves [-extent, extent]^3 (extent <= 0:
+// finiteness only).
 void launch_update(float4* pos, float4* vel, const float4* force, const uint32_t* type,
-                   uint32_t n, float dt, const astro_params& p, hipStream_t s);
+                   uint32_t n, float dt, const astro_params& p, hipStream_t s,
+                   const uint32_t* sys_of = nullptr, uint32_t* guard = nullptr,
+                   float extent = 0.0f);
 
 void launch_direct_fp64(const float4* pos, const uint32_t* sys_of, const uint32_t* d_sys_off,
                         uint32_t nt, const uint32_t* targets, uint32_t n_targets, uint32_t chunks,

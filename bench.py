@@ -53,17 +53,24 @@ def parse():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=4)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="serving stream: Poisson arrivals per second (200: the stream's own)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-steps-min", type=int, default=10)
-    ap.add_argument("--serve-steps-max", type=int, default=100)
-    ap.add_argument("--serve-pool", type=int, default=24)
+    ap.add_argument("--serve-steps-max", type=int, default=1000)
+    ap.add_argument("--serve-max-wait", type=float, default=5.0,
+                    help="load shedding: refuse a request whose estimated wait exceeds this (s)")
+    ap.add_argument("--serve-nominal-rate", type=float, default=2.0,
+                    help="a second stream below capacity, for latency without queueing")
+    ap.add_argument("--serve-nominal-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-drain-requests", type=int, default=48,
+                    help="the first requests of the serving stream, all admitted, through both "
+                         "forms (throughput on the same work, without shedding)")
+    ap.add_argument("--serve-pool", type=int, default=32)
     ap.add_argument("--serve-capacity", type=int, default=8 << 20)
     ap.add_argument("--acc-steps", type=int, default=50,
                     help="timed steps of the accuracy-matched configuration")
     ap.add_argument("--acc-targets", type=int, default=4096)
-    ap.add_argument("--overload-rate", type=float, default=200.0)
-    ap.add_argument("--overload-seconds", type=float, default=1.5)
     ap.add_argument("--skip-accuracy", action="store_true")
     ap.add_argument("--large-n", type=int, default=4194304)
     ap.add_argument("--large-steps", type=int, default=20)
@@ -74,6 +81,14 @@ def parse():
                     help="profile mode: also copy this many MiB device-to-device (a plain copy "
                          "of known size that scales the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+_T_START = time.perf_counter()
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON result is the one line on stdout)."""
+    print(f"[bench {time.perf_counter() - _T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def stage_bytes(n, V, passes=8, emit=True, quad=False):
@@ -191,9 +206,13 @@ def serving_pool(astro, a):
     return pool
 
 
-def bench_serving(astro, a, pool, rate, seconds, seed):
-    """Poisson arrivals of Plummer / two-disk systems (N log-uniform in [2048, 2^20]), served
-    with continuous batching; the same request sequence through the plain FIFO loop."""
+def bench_serving(astro, a, pool, rate, seconds, seed, drain=0):
+    """Poisson arrivals of Plummer / two-disk systems (N log-uniform in [2048, 2^20], steps
+    uniform in [steps_min, steps_max]) for `seconds`, served with continuous batching; the
+    same arrivals through the plain FIFO loop (one request at a time, each in a single-system
+    context).  Both shed load the same way: a request whose estimated wait (the work ahead of
+    it and its own, over the measured throughput) exceeds max_wait is refused.  Throughput is
+    the completed work over the time from the first arrival to the last completion."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
@@ -203,56 +222,97 @@ def bench_serving(astro, a, pool, rate, seconds, seed):
             break
         reqs.append((len(reqs), t, int(rng.integers(0, len(pool))),
                      int(rng.integers(a.serve_steps_min, a.serve_steps_max + 1))))
-    work = sum(len(pool[k][0]) * st for _, _, k, st in reqs)
+        if drain and len(reqs) == drain:
+            break
+    max_wait = 0.0 if drain else a.serve_max_wait
+    wk = lambda k, st: len(pool[k][0]) * st  # noqa: E731
+    offered = sum(wk(k, st) for _, _, k, st in reqs)
     P = astro.params_physics(21)
-    out = {"requests": len(reqs), "arrival_rate_per_s": rate,
-           "arrival_seconds": seconds,
+    out = {"requests": len(reqs), "arrival_rate_per_s": rate, "arrival_seconds": seconds,
            "steps_range": [a.serve_steps_min, a.serve_steps_max],
-           "offered_particle_steps_per_s": work / seconds,
+           "max_wait_s": max_wait,
+           "offered_particle_steps_per_s": offered / seconds,
            "pool": "%d seeded systems, N log-uniform in [2048, 1048576], Plummer and two-disk "
                    "alternating" % a.serve_pool}
-    srv = astro.Server(P, a.serve_capacity, 512)
+    # served: the continuous-batching server
+    srv = astro.Server(P, a.serve_capacity, 512, max_wait_s=max_wait)
     pending = list(reqs)
-    lat = []
+    lat, done_work, rejected = [], 0, 0
     t0 = time.perf_counter()
+    t_note = t0
+    progress(f"serving {rate:g}/s for {seconds:g} s: {len(reqs)} requests, served")
     while pending or srv.counters()["active"] or srv.counters()["queued"]:
         now = time.perf_counter() - t0
+        if time.perf_counter() - t_note > 15:
+            t_note = time.perf_counter()
+            progress(f"  served: {len(reqs) - len(pending)} arrived, {len(lat)} completed, "
+                     f"{rejected} refused")
         while pending and pending[0][1] <= now:
             rid, _, k, st = pending.pop(0)
-            srv.submit(rid, pool[k][0], pool[k][1], pool[k][2], st)
+            try:
+                srv.submit(rid, pool[k][0], pool[k][1], pool[k][2], st)
+            except astro.AstroError as e:
+                if e.code != -7:  # ASTRO_ERR_BUSY
+                    raise
+                rejected += 1
         if srv.counters()["active"] or srv.counters()["queued"]:
             srv.tick(a.dt)
         else:
             time.sleep(0.0005)
         for rec in srv.poll():
             lat.append(rec["t_done"] - rec["t_submit"])
+            done_work += rec["n"] * rec["steps_done"]
             srv.fetch(rec["id"], rec["n"])
-    el = time.perf_counter() - t0
+    el = time.perf_counter() - t0 - reqs[0][1]
     srv.close()
     lat = np.array(lat)
-    out["served"] = {"value": work / el, "seconds": el, "p50_latency_s": float(np.percentile(lat, 50)),
-                     "p99_latency_s": float(np.percentile(lat, 99)), "completed": len(lat)}
-    # plain loop: one request at a time, FIFO, each in a single-system context
+    out["served"] = {"value": done_work / el, "seconds": el, "admitted": len(reqs) - rejected,
+                     "rejected": rejected, "completed": len(lat),
+                     "p50_latency_s": float(np.percentile(lat, 50)),
+                     "p99_latency_s": float(np.percentile(lat, 99))}
+    # plain loop: one request at a time, FIFO, each in a single-system context, the same
+    # shedding rule with its own measured throughput
     ctx = astro.Context(int(max(len(p[0]) for p in pool)), bits_per_axis=21, emit_octree_nodes=False)
     ctx.set_params(P)
     pending = list(reqs)
-    lat = []
+    queue, lat, done_work, rejected, prate = [], [], 0, 0, 0.0
     t0 = time.perf_counter()
-    while pending:
+    t_note = t0
+    progress("  plain loop")
+    while pending or queue:
         now = time.perf_counter() - t0
-        if pending[0][1] > now:
-            time.sleep(pending[0][1] - now)
-        rid, tarr, k, st = pending.pop(0)
+        if time.perf_counter() - t_note > 15:
+            t_note = time.perf_counter()
+            progress(f"  plain: {len(reqs) - len(pending)} arrived, {len(lat)} completed, "
+                     f"{rejected} refused")
+        while pending and pending[0][1] <= now:
+            r = pending.pop(0)
+            ahead = sum(wk(q[2], q[3]) for q in queue) + wk(r[2], r[3])
+            if max_wait > 0 and prate > 0 and ahead / prate > max_wait:
+                rejected += 1
+            else:
+                queue.append(r)
+        if not queue:
+            if pending:
+                time.sleep(max(0.0, pending[0][1] - (time.perf_counter() - t0)))
+            continue
+        rid, tarr, k, st = queue.pop(0)
+        ts = time.perf_counter()
         ctx.upload(*pool[k])
         ctx.step(a.dt, st)
         ctx.sync()
         ctx.download()
-        lat.append(time.perf_counter() - t0 - tarr)
-    el = time.perf_counter() - t0
+        te = time.perf_counter()
+        r_now = wk(k, st) / (te - ts)
+        prate = r_now if prate == 0 else 0.7 * prate + 0.3 * r_now
+        done_work += wk(k, st)
+        lat.append(te - t0 - tarr)
+    el = time.perf_counter() - t0 - reqs[0][1]
     ctx.check()
     ctx.close()
     lat = np.array(lat)
-    out["plain_loop"] = {"value": work / el, "seconds": el,
+    out["plain_loop"] = {"value": done_work / el, "seconds": el, "admitted": len(reqs) - rejected,
+                         "rejected": rejected, "completed": len(lat),
                          "p50_latency_s": float(np.percentile(lat, 50)),
                          "p99_latency_s": float(np.percentile(lat, 99))}
     out["served_over_plain"] = out["served"]["value"] / out["plain_loop"]["value"]
@@ -516,19 +576,27 @@ def main():
                                     "what": "the same steps writing the 128-B OctreeNode array "
                                             "every step, beside the traversal"},
     }
+    progress(f"headline: {value:.4g} particle-steps/s")
     if not a.skip_accuracy:
+        progress("accuracy")
         out["accuracy"] = accuracy(astro_amd, ctx, pos, vel, col, P, a)
     ctx.close()
     if not a.skip_large:
+        progress("large system")
         out["large"] = bench_large(astro_amd, a)
     if not a.skip_batched:
+        progress("batched systems")
         out["batched"] = bench_batched(astro_amd, a)
     if not a.skip_serving:
         pool = serving_pool(astro_amd, a)
         out["serving"] = bench_serving(astro_amd, a, pool, a.serve_rate, a.serve_seconds, 5)
-        out["serving_overload"] = bench_serving(astro_amd, a, pool, a.overload_rate,
-                                                a.overload_seconds, 6)
+        out["serving_drain"] = bench_serving(astro_amd, a, pool, a.serve_rate, a.serve_seconds,
+                                             5, drain=a.serve_drain_requests)
+        out["serving_nominal"] = bench_serving(astro_amd, a, pool, a.serve_nominal_rate,
+                                               a.serve_nominal_seconds, 6)
+        del out["serving_nominal"]["served_over_plain"]  # both keep up: latency is the figure
     if not a.no_cpu_baseline:
+        progress("CPU baseline")
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         cv, el = cpu_baseline(pos, vel, col, a.dt, a.cpu_steps, threads)
         out["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": threads,

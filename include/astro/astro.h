@@ -29,6 +29,7 @@ extern "C" {
 #define ASTRO_ERR_STATE (-4)    /* call out of order (e.g. no particles uploaded)      */
 #define ASTRO_ERR_DEVICE (-5)   /* a kernel raised its error word (see astro_check)    */
 #define ASTRO_ERR_NOMEM (-6)    /* device allocation failed / HBM budget exceeded      */
+#define ASTRO_ERR_BUSY (-7)     /* the server sheds load: the request would wait too long */
 
 typedef struct astro_ctx astro_ctx;
 
@@ -253,23 +254,34 @@ int astro_device_bytes(const astro_ctx* ctx, uint64_t* bytes);
 
 /* ---- continuous-batching server (no reference counterpart: the reference renders one
  *      system).  Requests wait FIFO; at each tick the server retires finished systems,
- *      admits queued ones while the batch fits its HBM budget and capacities, and runs one
- *      batched step.  A request's result does not depend on what it was batched with. ---- */
+ *      evicts failed ones, admits queued ones while the batch fits its HBM budget and
+ *      capacities, and enqueues one batched step (a captured graph while the batch is
+ *      unchanged).  The host waits for the device only when a request finishes and every
+ *      16 ticks.  A request's result does not depend on what it was batched with. ---------- */
 typedef struct astro_server astro_server;
 typedef struct {
     uint32_t max_particles;  /* particles the batch holds at once                         */
     uint32_t max_systems;    /* systems the batch holds at once                           */
     uint64_t hbm_budget;     /* device bytes admitted requests may occupy (0: no limit)   */
     int32_t device;
+    float max_extent;        /* > 0: a system with a particle outside [-max_extent,       */
+                             /* max_extent]^3 fails; a non-finite state always fails       */
+    double max_wait_s;       /* > 0: submit refuses (ASTRO_ERR_BUSY) a request whose       */
+                             /* estimated wait before its last step exceeds this          */
 } astro_server_config;
+enum { ASTRO_REQUEST_DONE = 0, ASTRO_REQUEST_FAILED = 1 };
 typedef struct {
     uint64_t id;
     uint32_t n, steps;
     double t_submit, t_admit, t_done; /* seconds on the server's monotonic clock */
+    uint32_t status;                  /* ASTRO_REQUEST_DONE / ASTRO_REQUEST_FAILED */
+    uint32_t steps_done;              /* steps run (steps when done)               */
 } astro_completion;
 typedef struct {
     uint32_t admitted, completed, active, queued, particles, active_after;
-    double step_ms;
+    double step_ms;                   /* host time of the tick                      */
+    uint32_t failed;                  /* requests evicted by this tick              */
+    uint32_t synced;                  /* 1: the tick waited for the device          */
 } astro_tick_info;
 /* p->lighting_stars must be 0: requests carry positions and velocities only. */
 int astro_server_create(const astro_server_config* cfg, const astro_params* p,
@@ -280,7 +292,15 @@ int astro_server_device_bytes(const astro_server* s, uint64_t* bytes, uint64_t* 
 int astro_server_submit(astro_server* s, uint64_t id, const PositionMass* pos,
                         const VelocityRadius* vel, const ColorType* col, uint32_t n,
                         uint32_t steps);
+/* Errors: a request that can never fit is refused with ASTRO_ERR_CAPACITY, one with a
+ * non-finite position, mass or velocity with ASTRO_ERR_ARG, one that would wait longer than
+ * max_wait_s with ASTRO_ERR_BUSY (load shedding).  A request whose state turns non-finite or
+ * leaves max_extent is evicted (status FAILED, its state at eviction kept for fetch) while
+ * the others continue unchanged; a device error word fails every active request and the tick
+ * returns ASTRO_ERR_DEVICE, the server stays usable.  Steps are counted as enqueued. */
 int astro_server_tick(astro_server* s, float dt, astro_tick_info* info);
+/* Waits for the device and settles every enqueued tick (finished and failed requests). */
+int astro_server_sync(astro_server* s, astro_tick_info* info);
 int astro_server_poll(astro_server* s, astro_completion* out, uint32_t max, uint32_t* n_out);
 int astro_server_fetch(astro_server* s, uint64_t id, PositionMass* pos, VelocityRadius* vel);
 int astro_server_counters(const astro_server* s, uint64_t* ticks, uint64_t* particle_steps,

@@ -79,9 +79,9 @@ def test_batched_256_systems_properties(astro):
             assert np.all(keys[a + 1:b] >= keys[a:b - 1])
 
 
-def run_server(astro, P, reqs, max_particles, max_systems, budget=0, dt=1e-3):
+def run_server(astro, P, reqs, max_particles, max_systems, budget=0, dt=1e-3, **kw):
     """reqs: list of (id, tick_of_arrival, pos, vel, col, steps).  Returns results, records."""
-    srv = astro.Server(P, max_particles, max_systems, budget)
+    srv = astro.Server(P, max_particles, max_systems, budget, **kw)
     pending = sorted(reqs, key=lambda r: r[1])
     results, records, tick = {}, [], 0
     while pending or srv.counters()["active"] or srv.counters()["queued"]:
@@ -115,6 +115,56 @@ def test_server_alone_vs_mix(astro):
         ctx.upload(*a)
         ctx.step(1e-3, 6)
         assert same_bytes(ctx.download()[0], alone[1][0])
+
+
+def test_server_evicts_a_failing_request(astro):
+    # failure isolation: a request whose particles leave the server's max_extent (a system 100
+    # times wider than the others, all arithmetic finite) is evicted with status FAILED after
+    # its first step, while the others run to completion bit-identical to running alone
+    say("server evicts a failing request")
+    P = astro.params_physics(21)
+    ok1, ok2 = astro.ic_plummer(3000, 71), astro.ic_plummer(2000, 72)
+    wide = astro.ic_plummer(1500, 73)
+    wide[0]["position"] *= np.float32(100.0)
+    reqs = [(1, 0, *ok1, 6), (2, 0, *wide, 6), (3, 1, *ok2, 4)]
+    mix, recs = run_server(astro, P, reqs, 100000, 8, max_extent=50.0)
+    d = {r["id"]: r for r in recs}
+    assert d[2]["status"] == astro.REQUEST_FAILED and 1 <= d[2]["steps_done"] < 6
+    assert d[1]["status"] == d[3]["status"] == astro.REQUEST_DONE
+    assert d[1]["steps_done"] == 6 and d[3]["steps_done"] == 4
+    for rid, (p, v, c), steps in ((1, ok1, 6), (3, ok2, 4)):
+        alone, _ = run_server(astro, P, [(rid, 0, p, v, c, steps)], 100000, 8, max_extent=50.0)
+        assert same_bytes(mix[rid][0], alone[rid][0]) and same_bytes(mix[rid][1], alone[rid][1])
+    # non-finite input never reaches the device: refused at submit
+    bad = astro.ic_plummer(100, 74)
+    bad[0]["position"][5, 1] = np.nan
+    srv = astro.Server(P, 100000, 8)
+    with pytest.raises(astro.AstroError) as e:
+        srv.submit(9, *bad, 3)
+    assert e.value.code == -1  # ASTRO_ERR_ARG
+    srv.close()
+
+
+def test_server_sheds_load(astro):
+    # with max_wait_s set, a request whose estimated wait exceeds it is refused (ASTRO_ERR_BUSY)
+    # once the server has measured its throughput; the work already admitted completes
+    say("server sheds load")
+    P = astro.params_physics(21)
+    srv = astro.Server(P, 200000, 8, max_wait_s=0.05)
+    first = astro.ic_plummer(20000, 81)
+    srv.submit(1, *first, 40)
+    for _ in range(8):
+        srv.tick(1e-3)
+    srv.sync()  # the throughput is measured at the settles
+    big = astro.ic_plummer(150000, 82)
+    with pytest.raises(astro.AstroError) as e:
+        srv.submit(2, *big, 1000)  # far more than 0.05 s of work
+    assert e.value.code == -7  # ASTRO_ERR_BUSY
+    while srv.counters()["active"] or srv.counters()["queued"]:
+        srv.tick(1e-3)
+    recs = srv.poll()
+    assert [r["id"] for r in recs] == [1] and recs[0]["status"] == astro.REQUEST_DONE
+    srv.close()
 
 
 def test_server_admission_respects_budget(astro):
