@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "ctx.hpp"
 
 using namespace astro;
@@ -1052,18 +1054,38 @@ static int ensure_beside(astro_ctx* c) {
     return ASTRO_OK;
 }
 
+// roctx ranges around the host-side enqueue of each stage of a step (Renderer.swift's debug
+// groups, e.g. "Layer 0: Leaf Nodes", Renderer.swift:512,531,716,753), under the stage names
+// of astro_step_timed; rocprofv3 --marker-trace shows them (for a captured step, at capture).
+struct StageRange {
+    explicit StageRange(const char* name) { roctxRangePushA(name); }
+    ~StageRange() { roctxRangePop(); }
+    StageRange(const StageRange&) = delete;
+    StageRange& operator=(const StageRange&) = delete;
+};
+
 static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     int rc;
     const bool emit = c->emit_mode == ASTRO_EMIT_EVERY_BUILD;
+    StageRange step_range("astro.step");
     if (ev) record_stage(c, ev[0]);
-    if ((rc = enqueue_bbox_morton(c))) return rc;
+    {
+        StageRange r("astro.bbox_morton");
+        if ((rc = enqueue_bbox_morton(c))) return rc;
+    }
     if (ev) record_stage(c, ev[1]);
-    if ((rc = enqueue_sort(c))) return rc;
+    {
+        StageRange r("astro.sort");
+        if ((rc = enqueue_sort(c))) return rc;
+    }
     if (ev) record_stage(c, ev[2]);
     // the OctreeNode array only reads the compact tree: written beside the traversal, which
     // is bound by instruction issue and the vector memory pipeline, not by HBM
     const bool beside = emit && c->emit_beside && c->beside;
-    if ((rc = enqueue_layers(c, emit && !beside))) return rc;
+    {
+        StageRange r("astro.build");
+        if ((rc = enqueue_layers(c, emit && !beside))) return rc;
+    }
     if (beside) {
         HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->beside, c->ev_fork, 0));
@@ -1071,8 +1093,12 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
         HIPCHK(c, hipEventRecord(c->ev_join, c->beside));
     }
     if (ev) record_stage(c, ev[3]);
-    if ((rc = enqueue_forces(c, nullptr))) return rc;
+    {
+        StageRange r("astro.force");
+        if ((rc = enqueue_forces(c, nullptr))) return rc;
+    }
     if (ev) record_stage(c, ev[4]);
+    StageRange update_range("astro.update");
     // Renderer.swift:537-540: barnesHut, lightingPass, updateSpheres
     if (c->params.lighting_stars) enqueue_lighting(c, c->params.lighting_stars);
     launch_update(c->pos, c->vel, c->force, c->type, c->ns, dt, c->params, c->stream,
@@ -1119,6 +1145,7 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
         c->graph_valid = true;
         c->graph_dt = dt;
     }
+    StageRange r("astro.step (graph replays)");
     for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
     // each replay built a tree whose OctreeNode array is left for the next read
     if (nsteps) c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;

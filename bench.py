@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=30,
                     help="oracle steps timed for the CPU baseline (about 10 s on 16 threads)")
     ap.add_argument("--cpu-steps-matched", type=int, default=12)
+    ap.add_argument("--cpu-steps-1thread", type=int, default=2,
+                    help="oracle steps timed on one host thread (about 10 s)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (<= 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
@@ -606,6 +608,13 @@ def main():
                                          f"build single-threaded as transcribed, force loop on "
                                          f"{threads} OpenMP threads"}
         out["gpu_vs_cpu"] = value / cv
+        progress("CPU baseline, one thread")
+        c1, el1 = cpu_baseline(pos, vel, col, a.dt, a.cpu_steps_1thread, 1)
+        out["cpu_baseline"]["one_thread"] = {
+            "value": c1, "cores": 1,
+            "sample": f"{a.cpu_steps_1thread} full oracle steps of the same system on one "
+                      f"thread after one warm-up step ({el1:.1f} s)"}
+        out["gpu_vs_cpu_one_thread"] = value / c1
         if "accuracy" in out:
             cq, elq = cpu_baseline(pos, vel, col, a.dt, a.cpu_steps_matched, threads,
                                    theta=a.matched_theta, multipole=2)
