@@ -51,6 +51,9 @@ def parse():
                          "0.05 grid whose RMS force error on this workload stays <= 1e-3 "
                          "(0.55: 7.2e-4; 0.6: 1.02e-3)")
     ap.add_argument("--force-block", type=int, default=0, help="0: autotune")
+    ap.add_argument("--top-cache", type=int, default=1,
+                    help="1: the traversal reads the tree's top layers from LDS (blocks of "
+                         "512/1024); 0: every node from device memory (for A/B profiles)")
     ap.add_argument("--xcd-chunk", type=int, default=-1, help="-1: autotune")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -454,6 +457,7 @@ def main():
 
     ctx.build_tree()
     ctx.check()
+    ctx.set_force_top_cache(a.top_cache)
     if a.force_block and a.xcd_chunk >= 0:
         tuned_block, tuned_chunk = a.force_block, a.xcd_chunk
         ctx.set_force_block(tuned_block)
@@ -556,11 +560,12 @@ def main():
                      "traffic": traffic, "kernel_ms": force_ms,
                      "valu_issue_frac": valu_frac,
                      "valu_issue_what": "SQ_INSTS_VALU x 4 cycles per wave64 instruction over "
-                                        "1024 SIMDs at 2.4 GHz, against the kernel's duration "
-                                        "(rocprofv3, profiles/)",
-                     "limiter": "vector memory pipeline: texture address unit busy 86% of the "
-                                "kernel's cycles with the walk's divergent node loads, beside "
-                                "VALU issue (profiles/force_memory_pipeline_two_disks_500k.md)",
+                                        "1024 SIMDs and the kernel's busy cycles "
+                                        "(GRBM_GUI_ACTIVE per XCD; rocprofv3, profiles/)",
+                     "limiter": "VALU issue (89% of the SIMD cycles) with the top tree "
+                                "layers read from LDS; the texture address unit, busy 66%, "
+                                "bound it before (83%) "
+                                "(profiles/force_memory_pipeline_two_disks_500k.md)",
                      "interactions_per_step": inter, "flop_per_interaction": FLOP_PER_INTERACTION,
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
         "stages": stages,
