@@ -9,4 +9,4 @@ from .structs import (AstroParams, COLOR_TYPE, OCTREE_NODE, POSITION_MASS, TREE_
 from .capi import (AstroError, Context, Server, load, device_count, params_compat,  # noqa
                    params_physics, ic_plummer, ic_two_disks, ic_plummer_background,
                    ic_batch_sizes, snapshot_info, EMIT_NEVER, EMIT_EVERY_BUILD,
-                   EMIT_ON_READ, REQUEST_DONE, REQUEST_FAILED)
+                   EMIT_ON_READ, REQUEST_DONE, REQUEST_FAILED, SORT_RADIX, SORT_SEEDED)

@@ -119,6 +119,8 @@ SIGNATURES = [
     ("astro_set_force_top_cache", _I, [_P, _I]),
     ("astro_scan_noalloc", _I, [_P, _P, _U32, _P]),
     ("astro_set_wide_offsets", _I, [_P, _I]),
+    ("astro_set_sort_mode", _I, [_P, _I]),
+    ("astro_sort_info", _I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32),
                             ctypes.POINTER(_F)]),
     ("astro_diagnostics", _I, [_P, _P]),
@@ -242,6 +244,8 @@ def ic_batch_sizes(n_systems, n_min, n_max, seed):
 
 # When the OctreeNode array is written (astro.h ASTRO_EMIT_*)
 EMIT_NEVER, EMIT_EVERY_BUILD, EMIT_ON_READ = 0, 1, 2
+# How steps and builds sort the keys (astro.h ASTRO_SORT_*)
+SORT_RADIX, SORT_SEEDED = 0, 1
 
 
 def _emit_mode(m):
@@ -468,6 +472,17 @@ class Context:
 
     def set_wide_offsets(self, on):
         self._chk(self.lib.astro_set_wide_offsets(self.h, 1 if on else 0))
+
+    def set_sort_mode(self, mode):
+        """SORT_RADIX (from scratch every build) or SORT_SEEDED (from the previous order)."""
+        self._chk(self.lib.astro_set_sort_mode(self.h, int(mode)))
+
+    def sort_info(self):
+        """(seeded, overflow) of the last sort: whether it was the seeded one, and its sources
+        that did not fit in their bucket's slots."""
+        seeded, ovf = ctypes.c_int(), ctypes.c_uint32()
+        self._chk(self.lib.astro_sort_info(self.h, ctypes.byref(seeded), ctypes.byref(ovf)))
+        return bool(seeded.value), ovf.value
 
     def autotune(self, reps=3):
         """Picks the traversal block size and XCD chunk; returns (block, chunk, ms)."""

@@ -60,6 +60,7 @@ static void invalidate_graph(astro_ctx* c) {
 
 static int refresh_layers(astro_ctx* c) {
     c->tree_built = false;  // the layer table may change: build before traversing again
+    c->order_valid = false;  // the sources may change: the next sort starts from scratch
     std::vector<uint32_t> sizes;
     if (c->nsys <= 1) sizes.push_back(c->nt);
     else
@@ -133,7 +134,8 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->tpos, c->tlink, c->onodes, c->layer_counts, c->layer_counts_prev,
                     c->leaf_of_sorted, c->pnext, c->cubes, c->minmax, c->cube_counter, c->plevel, c->fused_buf,
                     c->err_word, c->stats, c->scratch,
-                    c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom};
+                    c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom, c->order,
+                    c->rs_keys, c->rs_idx, c->spl_keys, c->spl_idx, c->rs_count, c->spl_cube};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -239,6 +241,16 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->d_sys_off, c->cfg.max_systems + 1ull);
     e = e ? e : dalloc(c->device_bytes, &c->phi, NP);
     e = e ? e : dalloc(c->device_bytes, &c->diag_partial, (uint64_t)c->cfg.max_systems * kDiagChunks * 9);
+    // the seeded sort: previous order, bucket slots (kResortCap per bucket), splitters, counts
+    // + overflow count
+    c->rs_slots = resort_slots(NP);
+    e = e ? e : dalloc(c->device_bytes, &c->order, NP);
+    e = e ? e : dalloc(c->device_bytes, &c->rs_keys, c->rs_slots);
+    e = e ? e : dalloc(c->device_bytes, &c->rs_idx, c->rs_slots);
+    e = e ? e : dalloc(c->device_bytes, &c->spl_keys, kResortMaxBuckets);
+    e = e ? e : dalloc(c->device_bytes, &c->spl_idx, kResortMaxBuckets);
+    e = e ? e : dalloc(c->device_bytes, &c->rs_count, kResortMaxBuckets + 1ull);
+    e = e ? e : dalloc(c->device_bytes, &c->spl_cube, 1);
     if (e != hipSuccess) {
         astro_destroy(c);
         return ASTRO_ERR_NOMEM;
@@ -374,6 +386,7 @@ static int upload_particles(astro_ctx* c, const PositionMass* pos, const Velocit
 extern "C" int astro_upload(astro_ctx* c, const PositionMass* pos, const VelocityRadius* vel,
                             const ColorType* col, uint32_t n, uint32_t n_tree) {
     if (!c || !pos) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (n > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "too many particles");
     if (n_tree == 0 || n_tree > n) n_tree = n;
     if (n_tree > c->cfg.max_tree) return fail(c, ASTRO_ERR_CAPACITY, "too many tree sources");
@@ -394,6 +407,7 @@ extern "C" int astro_upload_systems(astro_ctx* c, const PositionMass* pos,
                                     const VelocityRadius* vel, const ColorType* col,
                                     const uint32_t* offsets, uint32_t n_systems) {
     if (!c || !pos || !offsets || n_systems == 0) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (n_systems > c->cfg.max_systems)
         return fail(c, ASTRO_ERR_CAPACITY, "more systems than the context was created for");
     if (offsets[0] != 0) return fail(c, ASTRO_ERR_ARG, "offsets[0] must be 0");
@@ -487,6 +501,7 @@ extern "C" int astro_download_forces(astro_ctx* c, float* force4) {
 
 extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t* bytes) {
     if (!c || !d_ptr) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     const uint64_t NP = c->cfg.max_particles, NT = c->cfg.max_tree;
     void* p = nullptr;
     uint64_t b = 0;
@@ -525,6 +540,7 @@ extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t*
 
 extern "C" int astro_memcpy_d2h(astro_ctx* c, void* host, const void* d_src, uint64_t bytes) {
     if (!c || (!host && bytes) || (!d_src && bytes)) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (bytes)
         HIPCHK(c, hipMemcpyAsync(host, d_src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -533,6 +549,7 @@ extern "C" int astro_memcpy_d2h(astro_ctx* c, void* host, const void* d_src, uin
 
 extern "C" int astro_memcpy_h2d(astro_ctx* c, void* d_dst, const void* host, uint64_t bytes) {
     if (!c || (!host && bytes) || (!d_dst && bytes)) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (bytes)
         HIPCHK(c, hipMemcpyAsync(d_dst, host, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -615,6 +632,7 @@ extern "C" int astro_radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_
                                 uint64_t* kout, uint32_t* iout, uint32_t estimated_len,
                                 const uint32_t* d_count) {
     if (!c || !kin || !iin || !kout || !iout || !d_count) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (estimated_len == 0) return ASTRO_OK;  // RadixSort.swift:106
     if (estimated_len > c->cfg.max_particles)
         return fail(c, ASTRO_ERR_CAPACITY, "estimated_len beyond the sorter's max length");
@@ -632,6 +650,7 @@ extern "C" int astro_radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_
 
 extern "C" int astro_scan(astro_ctx* c, const uint32_t* in, uint32_t* out, uint32_t len) {
     if (!c || (!in && len) || (!out && len)) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (len == 0) return ASTRO_OK;
     if (scan_tiles(len) > c->scratch_words)
         return fail(c, ASTRO_ERR_CAPACITY, "scan longer than the context supports");
@@ -644,6 +663,7 @@ extern "C" int astro_split_pass(astro_ctx* c, const uint64_t* kin, const uint32_
                                 uint64_t* kout, uint32_t* iout, uint32_t bit,
                                 uint32_t estimated_len, const uint32_t* d_count) {
     if (!c || !kin || !iin || !kout || !iout || !d_count || bit > 63) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (estimated_len == 0) return ASTRO_OK;
     if (estimated_len > c->cfg.max_particles)
         return fail(c, ASTRO_ERR_CAPACITY, "estimated_len beyond the sorter's max length");
@@ -676,6 +696,7 @@ static void enqueue_cube(astro_ctx* c, uint32_t count) {
 
 extern "C" int astro_generate_morton_codes(astro_ctx* c, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     if (count > c->nt) return fail(c, ASTRO_ERR_CAPACITY, "count beyond the tree sources");
     enqueue_cube(c, count);
@@ -687,6 +708,7 @@ extern "C" int astro_generate_morton_codes(astro_ctx* c, uint32_t count) {
 
 extern "C" int astro_fill_indices(astro_ctx* c, uint32_t* d_buf, uint32_t start, uint32_t end) {
     if (!c || !d_buf) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     launch_iota(d_buf, start, end, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -695,6 +717,7 @@ extern "C" int astro_fill_indices(astro_ctx* c, uint32_t* d_buf, uint32_t start,
 extern "C" int astro_clear_buffer(astro_ctx* c, void* d_buf, uint32_t count,
                                   uint32_t elem_bytes) {
     if (!c || !d_buf) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (elem_bytes == 8)
         launch_fill_u64(static_cast<uint64_t*>(d_buf), ASTRO_INVALID_MORTON_CODE, count,
                         c->stream);
@@ -720,6 +743,7 @@ extern "C" int astro_clear_octree_node_buffer(astro_ctx* c, OctreeNode* d_nodes,
 
 extern "C" int astro_count_unique_morton_codes(astro_ctx* c, int aggregate, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (count > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "count too large");
     if (count == 0) return ASTRO_OK;
     hipStream_t s = c->stream;
@@ -741,6 +765,7 @@ static uint32_t fp64_from(const astro_ctx* c) {
 
 extern "C" int astro_aggregate_leaf_nodes(astro_ctx* c, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     if (count > c->nt) return fail(c, ASTRO_ERR_CAPACITY, "count beyond the tree sources");
     launch_aggregate_leaf(c->sorted_codes, c->sorted_idx, c->unique_idx, c->pos, c->parent_count,
@@ -755,6 +780,7 @@ extern "C" int astro_aggregate_leaf_nodes(astro_ctx* c, uint32_t count) {
 extern "C" int astro_aggregate_internal_nodes(astro_ctx* c, uint32_t layer, uint32_t in_off,
                                               uint32_t out_off, uint32_t node_count) {
     if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (layer == 0 || layer >= c->layers) return fail(c, ASTRO_ERR_ARG, "layer out of range");
     if ((uint64_t)out_off + node_count > c->node_capacity)
         return fail(c, ASTRO_ERR_CAPACITY, "output range beyond the node buffer");
@@ -772,6 +798,7 @@ extern "C" int astro_aggregate_internal_nodes(astro_ctx* c, uint32_t layer, uint
 // reference; it is the identity.
 extern "C" int astro_build_internal_nodes(astro_ctx* c, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     for (uint32_t l = 1; l < c->layers; l++) {
         int rc;
@@ -839,6 +866,7 @@ static ForceArgs force_args(astro_ctx* c) {
 
 extern "C" int astro_barnes_hut(astro_ctx* c) {
     if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     if (c->nsys > 1) return fail(c, ASTRO_ERR_STATE, "stage traversal is single-system");
     if (c->params.multipole != 0)
@@ -902,18 +930,67 @@ static TreeArgs tree_args(astro_ctx* c) {
     return a;
 }
 
+// The seeded sort applies: one system, within its bucket limit.
+static bool seeded_sort_applies(const astro_ctx* c) {
+    const uint64_t nb = resort_buckets(c->nt);
+    return c->sort_mode == ASTRO_SORT_SEEDED && c->nsys <= 1 && c->nt > 0 && nb > 0 &&
+           nb * kResortCap <= c->rs_slots && c->nt <= c->cfg.max_particles;
+}
+// This build's sort is the seeded one: it applies and the previous order is there.
+static bool seeded_sort_now(const astro_ctx* c) { return seeded_sort_applies(c) && c->order_valid; }
+
+// The seeded sort computes the keys itself: only the cube here.
 static int enqueue_bbox_morton(astro_ctx* c) {
     enqueue_cube(c, c->nt);
-    launch_morton(c->pos, c->nt, c->nsys > 1 ? c->sys_of : nullptr, c->cubes, c->params,
-                  c->codes, c->unsorted_codes, c->unsorted_idx, c->stream);
+    if (!seeded_sort_now(c))
+        launch_morton(c->pos, c->nt, c->nsys > 1 ? c->sys_of : nullptr, c->cubes, c->params,
+                      c->codes, c->unsorted_codes, c->unsorted_idx, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
 
+static ResortArgs resort_args(astro_ctx* c) {
+    ResortArgs r{};
+    r.pos = c->pos;
+    r.cubes = c->cubes;
+    r.params = c->params;
+    r.order = c->order;
+    r.n = c->nt;
+    r.rk = c->rs_keys;
+    r.ri = c->rs_idx;
+    r.spl_k = c->spl_keys;
+    r.spl_i = c->spl_idx;
+    r.spl_cube = c->spl_cube;
+    r.cnt = c->rs_count;
+    r.ovf = c->rs_count + kResortMaxBuckets;
+    r.ok = c->tmp_keys_a;  // the overflow list (n entries at most)
+    r.oi = c->tmp_idx_a;
+    r.ob = c->tmp_idx_b;
+    r.out_k = c->sorted_codes;
+    r.out_i = c->sorted_idx;
+    return r;
+}
+
 static int enqueue_sort(astro_ctx* c) {
+    if (seeded_sort_now(c)) {
+        launch_resort(resort_args(c), c->stream);
+        c->last_sort_seeded = true;
+        c->keys_pending = true;  // the per-source key arrays: written on read
+        LAUNCHCHK(c);
+        return ASTRO_OK;
+    }
     RadixPlan plan = key_plan(3 * c->params.bits_per_axis, c->nsys);
-    return radix_sort(c, c->unsorted_codes, c->unsorted_idx, c->sorted_codes, c->sorted_idx,
-                      c->nt, c->dev_consts + 1, plan, c->nsys > 1 ? c->sys_of : nullptr);
+    if (int rc = radix_sort(c, c->unsorted_codes, c->unsorted_idx, c->sorted_codes, c->sorted_idx,
+                            c->nt, c->dev_consts + 1, plan, c->nsys > 1 ? c->sys_of : nullptr))
+        return rc;
+    c->last_sort_seeded = false;
+    c->keys_pending = false;
+    if (seeded_sort_applies(c)) {  // the order and splitters the next sort starts from
+        launch_resort_seed(c->sorted_codes, c->sorted_idx, resort_args(c), c->stream);
+        c->order_valid = true;
+        LAUNCHCHK(c);
+    }
+    return ASTRO_OK;
 }
 
 // The OctreeNode array of the compact tree (and the emptied stale slots).
@@ -1016,6 +1093,18 @@ int materialize_onodes(astro_ctx* c) {
         c->onodes_dirty = false;
     }
     enqueue_emit(c, c->stream);
+    LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// On read: the per-source key arrays (MORTON_CODES, UNSORTED_CODES, UNSORTED_INDICES) of the
+// last seeded sort, from its sorted keys and indices.
+int materialize_keys(astro_ctx* c) {
+    if (!c->keys_pending) return ASTRO_OK;
+    c->keys_pending = false;
+    HIPCHK(c, hipSetDevice(c->device));
+    launch_resort_keys(c->sorted_codes, c->sorted_idx, c->nt, c->codes, c->unsorted_codes,
+                       c->unsorted_idx, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
 }
@@ -1128,6 +1217,14 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
             if (int rc = enqueue_step(c, dt)) return rc;
         return ASTRO_OK;
     }
+    if (nsteps && seeded_sort_applies(c) && !c->order_valid) {
+        // no previous order yet: one step outside the graph sorts from scratch, so that the
+        // captured step holds the seeded sort
+        if (int rc = enqueue_step(c, dt)) return rc;
+        c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
+        nsteps--;
+        invalidate_graph(c);
+    }
     if (nsteps && (!c->graph_valid || c->graph_dt != dt)) {
         invalidate_graph(c);
         hipGraph_t g = nullptr;
@@ -1147,8 +1244,12 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
     }
     StageRange r("astro.step (graph replays)");
     for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
-    // each replay built a tree whose OctreeNode array is left for the next read
-    if (nsteps) c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
+    // each replay built a tree whose OctreeNode array (and, after a seeded sort, key arrays) is
+    // left for the next read
+    if (nsteps) {
+        c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
+        c->keys_pending = c->last_sort_seeded;
+    }
     return ASTRO_OK;
 }
 
@@ -1235,6 +1336,31 @@ extern "C" int astro_set_force_top_cache(astro_ctx* c, int on) {
     if (!c) return ASTRO_ERR_ARG;
     c->force_top = on != 0;
     invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_sort_mode(astro_ctx* c, int mode) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_keys(c)) return rc;
+    if (mode != ASTRO_SORT_RADIX && mode != ASTRO_SORT_SEEDED)
+        return fail(c, ASTRO_ERR_ARG, "sort mode must be 0 (radix) or 1 (seeded)");
+    c->sort_mode = (uint32_t)mode;
+    c->order_valid = false;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_sort_info(astro_ctx* c, int* seeded, uint32_t* overflow) {
+    if (!c) return ASTRO_ERR_ARG;
+    uint32_t o = 0;
+    if (c->last_sort_seeded) {
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipMemcpyAsync(&o, c->rs_count + kResortMaxBuckets, 4, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (seeded) *seeded = c->last_sort_seeded ? 1 : 0;
+    if (overflow) *overflow = o;
     return ASTRO_OK;
 }
 

@@ -68,6 +68,21 @@ struct astro_ctx {
     float guard_extent = 0.0f;         // |coordinate| bound of the guard (0: finiteness only)
     bool force_top = true;             // traversal reads the top layers from LDS (blocks >= 512)
     bool force_wide = false;           // 64-bit node offsets and unpacked moment rows on any tree
+    // the step's sort seeded by the previous order (one system): its previous order (a
+    // permutation of 0..nt-1 written by the last sort of these sources), bucket slots,
+    // splitters (from the last sort's rows), bucket counts (+ the overflow count)
+    uint32_t sort_mode = ASTRO_SORT_SEEDED;
+    bool order_valid = false;          // `order` holds the last sort's order of the current sources
+    bool last_sort_seeded = false;     // the last enqueued sort was the seeded one
+    bool keys_pending = false;         // its per-source key arrays are not written yet
+    uint32_t* order = nullptr;
+    uint64_t* rs_keys = nullptr;
+    uint32_t* rs_idx = nullptr;
+    uint64_t* spl_keys = nullptr;
+    uint32_t* spl_idx = nullptr;
+    uint32_t* rs_count = nullptr;
+    float4* spl_cube = nullptr;        // the cube of the splitters' keys
+    uint64_t rs_slots = 0;             // rows of rs_keys / rs_idx
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation
     uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
     uint64_t scratch_words = 0;
@@ -122,3 +137,5 @@ int astro_set_counts(astro_ctx* c, uint32_t ns, uint32_t nt);
 int astro_enqueue_step(astro_ctx* c, float dt);
 // On read: writes the pending OctreeNode array of the last build (no-op otherwise).
 int materialize_onodes(astro_ctx* c);
+// On read: writes the per-source key arrays of the last seeded sort (no-op otherwise).
+int materialize_keys(astro_ctx* c);

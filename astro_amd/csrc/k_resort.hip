@@ -1,0 +1,487 @@
+// k_resort.hip -- the step's key sort seeded by the previous step's order (gfx950).
+//
+// Between two steps the particles move a little, so the order the last sort produced is nearly
+// the new one.  This sort gives the radix sort's result -- ascending (key, source index), the
+// stable argsort of the keys -- in three launches instead of three per 8-bit digit, and computes
+// the keys itself (the key kernel does not run):
+//  * k_resort_sample (one block): evenly spaced entries of the previous order give samples
+//    (key, index), sorted in one block; every `over`-th becomes a splitter, so that the buckets
+//    between consecutive splitters hold kResortMean sources or fewer on average;
+//  * k_resort_place: every source, visited in the previous order, gets its key, finds its bucket
+//    (the bucket its previous position suggests, else a galloping search), takes a slot in the
+//    bucket with one atomic per run of equal buckets in its wave, and is stored in the bucket's
+//    kResortCap slots (past them, in an overflow list);
+//  * k_resort_bucket (one block per bucket): the bucket's offset is the sum of the counts before
+//    it; its sources are sorted by (key, index) and written to their final positions, and as the
+//    next sort's previous order.  A bucket beyond kResortCap gathers its overflow entries at its
+//    final positions and sorts them there, in device memory (slow, but only for orders far from
+//    the previous one).
+// The block sorts are bitonic networks over rows held two per thread: steps whose partner rows
+// lie in the same wave exchange them with lane shuffles, the others (10 of the 66 steps of 2,048
+// rows) through LDS with one barrier.  Splitters and comparisons use (key, index), a total order
+// of the sources: equal keys do not pile up in one bucket, and the result does not depend on the
+// order in which the atomics handed out slots.  Any permutation as previous order gives the same
+// result; a close one is fast.  The per-source key arrays the key kernel would have written are
+// rebuilt from the sorted keys by k_resort_keys when a caller reads them.
+#include "launch.hpp"
+#include "morton.hpp"
+
+namespace astro {
+
+__device__ __forceinline__ bool rs_less(uint64_t ka, uint32_t ia, uint64_t kb, uint32_t ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+// The bitonic network over rows 0..m-1 padded to P = 2^ceil(log2 m), in the form whose every
+// comparator (i, j), i < j, puts the smaller row at i (the first step of each merge pairs
+// mirrored rows, i with i ^ (k - 1)).  Rows m..P-1 then act as +infinity and never move, so the
+// comparators that reach them are skipped.  cx(i, j) compares and exchanges; bar() separates
+// the steps.  m is the same for the whole block.
+template <uint32_t NT, class CX, class BAR>
+__device__ inline void bitonic_rows(uint32_t m, CX cx, BAR bar) {
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+        for (uint32_t d = k >> 1; d >= 1; d >>= 1) {
+            for (uint32_t q = threadIdx.x; q < (P >> 1); q += NT) {
+                const uint32_t i = ((q & ~(d - 1)) << 1) | (q & (d - 1));  // bit log2(d) clear
+                const uint32_t j = d == (k >> 1) ? i ^ (k - 1) : i | d;
+                if (j < m) cx(i, j);
+            }
+            bar();
+        }
+    }
+}
+
+// One row of a block sort: the key and the source index.  kRowMax sorts after every source
+// (keys hold at most 63 bits).
+struct SortRow {
+    uint64_t k;
+    uint32_t i;
+};
+__device__ __forceinline__ bool row_less(const SortRow& a, const SortRow& b) {
+    return rs_less(a.k, a.i, b.k, b.i);
+}
+__device__ __forceinline__ SortRow row_max() { return SortRow{~0ull, ~0u}; }
+// The row of the lane whose byte address (lane * 4) is addr.
+__device__ __forceinline__ SortRow bperm_row(uint32_t addr, const SortRow& v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)(uint32_t)v.k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)(uint32_t)(v.k >> 32));
+    const uint32_t ix = (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)v.i);
+    return SortRow{((uint64_t)hi << 32) | lo, ix};
+}
+// r = take ? o : r, as selects.
+__device__ __forceinline__ void row_take(SortRow& r, const SortRow& o, bool take) {
+    r.k = take ? o.k : r.k;
+    r.i = take ? o.i : r.i;
+}
+
+// The same network over rows 0..P-1 (P a power of two, at most 2 NT) held two per thread:
+// thread t holds rows 2t (r[0]) and 2t + 1 (r[1]).  Step (k, d) pairs row j with j ^ mask (mask
+// k - 1 for the first step of a merge, else d), the lower row keeping the smaller: mask 1 pairs
+// a thread's own rows; otherwise the partner thread is t ^ (mask >> 1), reached by lane shuffles
+// inside the wave, or through LDS rows xk/xi (4 NT each: two alternating halves, so one barrier
+// per step) beyond it.  Waves whose rows all lie at or past P sit the steps out.
+// Inlined with a constant P the steps unroll and their masks, shuffle addresses and slot
+// pairings become constants.
+template <uint32_t NT>
+__device__ __forceinline__ void bitonic_pairs(SortRow r[2], uint32_t P, uint64_t* __restrict__ xk,
+                                              uint32_t* __restrict__ xi) {
+    const uint32_t t = threadIdx.x;
+    const bool live = 2u * (t & ~63u) < P;  // wave-uniform
+    uint32_t half = 0;
+#pragma unroll
+    for (uint32_t k = 2; k <= P; k <<= 1) {
+#pragma unroll
+        for (uint32_t d = k >> 1; d >= 1; d >>= 1) {
+            const uint32_t mask = d == (k >> 1) ? k - 1 : d;
+            const uint32_t tm = mask >> 1;
+            if (tm == 0) {  // rows 2t and 2t + 1
+                if (live) {
+                    const SortRow r0 = r[0], r1 = r[1];
+                    const bool sw = row_less(r1, r0);
+                    row_take(r[0], r1, sw);
+                    row_take(r[1], r0, sw);
+                }
+                continue;
+            }
+            SortRow q0, q1;  // the partner thread's rows
+            if (tm >= 64) {
+                uint64_t* hk = xk + half * 2 * NT;
+                uint32_t* hi = xi + half * 2 * NT;
+                if (live) {
+                    hk[2 * t] = r[0].k;
+                    hk[2 * t + 1] = r[1].k;
+                    hi[2 * t] = r[0].i;
+                    hi[2 * t + 1] = r[1].i;
+                }
+                lds_barrier();
+                const uint32_t pt = t ^ tm;
+                if (live) {
+                    q0 = SortRow{hk[2 * pt], hi[2 * pt]};
+                    q1 = SortRow{hk[2 * pt + 1], hi[2 * pt + 1]};
+                }
+                half ^= 1u;
+            } else if (live) {
+                const uint32_t addr = ((t & 63u) ^ tm) << 2;
+                q0 = bperm_row(addr, r[0]);
+                q1 = bperm_row(addr, r[1]);
+            }
+            if (live) {
+                // an odd mask pairs row slot s with the partner's slot 1 - s; the lower row of a
+                // pair (bit d clear) keeps the partner's row when that is smaller, the upper one
+                // when it is not
+                const bool swap_slots = mask & 1u;
+                const SortRow o0 = swap_slots ? q1 : q0, o1 = swap_slots ? q0 : q1;
+                const bool lower = ((2u * t) & d) == 0u;
+                row_take(r[0], o0, row_less(o0, r[0]) == lower);
+                row_take(r[1], o1, row_less(o1, r[1]) == lower);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t m) {
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    return P;
+}
+
+// --------------------------------------------------------------------------------- samples
+__device__ __forceinline__ SortRow sample_row(const ResortArgs& a, uint32_t j, uint32_t ns) {
+    const uint32_t p = a.order[(uint64_t)j * a.n / ns];
+    return SortRow{morton_key(a.pos[p], a.cubes, 0u, a.params), p};
+}
+
+// The last sort's splitters still divide this sort's keys evenly: the key grid has the same cell
+// size and moved by less than kResortShift cells along each axis (or does not depend on the
+// cube).
+constexpr float kResortShift = 1024.0f;
+__device__ inline bool splitters_current(const ResortArgs& a) {
+    if (a.params.quantizer == ASTRO_QUANT_OFFSET_WRAP) return true;
+    const float4 c = a.cubes[0], o = *a.spl_cube;
+    int e, eo;
+    frexpf(c.w, &e);
+    frexpf(o.w, &eo);
+    if (e != eo) return false;
+    const float scale = ldexpf(1.0f, 21 - (e - 1));
+    return fabsf(c.x - o.x) * scale < kResortShift && fabsf(c.y - o.y) * scale < kResortShift &&
+           fabsf(c.z - o.z) * scale < kResortShift;
+}
+
+__global__ __launch_bounds__(kResortThreads) void k_resort_sample(ResortArgs a, uint32_t nb,
+                                                                  uint32_t over) {
+    __shared__ uint64_t s_k[kResortMaxSamples];
+    __shared__ uint32_t s_i[kResortMaxSamples];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ns = nb * over;
+    // the counts and the overflow list of this sort start empty
+    for (uint32_t b = tid; b < nb; b += kResortThreads) a.cnt[b] = 0;
+    if (tid == 0) *a.ovf = 0;
+    if (splitters_current(a)) return;  // the last sort's splitters (block-uniform)
+    if (tid == 0) *a.spl_cube = a.cubes[0];
+    if (ns <= 2 * kResortThreads) {
+        SortRow r[2];
+        for (uint32_t s = 0; s < 2; s++) {
+            const uint32_t j = 2 * tid + s;
+            r[s] = j < ns ? sample_row(a, j, ns) : row_max();
+        }
+        bitonic_pairs<kResortThreads>(r, pow2_at_least(ns), s_k, s_i);
+        // splitter t is sample (t + 1) * over
+        for (uint32_t s = 0; s < 2; s++) {
+            const uint32_t j = 2 * tid + s;
+            if (j < ns && j > 0 && j % over == 0) {
+                a.spl_k[j / over - 1] = r[s].k;
+                a.spl_i[j / over - 1] = r[s].i;
+            }
+        }
+        return;
+    }
+    for (uint32_t j = tid; j < ns; j += kResortThreads) {
+        const SortRow v = sample_row(a, j, ns);
+        s_k[j] = v.k;
+        s_i[j] = v.i;
+    }
+    lds_barrier();
+    bitonic_rows<kResortThreads>(
+        ns,
+        [](uint32_t i, uint32_t j) {
+            const uint64_t ki = s_k[i], kj = s_k[j];
+            const uint32_t ii = s_i[i], ij = s_i[j];
+            if (rs_less(kj, ij, ki, ii)) {
+                s_k[i] = kj;
+                s_k[j] = ki;
+                s_i[i] = ij;
+                s_i[j] = ii;
+            }
+        },
+        [] { lds_barrier(); });
+    for (uint32_t t = tid; t + 1 < nb; t += kResortThreads) {
+        a.spl_k[t] = s_k[(t + 1) * over];
+        a.spl_i[t] = s_i[(t + 1) * over];
+    }
+}
+
+// ----------------------------------------------------------------------------------- place
+// Splitter t lies at or before source (key, p).
+__device__ __forceinline__ bool splitter_le(const ResortArgs& a, int t, uint64_t key, uint32_t p) {
+    const uint64_t sk = a.spl_k[t];
+    return sk < key || (sk == key && a.spl_i[t] <= p);
+}
+
+// The bucket of (key, p): the number of splitters (nb - 1, ascending) at or before it.  A
+// galloping search from the guess g: two loads when the guess is right.
+__device__ inline uint32_t find_bucket(const ResortArgs& a, uint64_t key, uint32_t p, uint32_t g,
+                                       uint32_t nb) {
+    const int ns = (int)nb - 1, gg = (int)g;
+    int lo, hi;  // splitter lo at or before (p), or lo = -1; splitter hi after, or hi = ns
+    if (gg > 0 && !splitter_le(a, gg - 1, key, p)) {
+        hi = gg - 1;
+        int step = 1;
+        lo = hi - step;
+        while (lo >= 0 && !splitter_le(a, lo, key, p)) {
+            hi = lo;
+            step <<= 1;
+            lo = hi - step;
+        }
+        if (lo < -1) lo = -1;
+    } else if (gg < ns && splitter_le(a, gg, key, p)) {
+        lo = gg;
+        int step = 1;
+        hi = lo + step;
+        while (hi < ns && splitter_le(a, hi, key, p)) {
+            lo = hi;
+            step <<= 1;
+            hi = lo + step;
+        }
+        if (hi > ns) hi = ns;
+    } else {
+        return g;
+    }
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (splitter_le(a, mid, key, p)) lo = mid;
+        else hi = mid;
+    }
+    return (uint32_t)hi;
+}
+
+__global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs a, uint32_t nb) {
+    const uint32_t i = blockIdx.x * kResortPlaceThreads + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const bool valid = i < a.n;
+    uint64_t key = 0;
+    uint32_t p = 0, b = 0;
+    if (valid) {
+        // the splitters around the guessed bucket load beside the source's own chain: the
+        // bucket of previous position i under the splitters of the previous order's rows
+        // (t + 1) n / nb
+        const uint32_t g = (uint32_t)(((uint64_t)(i + 1) * nb + a.n - 1) / a.n - 1);
+        SortRow s0 = SortRow{0ull, 0u}, s1 = row_max();
+        if (g > 0) s0 = SortRow{a.spl_k[g - 1], a.spl_i[g - 1]};
+        if (g + 1 < nb) s1 = SortRow{a.spl_k[g], a.spl_i[g]};
+        p = a.order[i];
+        key = morton_key(a.pos[p], a.cubes, 0u, a.params);
+        const SortRow x{key, p};
+        const bool at_or_after = g == 0 || !row_less(x, s0);
+        const bool before = g + 1 >= nb || row_less(x, s1);
+        b = at_or_after && before ? g : find_bucket(a, key, p, g, nb);
+    }
+    // runs of lanes with equal buckets (the valid lanes are a prefix of the wave)
+    const uint64_t act = __ballot(valid);
+    const uint32_t bup = __shfl_up(b, 1u, 64);
+    const bool head = valid && (lane == 0 || bup != b);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    const uint64_t own = heads & upto;            // nonzero on a valid lane: lane 0 is a head
+    const uint32_t s = valid ? 63u - (uint32_t)__builtin_clzll(own) : 0u;
+    const uint64_t later = heads & ~upto;
+    const uint32_t e = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)__popcll(act);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&a.cnt[b], e - lane);
+    base = __shfl(base, (int)s, 64);
+    if (!valid) return;
+    const uint32_t rank = base + (lane - s);
+    if (rank < kResortCap) {
+        const uint64_t slot = (uint64_t)b * kResortCap + rank;
+        a.rk[slot] = key;
+        a.ri[slot] = p;
+    } else {
+        const uint32_t o = atomicAdd(a.ovf, 1u);
+        a.ok[o] = key;
+        a.oi[o] = p;
+        a.ob[o] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------------- bucket
+// Splitter t of the next sort is the row at sorted position (t + 1) n / nb; row position q writes
+// the one it is, if any.
+__device__ __forceinline__ void emit_splitter(const ResortArgs& a, uint32_t nb, uint32_t q,
+                                              uint64_t k, uint32_t i) {
+    const uint64_t t1 = ((uint64_t)q * nb + a.n - 1) / a.n;  // the smallest t + 1 reaching q
+    if (t1 >= 1 && t1 < nb && t1 * a.n / nb == q) {
+        a.spl_k[t1 - 1] = k;
+        a.spl_i[t1 - 1] = i;
+    }
+}
+
+__global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortArgs a, uint32_t nb) {
+    constexpr uint32_t NT = kResortBucketThreads;
+    __shared__ uint64_t s_k[kResortCap];  // the bucket's rows (LDS sort), or exchange rows
+    __shared__ uint32_t s_i[kResortCap];
+    __shared__ uint32_t s_tmp[NT / 64 + 1];
+    __shared__ uint32_t s_fill;
+    static_assert(kResortCap >= 4 * NT && kResortRegRows == 2 * NT, "row layout");
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    const uint32_t m = a.cnt[b];
+    uint32_t part = 0;
+    for (uint32_t t = tid; t < b; t += NT) part += a.cnt[t];
+    uint32_t start;
+    block_exclusive_scan<NT>(part, s_tmp, &start);
+    if (b == 0 && tid == 0) *a.spl_cube = a.cubes[0];  // the next sort's splitters' cube
+    if (start > a.n || m > a.n - start) return;       // counts always add up to n
+    const uint64_t slots = (uint64_t)b * kResortCap;
+    if (m <= kResortRegRows) {
+        SortRow r[2];
+        for (uint32_t s = 0; s < 2; s++) {
+            const uint32_t j = 2 * tid + s;
+            r[s] = j < m ? SortRow{a.rk[slots + j], a.ri[slots + j]} : row_max();
+        }
+        if (m > kResortRegRows / 2) bitonic_pairs<NT>(r, kResortRegRows, s_k, s_i);
+        else bitonic_pairs<NT>(r, pow2_at_least(m), s_k, s_i);
+        for (uint32_t s = 0; s < 2; s++) {
+            const uint32_t j = 2 * tid + s;
+            if (j < m) {
+                a.out_k[start + j] = r[s].k;
+                a.out_i[start + j] = r[s].i;
+                a.order[start + j] = r[s].i;
+                emit_splitter(a, nb, start + j, r[s].k, r[s].i);
+            }
+        }
+        return;
+    }
+    if (m <= kResortCap) {
+        for (uint32_t j = tid; j < m; j += NT) {
+            s_k[j] = a.rk[slots + j];
+            s_i[j] = a.ri[slots + j];
+        }
+        lds_barrier();
+        bitonic_rows<NT>(
+            m,
+            [](uint32_t i, uint32_t j) {
+                const uint64_t ki = s_k[i], kj = s_k[j];
+                const uint32_t ii = s_i[i], ij = s_i[j];
+                if (rs_less(kj, ij, ki, ii)) {
+                    s_k[i] = kj;
+                    s_k[j] = ki;
+                    s_i[i] = ij;
+                    s_i[j] = ii;
+                }
+            },
+            [] { lds_barrier(); });
+        for (uint32_t j = tid; j < m; j += NT) {
+            a.out_k[start + j] = s_k[j];
+            a.out_i[start + j] = s_i[j];
+            a.order[start + j] = s_i[j];
+            emit_splitter(a, nb, start + j, s_k[j], s_i[j]);
+        }
+        return;
+    }
+    // past the slots: the bucket's sources are gathered at their final positions and sorted
+    // there (__syncthreads orders the block's device-memory accesses between the steps)
+    uint64_t* gk = a.out_k + start;
+    uint32_t* gi = a.out_i + start;
+    for (uint32_t j = tid; j < kResortCap; j += NT) {
+        gk[j] = a.rk[slots + j];
+        gi[j] = a.ri[slots + j];
+    }
+    if (tid == 0) s_fill = 0;
+    __syncthreads();
+    const uint32_t no = *a.ovf;
+    for (uint32_t o = tid; o < no; o += NT) {
+        if (a.ob[o] != b) continue;
+        const uint32_t j = kResortCap + atomicAdd(&s_fill, 1u);
+        if (j < m) {  // exactly m - kResortCap entries name this bucket
+            gk[j] = a.ok[o];
+            gi[j] = a.oi[o];
+        }
+    }
+    __syncthreads();
+    bitonic_rows<NT>(
+        m,
+        [gk, gi](uint32_t i, uint32_t j) {
+            const uint64_t ki = gk[i], kj = gk[j];
+            const uint32_t ii = gi[i], ij = gi[j];
+            if (rs_less(kj, ij, ki, ii)) {
+                gk[i] = kj;
+                gk[j] = ki;
+                gi[i] = ij;
+                gi[j] = ii;
+            }
+        },
+        [] { __syncthreads(); });
+    for (uint32_t j = tid; j < m; j += NT) {
+        a.order[start + j] = gi[j];
+        emit_splitter(a, nb, start + j, gk[j], gi[j]);
+    }
+}
+
+// After a radix sort: its order as the previous order, its rows at the splitter positions as
+// the splitters, and this build's cube as theirs.
+__global__ void k_resort_seed(const uint64_t* __restrict__ sk, const uint32_t* __restrict__ si,
+                              ResortArgs a, uint32_t nb) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.spl_cube = a.cubes[0];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += gridDim.x * blockDim.x) {
+        const uint32_t i = si[q];
+        a.order[q] = i;
+        emit_splitter(a, nb, q, sk[q], i);
+    }
+}
+void launch_resort_seed(const uint64_t* sk, const uint32_t* si, const ResortArgs& a,
+                        hipStream_t s) {
+    const uint32_t nb = resort_buckets(a.n);
+    if (!a.n || !nb) return;
+    uint32_t blocks = (a.n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_resort_seed, dim3(blocks), dim3(256), 0, s, sk, si, a, nb);
+}
+
+// The per-source key arrays of a seeded sort, from its sorted keys and indices: what the key
+// kernel writes (codes[p] = codes2[p] = key of p, idx[i] = i).
+__global__ void k_resort_keys(const uint64_t* __restrict__ sk, const uint32_t* __restrict__ si,
+                              uint32_t n, uint64_t* __restrict__ codes,
+                              uint64_t* __restrict__ codes2, uint32_t* __restrict__ idx) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = si[i];
+        const uint64_t k = sk[i];
+        codes[p] = k;
+        codes2[p] = k;
+        idx[i] = i;
+    }
+}
+void launch_resort_keys(const uint64_t* sk, const uint32_t* si, uint32_t n, uint64_t* codes,
+                        uint64_t* codes2, uint32_t* idx, hipStream_t s) {
+    if (!n) return;
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_resort_keys, dim3(blocks), dim3(256), 0, s, sk, si, n, codes, codes2, idx);
+}
+
+uint32_t resort_buckets(uint32_t n) {
+    const uint32_t nb = (uint32_t)(((uint64_t)n + kResortMean - 1) / kResortMean);
+    return nb <= kResortMaxBuckets ? nb : 0u;
+}
+
+void launch_resort(const ResortArgs& a, hipStream_t s) {
+    const uint32_t nb = resort_buckets(a.n);
+    if (!a.n || !nb) return;
+    uint32_t over = kResortMaxSamples / nb;  // samples per bucket
+    if (over > kResortOversample) over = kResortOversample;
+    hipLaunchKernelGGL(k_resort_sample, dim3(1), dim3(kResortThreads), 0, s, a, nb, over);
+    hipLaunchKernelGGL(k_resort_place, dim3((a.n + kResortPlaceThreads - 1) / kResortPlaceThreads),
+                       dim3(kResortPlaceThreads), 0, s, a, nb);
+    hipLaunchKernelGGL(k_resort_bucket, dim3(nb), dim3(kResortBucketThreads), 0, s, a, nb);
+}
+
+}  // namespace astro

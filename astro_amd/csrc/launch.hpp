@@ -57,6 +57,54 @@ void launch_radix_pass(const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
                        RadixPassDesc desc, const uint32_t* seg_of, uint32_t* work,
                        hipStream_t s);
 
+// ---- k_resort.hip -------------------------------------------------------------------------
+// The sort seeded by the previous order: same result as the radix sort of (key, index) pairs
+// (ascending key, ties by index), for any permutation `order` of 0..n-1.
+constexpr uint32_t kResortThreads = 1024;      // the sample block
+constexpr uint32_t kResortBucketThreads = 512; // bucket blocks
+constexpr uint32_t kResortPlaceThreads = 256;
+constexpr uint32_t kResortMean = 896;          // buckets = ceil(n / kResortMean)
+constexpr uint32_t kResortRegRows = 2 * kResortBucketThreads;  // sorted in registers up to this
+constexpr uint32_t kResortCap = 2048;          // slots per bucket (sorted in LDS up to this)
+constexpr uint32_t kResortMaxBuckets = 8192;
+constexpr uint32_t kResortMaxSamples = 8192;   // samples sorted in one block's LDS
+constexpr uint32_t kResortOversample = 2;      // samples per bucket (at most)
+struct ResortArgs {
+    const float4* pos;       // [n] source positions (one system)
+    const float4* cubes;     // [1] its cube
+    astro_params params;     // quantizer and key bits
+    uint32_t* order;         // [n] previous sorted order in, this sort's order out
+    uint32_t n;
+    uint64_t* rk;            // [buckets * kResortCap] bucket slots
+    uint32_t* ri;
+    uint64_t* spl_k;         // [kResortMaxBuckets] splitters (the bucket kernel leaves the next
+    uint32_t* spl_i;         //  sort's, from its own sorted rows)
+    float4* spl_cube;        // [1] the cube the splitters' keys were made with
+    uint32_t* cnt;           // [kResortMaxBuckets] sources per bucket
+    uint32_t* ovf;           // [1] overflow entries
+    uint64_t* ok;            // [n] overflow list: key, index, bucket
+    uint32_t* oi;
+    uint32_t* ob;
+    uint64_t* out_k;         // [n] sorted keys and indices
+    uint32_t* out_i;
+};
+// Buckets of a seeded sort of n sources; 0 when n is beyond it.
+uint32_t resort_buckets(uint32_t n);
+// Bucket slots (rows of rk / ri) a seeded sort of up to nmax sources needs.
+inline uint64_t resort_slots(uint32_t nmax) {
+    uint64_t nb = ((uint64_t)nmax + kResortMean - 1) / kResortMean;
+    if (nb > kResortMaxBuckets) nb = kResortMaxBuckets;
+    return (nb ? nb : 1) * kResortCap;
+}
+void launch_resort(const ResortArgs& a, hipStream_t s);
+// After a radix sort of the same sources: the previous order and the splitters of the next
+// seeded sort from its result (sk, si), and the cube they were made with.
+void launch_resort_seed(const uint64_t* sk, const uint32_t* si, const ResortArgs& a,
+                        hipStream_t s);
+// The key kernel's per-source outputs (codes, their copy, 0..n-1) from a sort's result.
+void launch_resort_keys(const uint64_t* sk, const uint32_t* si, uint32_t n, uint64_t* codes,
+                        uint64_t* codes2, uint32_t* idx, hipStream_t s);
+
 // ---- k_tree.hip ---------------------------------------------------------------------------
 struct TreeArgs {
     const float4* pos;          // particle positions + mass

@@ -55,6 +55,9 @@ def parse():
                     help="1: the traversal reads the tree's top layers from LDS (blocks of "
                          "512/1024); 0: every node from device memory (for A/B profiles)")
     ap.add_argument("--xcd-chunk", type=int, default=-1, help="-1: autotune")
+    ap.add_argument("--sort", choices=["seeded", "radix"], default="seeded",
+                    help="the step's key sort: seeded by the previous step's order (default) or "
+                         "a radix sort from scratch (for A/B profiles); the same order either way")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=4)
@@ -458,6 +461,7 @@ def main():
     ctx.build_tree()
     ctx.check()
     ctx.set_force_top_cache(a.top_cache)
+    ctx.set_sort_mode(astro_amd.SORT_SEEDED if a.sort == "seeded" else astro_amd.SORT_RADIX)
     if a.force_block and a.xcd_chunk >= 0:
         tuned_block, tuned_chunk = a.force_block, a.xcd_chunk
         ctx.set_force_block(tuned_block)

@@ -53,6 +53,14 @@ typedef struct {
  *               ASTRO_BUF_OCTREE_NODES or astro_barnes_hut (default). */
 enum { ASTRO_EMIT_NEVER = 0, ASTRO_EMIT_EVERY_BUILD = 1, ASTRO_EMIT_ON_READ = 2 };
 
+/* How a step or astro_build_tree sorts the keys (the result is the same either way: ascending
+ * key, ties by source index, RadixSort.swift:98-185):
+ *   RADIX   an LSD radix sort from scratch, three launches per 8-bit digit;
+ *   SEEDED  buckets between splitters sampled from the previous sort's order, each sorted in
+ *           LDS (default; one system of at most 7,340,032 sources, after a first radix sort of
+ *           the current sources; otherwise RADIX). */
+enum { ASTRO_SORT_RADIX = 0, ASTRO_SORT_SEEDED = 1 };
+
 /* Buffers reachable through astro_get_buffer (names follow Renderer.swift:56-70). */
 enum {
     ASTRO_BUF_POSITION_MASS = 0,   /* float4 {x,y,z,m} per particle                       */
@@ -208,6 +216,12 @@ int astro_set_force_top_cache(astro_ctx* ctx, int on);
  * offsets, no links in the moment rows), so that path can be compared with the oracle on small
  * trees.  Takes effect at the next build.  No counterpart in the reference. */
 int astro_set_wide_offsets(astro_ctx* ctx, int on);
+/* Key sort of steps and builds: ASTRO_SORT_RADIX or ASTRO_SORT_SEEDED (see above).  Replaces
+ * nothing in the reference, whose every build sorts from scratch. */
+int astro_set_sort_mode(astro_ctx* ctx, int mode);
+/* The last sort's form (1: seeded, 0: radix) and, for a seeded sort, its sources that did not fit
+ * in their bucket's slots (sorted on the slow path); synchronises the stream. */
+int astro_sort_info(astro_ctx* ctx, int* seeded, uint32_t* overflow);
 /* How target tiles are dealt to the 8 XCDs: 0 (default) gives each XCD one contiguous eighth
  * of the Morton curve; K > 0 deals runs of K consecutive tiles to the XCDs in turn. */
 int astro_set_force_xcd_chunk(astro_ctx* ctx, uint32_t chunk);
