@@ -249,7 +249,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->rs_idx, c->rs_slots);
     e = e ? e : dalloc(c->device_bytes, &c->spl_keys, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->spl_idx, kResortMaxBuckets);
-    e = e ? e : dalloc(c->device_bytes, &c->rs_count, kResortMaxBuckets + 1ull);
+    e = e ? e : dalloc(c->device_bytes, &c->rs_count, 2ull * kResortMaxBuckets + 1);
     e = e ? e : dalloc(c->device_bytes, &c->spl_cube, 1);
     if (e != hipSuccess) {
         astro_destroy(c);
@@ -961,8 +961,9 @@ static ResortArgs resort_args(astro_ctx* c) {
     r.spl_k = c->spl_keys;
     r.spl_i = c->spl_idx;
     r.spl_cube = c->spl_cube;
-    r.cnt = c->rs_count;
-    r.ovf = c->rs_count + kResortMaxBuckets;
+    r.moved_in = c->rs_count;
+    r.moved_out = c->rs_count + kResortMaxBuckets;
+    r.ovf = c->rs_count + 2 * kResortMaxBuckets;
     r.ok = c->tmp_keys_a;  // the overflow list (n entries at most)
     r.oi = c->tmp_idx_a;
     r.ob = c->tmp_idx_b;
@@ -1355,8 +1356,8 @@ extern "C" int astro_sort_info(astro_ctx* c, int* seeded, uint32_t* overflow) {
     uint32_t o = 0;
     if (c->last_sort_seeded) {
         HIPCHK(c, hipSetDevice(c->device));
-        HIPCHK(c, hipMemcpyAsync(&o, c->rs_count + kResortMaxBuckets, 4, hipMemcpyDeviceToHost,
-                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(&o, c->rs_count + 2 * kResortMaxBuckets, 4,
+                                 hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (seeded) *seeded = c->last_sort_seeded ? 1 : 0;

@@ -7,15 +7,17 @@
 //  * k_resort_sample (one block): evenly spaced entries of the previous order give samples
 //    (key, index), sorted in one block; every `over`-th becomes a splitter, so that the buckets
 //    between consecutive splitters hold kResortMean sources or fewer on average;
-//  * k_resort_place: every source, visited in the previous order, gets its key, finds its bucket
-//    (the bucket its previous position suggests, else a galloping search), takes a slot in the
-//    bucket with one atomic per run of equal buckets in its wave, and is stored in the bucket's
-//    kResortCap slots (past them, in an overflow list);
-//  * k_resort_bucket (one block per bucket): the bucket's offset is the sum of the counts before
-//    it; its sources are sorted by (key, index) and written to their final positions, and as the
-//    next sort's previous order.  A bucket beyond kResortCap gathers its overflow entries at its
-//    final positions and sorts them there, in device memory (slow, but only for orders far from
-//    the previous one).
+//  * k_resort_place: every source, visited in the previous order, gets its key and its bucket.
+//    Bucket g's first slots are the positions that held its sources last time; a source still
+//    in the bucket of its position is stored at that position's slot, with no atomic.  One that
+//    moved leaves a hole there, counts itself out of the old bucket and into the new one, and
+//    takes a slot past the new bucket's positions (past its kResortCap slots, in an overflow
+//    list);
+//  * k_resort_bucket (one block per bucket): the bucket's offset follows from the positions and
+//    the counts in and out before it; its slots are sorted by (key, index), holes last, and its
+//    sources written to their final positions, and as the next sort's previous order.  A bucket
+//    beyond kResortCap gathers its sources at their final positions and sorts them there, in
+//    device memory (slow, but only for orders far from the previous one).
 // The block sorts are bitonic networks over rows held two per thread: steps whose partner rows
 // lie in the same wave exchange them with lane shuffles, the others (10 of the 66 steps of 2,048
 // rows) through LDS with one barrier.  Splitters and comparisons use (key, index), a total order
@@ -82,11 +84,13 @@ __device__ __forceinline__ void row_take(SortRow& r, const SortRow& o, bool take
 // a thread's own rows; otherwise the partner thread is t ^ (mask >> 1), reached by lane shuffles
 // inside the wave, or through LDS rows xk/xi (4 NT each: two alternating halves, so one barrier
 // per step) beyond it.  Waves whose rows all lie at or past P sit the steps out.
-// Inlined with a constant P the steps unroll and their masks, shuffle addresses and slot
-// pairings become constants.
-template <uint32_t NT>
-__device__ __forceinline__ void bitonic_pairs(SortRow r[2], uint32_t P, uint64_t* __restrict__ xk,
+// PFIX: a fixed P (0: the argument), whose steps unroll with constant masks, shuffle addresses
+// and slot pairings.
+template <uint32_t NT, uint32_t PFIX = 0>
+__device__ __forceinline__ void bitonic_pairs(SortRow r[2], uint32_t P_arg,
+                                              uint64_t* __restrict__ xk,
                                               uint32_t* __restrict__ xi) {
+    const uint32_t P = PFIX ? PFIX : P_arg;
     const uint32_t t = threadIdx.x;
     const bool live = 2u * (t & ~63u) < P;  // wave-uniform
     uint32_t half = 0;
@@ -176,7 +180,10 @@ __global__ __launch_bounds__(kResortThreads) void k_resort_sample(ResortArgs a, 
     const uint32_t tid = threadIdx.x;
     const uint32_t ns = nb * over;
     // the counts and the overflow list of this sort start empty
-    for (uint32_t b = tid; b < nb; b += kResortThreads) a.cnt[b] = 0;
+    for (uint32_t b = tid; b < nb; b += kResortThreads) {
+        a.moved_in[b] = 0;
+        a.moved_out[b] = 0;
+    }
     if (tid == 0) *a.ovf = 0;
     if (splitters_current(a)) return;  // the last sort's splitters (block-uniform)
     if (tid == 0) *a.spl_cube = a.cubes[0];
@@ -266,46 +273,52 @@ __device__ inline uint32_t find_bucket(const ResortArgs& a, uint64_t key, uint32
     return (uint32_t)hi;
 }
 
+// Row c n / nb (rounded down) of the previous order, c = 0 .. nb: where the positions of
+// bucket c start.
+__device__ __forceinline__ uint32_t bucket_pos(const ResortArgs& a, uint32_t nb, uint32_t c) {
+    return (uint32_t)((uint64_t)c * a.n / nb);
+}
+
+// A source whose bucket is the one its previous position lies in (a "stayer") is stored at that
+// position's slot: bucket g's slots 0 .. S_g - 1 are its positions.  A source that changed
+// bucket leaves a hole (a row that sorts last) at its position's slot, counts itself out of
+// the old bucket and takes a slot past the positions of the new one with one atomic.
 __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs a, uint32_t nb) {
     const uint32_t i = blockIdx.x * kResortPlaceThreads + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63;
-    const bool valid = i < a.n;
-    uint64_t key = 0;
-    uint32_t p = 0, b = 0;
-    if (valid) {
-        // the splitters around the guessed bucket load beside the source's own chain: the
-        // bucket of previous position i under the splitters of the previous order's rows
-        // (t + 1) n / nb
-        const uint32_t g = (uint32_t)(((uint64_t)(i + 1) * nb + a.n - 1) / a.n - 1);
-        SortRow s0 = SortRow{0ull, 0u}, s1 = row_max();
-        if (g > 0) s0 = SortRow{a.spl_k[g - 1], a.spl_i[g - 1]};
-        if (g + 1 < nb) s1 = SortRow{a.spl_k[g], a.spl_i[g]};
-        p = a.order[i];
-        key = morton_key(a.pos[p], a.cubes, 0u, a.params);
-        const SortRow x{key, p};
-        const bool at_or_after = g == 0 || !row_less(x, s0);
-        const bool before = g + 1 >= nb || row_less(x, s1);
-        b = at_or_after && before ? g : find_bucket(a, key, p, g, nb);
+    // The guess: the bucket whose positions hold i, i.e. the number of splitter rows
+    // q(c) = c n / nb (c = 1 .. nb - 1) at or before i.  Buckets span at least n / nb >= 448
+    // positions (or there is one), so the block's 256 positions meet at most one q(c): the
+    // guess is g0 (the first position's) or g0 + 1 from q(g0 + 1) on.
+    const uint32_t i0 = blockIdx.x * kResortPlaceThreads;
+    const uint32_t g0 = (uint32_t)(((uint64_t)(i0 + 1) * nb + a.n - 1) / a.n - 1);
+    const uint32_t q0 = bucket_pos(a, nb, g0), q1 = bucket_pos(a, nb, g0 + 1);
+    if (i >= a.n) return;
+    const uint32_t g = i >= q1 ? g0 + 1 : g0;
+    const uint32_t qs = i >= q1 ? q1 : q0;
+    // the splitters around the guessed bucket load beside the source's own chain
+    SortRow s0 = SortRow{0ull, 0u}, s1 = row_max();
+    if (g > 0) s0 = SortRow{a.spl_k[g - 1], a.spl_i[g - 1]};
+    if (g + 1 < nb) s1 = SortRow{a.spl_k[g], a.spl_i[g]};
+    const uint32_t p = a.order[i];
+    const uint64_t key = morton_key(a.pos[p], a.cubes, 0u, a.params);
+    const SortRow x{key, p};
+    const bool at_or_after = g == 0 || !row_less(x, s0);
+    const bool before = g + 1 >= nb || row_less(x, s1);
+    const uint64_t stay = (uint64_t)g * kResortCap + (i - qs);
+    if (at_or_after && before) {
+        a.rk[stay] = key;
+        a.ri[stay] = p;
+        return;
     }
-    // runs of lanes with equal buckets (the valid lanes are a prefix of the wave)
-    const uint64_t act = __ballot(valid);
-    const uint32_t bup = __shfl_up(b, 1u, 64);
-    const bool head = valid && (lane == 0 || bup != b);
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane
-    const uint64_t own = heads & upto;            // nonzero on a valid lane: lane 0 is a head
-    const uint32_t s = valid ? 63u - (uint32_t)__builtin_clzll(own) : 0u;
-    const uint64_t later = heads & ~upto;
-    const uint32_t e = later ? (uint32_t)__builtin_ctzll(later) : (uint32_t)__popcll(act);
-    uint32_t base = 0;
-    if (head) base = atomicAdd(&a.cnt[b], e - lane);
-    base = __shfl(base, (int)s, 64);
-    if (!valid) return;
-    const uint32_t rank = base + (lane - s);
-    if (rank < kResortCap) {
-        const uint64_t slot = (uint64_t)b * kResortCap + rank;
-        a.rk[slot] = key;
-        a.ri[slot] = p;
+    const uint32_t b = find_bucket(a, key, p, g, nb);
+    a.rk[stay] = ~0ull;  // a hole
+    a.ri[stay] = ~0u;
+    atomicAdd(&a.moved_out[g], 1u);
+    const uint32_t slot =
+        bucket_pos(a, nb, b + 1) - bucket_pos(a, nb, b) + atomicAdd(&a.moved_in[b], 1u);
+    if (slot < kResortCap) {
+        a.rk[(uint64_t)b * kResortCap + slot] = key;
+        a.ri[(uint64_t)b * kResortCap + slot] = p;
     } else {
         const uint32_t o = atomicAdd(a.ovf, 1u);
         a.ok[o] = key;
@@ -326,49 +339,73 @@ __device__ __forceinline__ void emit_splitter(const ResortArgs& a, uint32_t nb, 
     }
 }
 
+// The smallest c >= 1 whose splitter position c n / nb (rounded down) is at or past q0.
+__device__ __forceinline__ uint32_t first_splitter(const ResortArgs& a, uint32_t nb, uint32_t q0) {
+    const uint32_t c = (uint32_t)(((uint64_t)q0 * nb + a.n - 1) / a.n);
+    return c ? c : 1u;
+}
+
 __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortArgs a, uint32_t nb) {
     constexpr uint32_t NT = kResortBucketThreads;
     __shared__ uint64_t s_k[kResortCap];  // the bucket's rows (LDS sort), or exchange rows
     __shared__ uint32_t s_i[kResortCap];
-    __shared__ uint32_t s_tmp[NT / 64 + 1];
+    __shared__ uint32_t s_tmp[2 * (NT / 64 + 1)];
     __shared__ uint32_t s_fill;
     static_assert(kResortCap >= 4 * NT && kResortRegRows == 2 * NT, "row layout");
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
-    const uint32_t m = a.cnt[b];
-    uint32_t part = 0;
-    for (uint32_t t = tid; t < b; t += NT) part += a.cnt[t];
-    uint32_t start;
-    block_exclusive_scan<NT>(part, s_tmp, &start);
+    // the bucket's positions, sources that left and sources that came: its rows are its
+    // positions' slots (stayers and holes) and then those that came
+    const uint32_t q_b = bucket_pos(a, nb, b), npos = bucket_pos(a, nb, b + 1) - q_b;
+    const uint32_t came = a.moved_in[b], left = a.moved_out[b];
+    uint32_t part_in = 0, part_out = 0;
+    for (uint32_t t = tid; t < b; t += NT) {
+        part_in += a.moved_in[t];
+        part_out += a.moved_out[t];
+    }
+    uint32_t in_before, out_before;
+    block_sum_and_scan<NT>(part_out, part_in, s_tmp, &out_before, &in_before);
+    const uint32_t start = __builtin_amdgcn_readfirstlane(q_b + in_before - out_before);
+    const uint32_t m = npos + came - left;  // the bucket's sources
     if (b == 0 && tid == 0) *a.spl_cube = a.cubes[0];  // the next sort's splitters' cube
-    if (start > a.n || m > a.n - start) return;       // counts always add up to n
+    if (left > npos || start > a.n || m > a.n - start) return;  // never, on counts of a pass
     const uint64_t slots = (uint64_t)b * kResortCap;
-    if (m <= kResortRegRows) {
+    const uint32_t R = npos + came;  // rows in the slots when none overflowed
+    if (R <= kResortRegRows) {  // holes sort after every source
         SortRow r[2];
         for (uint32_t s = 0; s < 2; s++) {
             const uint32_t j = 2 * tid + s;
-            r[s] = j < m ? SortRow{a.rk[slots + j], a.ri[slots + j]} : row_max();
+            r[s] = j < R ? SortRow{a.rk[slots + j], a.ri[slots + j]} : row_max();
         }
-        if (m > kResortRegRows / 2) bitonic_pairs<NT>(r, kResortRegRows, s_k, s_i);
-        else bitonic_pairs<NT>(r, pow2_at_least(m), s_k, s_i);
+        if (R > kResortRegRows / 2) bitonic_pairs<NT, kResortRegRows>(r, 0u, s_k, s_i);
+        else bitonic_pairs<NT>(r, pow2_at_least(R), s_k, s_i);
         for (uint32_t s = 0; s < 2; s++) {
             const uint32_t j = 2 * tid + s;
             if (j < m) {
                 a.out_k[start + j] = r[s].k;
                 a.out_i[start + j] = r[s].i;
                 a.order[start + j] = r[s].i;
-                emit_splitter(a, nb, start + j, r[s].k, r[s].i);
+            }
+        }
+        // the next sort's splitters among these rows (a few per bucket, found once per block)
+        for (uint32_t c = first_splitter(a, nb, start); c < nb; c++) {
+            const uint32_t q = (uint32_t)((uint64_t)c * a.n / nb) - start;
+            if (q >= m) break;
+            if ((q >> 1) == tid) {
+                const SortRow v = (q & 1u) ? r[1] : r[0];
+                a.spl_k[c - 1] = v.k;
+                a.spl_i[c - 1] = v.i;
             }
         }
         return;
     }
-    if (m <= kResortCap) {
-        for (uint32_t j = tid; j < m; j += NT) {
+    if (R <= kResortCap) {
+        for (uint32_t j = tid; j < R; j += NT) {
             s_k[j] = a.rk[slots + j];
             s_i[j] = a.ri[slots + j];
         }
         lds_barrier();
         bitonic_rows<NT>(
-            m,
+            R,
             [](uint32_t i, uint32_t j) {
                 const uint64_t ki = s_k[i], kj = s_k[j];
                 const uint32_t ii = s_i[i], ij = s_i[j];
@@ -388,23 +425,29 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
         }
         return;
     }
-    // past the slots: the bucket's sources are gathered at their final positions and sorted
-    // there (__syncthreads orders the block's device-memory accesses between the steps)
+    // past the slots: the bucket's sources (its slots but the holes, and its overflow entries)
+    // are gathered at their final positions and sorted there (__syncthreads orders the block's
+    // device-memory accesses between the steps)
     uint64_t* gk = a.out_k + start;
     uint32_t* gi = a.out_i + start;
-    for (uint32_t j = tid; j < kResortCap; j += NT) {
-        gk[j] = a.rk[slots + j];
-        gi[j] = a.ri[slots + j];
-    }
     if (tid == 0) s_fill = 0;
     __syncthreads();
+    for (uint32_t j = tid; j < kResortCap; j += NT) {
+        const uint32_t i = a.ri[slots + j];
+        if (i == ~0u) continue;
+        const uint32_t d = atomicAdd(&s_fill, 1u);
+        if (d < m) {
+            gk[d] = a.rk[slots + j];
+            gi[d] = i;
+        }
+    }
     const uint32_t no = *a.ovf;
     for (uint32_t o = tid; o < no; o += NT) {
         if (a.ob[o] != b) continue;
-        const uint32_t j = kResortCap + atomicAdd(&s_fill, 1u);
-        if (j < m) {  // exactly m - kResortCap entries name this bucket
-            gk[j] = a.ok[o];
-            gi[j] = a.oi[o];
+        const uint32_t d = atomicAdd(&s_fill, 1u);
+        if (d < m) {  // slots and entries of this bucket hold exactly m sources
+            gk[d] = a.ok[o];
+            gi[d] = a.oi[o];
         }
     }
     __syncthreads();

@@ -80,7 +80,8 @@ struct ResortArgs {
     uint64_t* spl_k;         // [kResortMaxBuckets] splitters (the bucket kernel leaves the next
     uint32_t* spl_i;         //  sort's, from its own sorted rows)
     float4* spl_cube;        // [1] the cube the splitters' keys were made with
-    uint32_t* cnt;           // [kResortMaxBuckets] sources per bucket
+    uint32_t* moved_in;      // [kResortMaxBuckets] sources that came into each bucket
+    uint32_t* moved_out;     // [kResortMaxBuckets] sources that left each bucket's positions
     uint32_t* ovf;           // [1] overflow entries
     uint64_t* ok;            // [n] overflow list: key, index, bucket
     uint32_t* oi;
