@@ -99,12 +99,17 @@ def progress(msg):
     print(f"[bench {time.perf_counter() - _T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_bytes(n, V, passes=8, emit=True, quad=False):
-    """Algorithmic HBM bytes per step of the HBM-bound stages."""
+def stage_bytes(n, V, passes=8, emit=True, quad=False, seeded=True):
+    """Algorithmic HBM bytes per step of the stages.  seeded: the sort from the previous order
+    (the keys computed in its place pass: the first stage only reads the positions for the
+    cube)."""
     node = 66 + (128 if emit else 0) + (64 if quad else 0)  # write 33 + read 33 (+ OctreeNode, moments)
     return {
-        "bbox_morton": n * (16 + 16 + 8 + 8 + 4),  # bbox read; Morton read, 2 key writes, iota
-        "sort": n * 32 * passes,                   # per pass: count reads 8, scatter moves 2 x 12
+        # bbox read (+ Morton read, 2 key writes, iota for the radix sort)
+        "bbox_morton": n * 16 if seeded else n * (16 + 16 + 8 + 8 + 4),
+        # seeded: order 4 + position 16 read, slot 12 written; slot 12 read, key 8 + index 4 +
+        # next order 4 written.  radix: per pass count reads 8, scatter moves 2 x 12
+        "sort": n * 60 if seeded else n * 32 * passes,
         "build": n * 40 + V * node,                # leaf gather + per-node traffic
         "update": n * (16 + 16 + 16 + 4 + 16 + 16),
     }
@@ -529,7 +534,8 @@ def main():
         tj = json.load(open(tpath))
         traffic = tj.get("force_compact_bytes_per_launch")
         valu_frac = tj.get("valu_issue_frac")
-    sb = stage_bytes(n, V, emit=False)  # the OctreeNode array is written on read only
+    # the OctreeNode array is written on read only
+    sb = stage_bytes(n, V, emit=False, seeded=a.sort == "seeded")
     stages = {}
     for s, ms in stage_ms.items():
         per = ms / K
