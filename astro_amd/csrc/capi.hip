@@ -135,7 +135,8 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->leaf_of_sorted, c->pnext, c->cubes, c->minmax, c->cube_counter, c->plevel, c->fused_buf,
                     c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom, c->order,
-                    c->rs_keys, c->rs_idx, c->spl_keys, c->spl_idx, c->rs_count, c->spl_cube};
+                    c->rs_keys, c->rs_idx, c->spl_keys, c->spl_idx, c->rs_count, c->spl_cube,
+                    c->spos, c->stype, c->rs_pos, c->rs_type};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -247,10 +248,14 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->order, NP);
     e = e ? e : dalloc(c->device_bytes, &c->rs_keys, c->rs_slots);
     e = e ? e : dalloc(c->device_bytes, &c->rs_idx, c->rs_slots);
+    e = e ? e : dalloc(c->device_bytes, &c->rs_pos, c->rs_slots);
+    e = e ? e : dalloc(c->device_bytes, &c->rs_type, c->rs_slots);
     e = e ? e : dalloc(c->device_bytes, &c->spl_keys, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->spl_idx, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->rs_count, 2ull * kResortMaxBuckets + 1);
     e = e ? e : dalloc(c->device_bytes, &c->spl_cube, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->spos, NT);  // the sort's positions and types in
+    e = e ? e : dalloc(c->device_bytes, &c->stype, NT); // sorted order
     if (e != hipSuccess) {
         astro_destroy(c);
         return ASTRO_ERR_NOMEM;
@@ -903,6 +908,7 @@ extern "C" int astro_update_spheres(astro_ctx* c, float dt) {
 static TreeArgs tree_args(astro_ctx* c) {
     TreeArgs a{};
     a.pos = c->pos;
+    a.spos = c->spos;  // written by the sort that precedes every fused build
     a.sys_of = c->sys_of;
     a.sorted_keys = c->sorted_codes;
     a.sorted_idx = c->sorted_idx;
@@ -958,6 +964,8 @@ static ResortArgs resort_args(astro_ctx* c) {
     r.n = c->nt;
     r.rk = c->rs_keys;
     r.ri = c->rs_idx;
+    r.rp = c->rs_pos;
+    r.rt = c->rs_type;
     r.spl_k = c->spl_keys;
     r.spl_i = c->spl_idx;
     r.spl_cube = c->spl_cube;
@@ -969,6 +977,9 @@ static ResortArgs resort_args(astro_ctx* c) {
     r.ob = c->tmp_idx_b;
     r.out_k = c->sorted_codes;
     r.out_i = c->sorted_idx;
+    r.type = c->type;
+    r.spos = c->spos;
+    r.stype = c->stype;
     return r;
 }
 
@@ -986,11 +997,14 @@ static int enqueue_sort(astro_ctx* c) {
         return rc;
     c->last_sort_seeded = false;
     c->keys_pending = false;
-    if (seeded_sort_applies(c)) {  // the order and splitters the next sort starts from
+    if (seeded_sort_applies(c)) {
+        // the order and splitters the next sort starts from, and positions in sorted order
         launch_resort_seed(c->sorted_codes, c->sorted_idx, resort_args(c), c->stream);
         c->order_valid = true;
-        LAUNCHCHK(c);
+    } else {
+        launch_sorted_gather(c->sorted_idx, c->pos, c->type, c->nt, c->spos, c->stype, c->stream);
     }
+    LAUNCHCHK(c);
     return ASTRO_OK;
 }
 
@@ -1069,13 +1083,19 @@ static int enqueue_build(astro_ctx* c) {
     return enqueue_layers(c, true);
 }
 
-static int enqueue_forces(astro_ctx* c, unsigned long long* stats) {
+// sorted_pos: in a step, right after its sort and build, the targets' positions and types are
+// read in sorted order; elsewhere the positions may have changed since the sort.
+static int enqueue_forces(astro_ctx* c, unsigned long long* stats, bool sorted_pos = false) {
     // the traversal trusts the tree's child indices (no per-visit bounds check): only a tree
     // built on the current layer table is traversed
     if (!c->tree_built)
         return fail(c, ASTRO_ERR_STATE, "no tree for the current particles: call astro_build_tree");
     ForceArgs a = force_args(c);
     a.stats = stats;
+    if (sorted_pos) {
+        a.spos = c->spos;
+        a.stype = c->stype;
+    }
     launch_force_compact(a, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -1185,7 +1205,7 @@ static int enqueue_step(astro_ctx* c, float dt, hipEvent_t* ev = nullptr) {
     if (ev) record_stage(c, ev[3]);
     {
         StageRange r("astro.force");
-        if ((rc = enqueue_forces(c, nullptr))) return rc;
+        if ((rc = enqueue_forces(c, nullptr, true))) return rc;
     }
     if (ev) record_stage(c, ev[4]);
     StageRange update_range("astro.update");

@@ -78,11 +78,17 @@ struct astro_ctx {
     uint32_t* order = nullptr;
     uint64_t* rs_keys = nullptr;
     uint32_t* rs_idx = nullptr;
+    float4* rs_pos = nullptr;
+    uint32_t* rs_type = nullptr;
     uint64_t* spl_keys = nullptr;
     uint32_t* spl_idx = nullptr;
     uint32_t* rs_count = nullptr;
     float4* spl_cube = nullptr;        // the cube of the splitters' keys
     uint64_t rs_slots = 0;             // rows of rs_keys / rs_idx
+    // positions and types in sorted order, written by every sort of a build (read by the fused
+    // build, and by the traversal of the same step)
+    float4* spos = nullptr;
+    uint32_t* stype = nullptr;
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation
     uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
     uint64_t scratch_words = 0;

@@ -485,8 +485,10 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     Visits st;
     if (active) {
         const uint32_t pidx = j < a.nt ? a.sorted_idx[j] : j;
-        const float4 p = a.pos[pidx];
-        const uint32_t type = a.type[pidx];
+        // in a step, the sort left the targets' positions and types in sorted order
+        const bool sorted = a.spos && j < a.nt;
+        const float4 p = sorted ? a.spos[j] : a.pos[pidx];
+        const uint32_t type = sorted ? a.stype[j] : a.type[pidx];
         uint32_t self =
             (type != 2u && j < a.nt) ? (uint32_t)(a.off0 + a.leaf_of_sorted[j]) : 0xFFFFFFFFu;
         uint32_t root = (uint32_t)(a.root_off + sys);

@@ -15,7 +15,8 @@
 //    list);
 //  * k_resort_bucket (one block per bucket): the bucket's offset follows from the positions and
 //    the counts in and out before it; its slots are sorted by (key, index), holes last, and its
-//    sources written to their final positions, and as the next sort's previous order.  A bucket
+//    sources written to their final positions (with their positions and types, read once here
+//    for the build and the traversal), and as the next sort's previous order.  A bucket
 //    beyond kResortCap gathers its sources at their final positions and sorts them there, in
 //    device memory (slow, but only for orders far from the previous one).
 // The block sorts are bitonic networks over rows held two per thread: steps whose partner rows
@@ -300,7 +301,9 @@ __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs
     if (g > 0) s0 = SortRow{a.spl_k[g - 1], a.spl_i[g - 1]};
     if (g + 1 < nb) s1 = SortRow{a.spl_k[g], a.spl_i[g]};
     const uint32_t p = a.order[i];
-    const uint64_t key = morton_key(a.pos[p], a.cubes, 0u, a.params);
+    const float4 xp = a.pos[p];
+    const uint32_t ty = a.type[p];
+    const uint64_t key = morton_key(xp, a.cubes, 0u, a.params);
     const SortRow x{key, p};
     const bool at_or_after = g == 0 || !row_less(x, s0);
     const bool before = g + 1 >= nb || row_less(x, s1);
@@ -308,6 +311,8 @@ __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs
     if (at_or_after && before) {
         a.rk[stay] = key;
         a.ri[stay] = p;
+        a.rp[stay] = xp;
+        a.rt[stay] = ty;
         return;
     }
     const uint32_t b = find_bucket(a, key, p, g, nb);
@@ -317,8 +322,11 @@ __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs
     const uint32_t slot =
         bucket_pos(a, nb, b + 1) - bucket_pos(a, nb, b) + atomicAdd(&a.moved_in[b], 1u);
     if (slot < kResortCap) {
-        a.rk[(uint64_t)b * kResortCap + slot] = key;
-        a.ri[(uint64_t)b * kResortCap + slot] = p;
+        const uint64_t q = (uint64_t)b * kResortCap + slot;
+        a.rk[q] = key;
+        a.ri[q] = p;
+        a.rp[q] = xp;
+        a.rt[q] = ty;
     } else {
         const uint32_t o = atomicAdd(a.ovf, 1u);
         a.ok[o] = key;
@@ -371,19 +379,38 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
     const uint64_t slots = (uint64_t)b * kResortCap;
     const uint32_t R = npos + came;  // rows in the slots when none overflowed
     if (R <= kResortRegRows) {  // holes sort after every source
+        // with n <= 2^22 a row's index word is index << 10 | slot: the same order (indices are
+        // distinct), and the row's position and type are read from its slot afterwards
+        static_assert(kResortRegRows <= 1024u, "slot in 10 bits");
+        const bool packed = a.n <= kResortPackMax;
         SortRow r[2];
         for (uint32_t s = 0; s < 2; s++) {
             const uint32_t j = 2 * tid + s;
-            r[s] = j < R ? SortRow{a.rk[slots + j], a.ri[slots + j]} : row_max();
+            if (j < R) {
+                const uint32_t i = a.ri[slots + j];
+                r[s] = SortRow{a.rk[slots + j], packed && i != ~0u ? (i << 10) | j : i};
+            } else {
+                r[s] = row_max();
+            }
         }
         if (R > kResortRegRows / 2) bitonic_pairs<NT, kResortRegRows>(r, 0u, s_k, s_i);
         else bitonic_pairs<NT>(r, pow2_at_least(R), s_k, s_i);
         for (uint32_t s = 0; s < 2; s++) {
             const uint32_t j = 2 * tid + s;
             if (j < m) {
+                const uint32_t i = packed ? r[s].i >> 10 : r[s].i;
                 a.out_k[start + j] = r[s].k;
-                a.out_i[start + j] = r[s].i;
-                a.order[start + j] = r[s].i;
+                a.out_i[start + j] = i;
+                a.order[start + j] = i;
+                if (packed) {
+                    const uint32_t slot = r[s].i & 1023u;
+                    a.spos[start + j] = a.rp[slots + slot];
+                    a.stype[start + j] = a.rt[slots + slot];
+                } else {
+                    a.spos[start + j] = a.pos[i];
+                    a.stype[start + j] = a.type[i];
+                }
+                r[s].i = i;  // the splitters carry the plain index
             }
         }
         // the next sort's splitters among these rows (a few per bucket, found once per block)
@@ -421,6 +448,8 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
             a.out_k[start + j] = s_k[j];
             a.out_i[start + j] = s_i[j];
             a.order[start + j] = s_i[j];
+            a.spos[start + j] = a.pos[s_i[j]];
+            a.stype[start + j] = a.type[s_i[j]];
             emit_splitter(a, nb, start + j, s_k[j], s_i[j]);
         }
         return;
@@ -466,6 +495,8 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
         [] { __syncthreads(); });
     for (uint32_t j = tid; j < m; j += NT) {
         a.order[start + j] = gi[j];
+        a.spos[start + j] = a.pos[gi[j]];
+        a.stype[start + j] = a.type[gi[j]];
         emit_splitter(a, nb, start + j, gk[j], gi[j]);
     }
 }
@@ -478,8 +509,28 @@ __global__ void k_resort_seed(const uint64_t* __restrict__ sk, const uint32_t* _
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += gridDim.x * blockDim.x) {
         const uint32_t i = si[q];
         a.order[q] = i;
+        a.spos[q] = a.pos[i];
+        a.stype[q] = a.type[i];
         emit_splitter(a, nb, q, sk[q], i);
     }
+}
+
+__global__ void k_sorted_gather(const uint32_t* __restrict__ si, const float4* __restrict__ pos,
+                                const uint32_t* __restrict__ type, uint32_t n,
+                                float4* __restrict__ spos, uint32_t* __restrict__ stype) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const uint32_t i = si[q];
+        spos[q] = pos[i];
+        stype[q] = type[i];
+    }
+}
+void launch_sorted_gather(const uint32_t* si, const float4* pos, const uint32_t* type, uint32_t n,
+                          float4* spos, uint32_t* stype, hipStream_t s) {
+    if (!n) return;
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_sorted_gather, dim3(blocks), dim3(256), 0, s, si, pos, type, n, spos,
+                       stype);
 }
 void launch_resort_seed(const uint64_t* sk, const uint32_t* si, const ResortArgs& a,
                         hipStream_t s) {

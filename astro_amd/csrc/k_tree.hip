@@ -375,7 +375,7 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     if (a.quirk_last_leaf && end == a.nt) end = u + 1;
     AGG g;
     for (uint32_t i = j; i < end; i++) {
-        float4 x = a.pos[a.sorted_idx[i]];
+        float4 x = a.spos[i];
         g.add(x.x, x.y, x.z, x.w);
     }
     float cm[4];
@@ -401,14 +401,14 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
         if (sizeof(AGG) == sizeof(Agg64)) {
             Mom64 q;
             for (uint32_t i = j; i < end; i++) {
-                float4 x = a.pos[a.sorted_idx[i]];
+                float4 x = a.spos[i];
                 q.add(x.x, x.y, x.z, x.w, cm, nullptr);
             }
             q.fin(S);
         } else {
             Mom32 q;
             for (uint32_t i = j; i < end; i++) {
-                float4 x = a.pos[a.sorted_idx[i]];
+                float4 x = a.spos[i];
                 q.add(x.x, x.y, x.z, x.w, cm, nullptr);
             }
             q.fin(S);
@@ -857,7 +857,7 @@ template <typename AGG>
 __device__ inline void agg_particles(const TreeArgs& a, uint32_t j0, uint32_t j1, float* cm) {
     AGG g;
     for (uint32_t i = j0; i < j1; i++) {
-        const float4 x = a.pos[a.sorted_idx[i]];
+        const float4 x = a.spos[i];
         g.add(x.x, x.y, x.z, x.w);
     }
     g.fin(cm);
@@ -874,7 +874,7 @@ __device__ inline void mom_particles(const TreeArgs& a, uint32_t j0, uint32_t j1
                                      float* S) {
     MOM q;
     for (uint32_t i = j0; i < j1; i++) {
-        const float4 x = a.pos[a.sorted_idx[i]];
+        const float4 x = a.spos[i];
         q.add(x.x, x.y, x.z, x.w, cm, nullptr);
     }
     q.fin(S);

@@ -69,14 +69,18 @@ constexpr uint32_t kResortCap = 2048;          // slots per bucket (sorted in LD
 constexpr uint32_t kResortMaxBuckets = 8192;
 constexpr uint32_t kResortMaxSamples = 8192;   // samples sorted in one block's LDS
 constexpr uint32_t kResortOversample = 2;      // samples per bucket (at most)
+// Up to this many sources the register sort's rows carry index << 10 | slot (slot < 1,024)
+constexpr uint32_t kResortPackMax = 1u << 22;
 struct ResortArgs {
     const float4* pos;       // [n] source positions (one system)
     const float4* cubes;     // [1] its cube
     astro_params params;     // quantizer and key bits
     uint32_t* order;         // [n] previous sorted order in, this sort's order out
     uint32_t n;
-    uint64_t* rk;            // [buckets * kResortCap] bucket slots
+    uint64_t* rk;            // [buckets * kResortCap] bucket slots: key, index, position, type
     uint32_t* ri;
+    float4* rp;
+    uint32_t* rt;
     uint64_t* spl_k;         // [kResortMaxBuckets] splitters (the bucket kernel leaves the next
     uint32_t* spl_i;         //  sort's, from its own sorted rows)
     float4* spl_cube;        // [1] the cube the splitters' keys were made with
@@ -88,6 +92,9 @@ struct ResortArgs {
     uint32_t* ob;
     uint64_t* out_k;         // [n] sorted keys and indices
     uint32_t* out_i;
+    const uint32_t* type;    // [n] source types
+    float4* spos;            // [n] positions and types in sorted order (out)
+    uint32_t* stype;
 };
 // Buckets of a seeded sort of n sources; 0 when n is beyond it.
 uint32_t resort_buckets(uint32_t n);
@@ -99,9 +106,13 @@ inline uint64_t resort_slots(uint32_t nmax) {
 }
 void launch_resort(const ResortArgs& a, hipStream_t s);
 // After a radix sort of the same sources: the previous order and the splitters of the next
-// seeded sort from its result (sk, si), and the cube they were made with.
+// seeded sort from its result (sk, si), and the cube they were made with; the positions and
+// types in sorted order.
 void launch_resort_seed(const uint64_t* sk, const uint32_t* si, const ResortArgs& a,
                         hipStream_t s);
+// Positions and types in sorted order: spos[j] = pos[si[j]], stype[j] = type[si[j]], j < n.
+void launch_sorted_gather(const uint32_t* si, const float4* pos, const uint32_t* type, uint32_t n,
+                          float4* spos, uint32_t* stype, hipStream_t s);
 // The key kernel's per-source outputs (codes, their copy, 0..n-1) from a sort's result.
 void launch_resort_keys(const uint64_t* sk, const uint32_t* si, uint32_t n, uint64_t* codes,
                         uint64_t* codes2, uint32_t* idx, hipStream_t s);
@@ -109,6 +120,7 @@ void launch_resort_keys(const uint64_t* sk, const uint32_t* si, uint32_t n, uint
 // ---- k_tree.hip ---------------------------------------------------------------------------
 struct TreeArgs {
     const float4* pos;          // particle positions + mass
+    const float4* spos;         // [nt] the same in sorted order (the sort writes it)
     const uint32_t* sys_of;     // system of each particle / sorted position
     const uint64_t* sorted_keys;
     const uint32_t* sorted_idx;
@@ -210,6 +222,8 @@ struct ForceArgs {
     const OctreeNode* onodes;
     const float4* pos;
     const uint32_t* type;
+    const float4* spos;         // [nt] positions and types in sorted order, when the step's sort
+    const uint32_t* stype;      //  wrote them since the positions last changed; else null
     const uint32_t* sorted_idx;
     const uint32_t* leaf_of_sorted;
     const uint32_t* sys_of;
