@@ -120,6 +120,8 @@ SIGNATURES = [
     ("astro_scan_noalloc", _I, [_P, _P, _U32, _P]),
     ("astro_set_wide_offsets", _I, [_P, _I]),
     ("astro_set_sort_mode", _I, [_P, _I]),
+    ("astro_set_skip_unreached_layers", _I, [_P, _I]),
+    ("astro_tree_info", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_I)]),
     ("astro_sort_info", _I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32),
                             ctypes.POINTER(_F)]),
@@ -472,6 +474,17 @@ class Context:
 
     def set_wide_offsets(self, on):
         self._chk(self.lib.astro_set_wide_offsets(self.h, 1 if on else 0))
+
+    def set_skip_unreached_layers(self, on):
+        """True (default): a build leaves out the tree layers no walk can reach (written in full
+        when the tree is read)."""
+        self._chk(self.lib.astro_set_skip_unreached_layers(self.h, 1 if on else 0))
+
+    def tree_info(self):
+        """(lowest layer the last build wrote, whether layers are left out until read)."""
+        low, part = ctypes.c_uint32(), ctypes.c_int()
+        self._chk(self.lib.astro_tree_info(self.h, ctypes.byref(low), ctypes.byref(part)))
+        return low.value, bool(part.value)
 
     def set_sort_mode(self, mode):
         """SORT_RADIX (from scratch every build) or SORT_SEEDED (from the previous order)."""

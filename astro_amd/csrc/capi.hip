@@ -136,7 +136,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom, c->order,
                     c->rs_keys, c->rs_idx, c->spl_keys, c->spl_idx, c->rs_count, c->spl_cube,
-                    c->spos, c->stype, c->rs_pos, c->rs_type};
+                    c->spos, c->stype, c->rs_pos, c->rs_type, c->wlow};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -254,6 +254,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->spl_idx, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->rs_count, 2ull * kResortMaxBuckets + 1);
     e = e ? e : dalloc(c->device_bytes, &c->spl_cube, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->wlow, 2);
     e = e ? e : dalloc(c->device_bytes, &c->spos, NT);  // the sort's positions and types in
     e = e ? e : dalloc(c->device_bytes, &c->stype, NT); // sorted order
     if (e != hipSuccess) {
@@ -265,6 +266,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     hipStream_t s = c->stream;
     launch_fill_u32(c->err_word, 0, 1, s);
     launch_fill_u32(c->cube_counter, 0, 1, s);
+    launch_fill_u32(c->wlow, 0, 2, s);
     launch_fill_u32(c->layer_counts, 0, kMaxLayers, s);
     launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
     launch_fill_u32(c->sys_of, 0, NP, s);
@@ -392,6 +394,7 @@ extern "C" int astro_upload(astro_ctx* c, const PositionMass* pos, const Velocit
                             const ColorType* col, uint32_t n, uint32_t n_tree) {
     if (!c || !pos) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (n > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "too many particles");
     if (n_tree == 0 || n_tree > n) n_tree = n;
     if (n_tree > c->cfg.max_tree) return fail(c, ASTRO_ERR_CAPACITY, "too many tree sources");
@@ -413,6 +416,7 @@ extern "C" int astro_upload_systems(astro_ctx* c, const PositionMass* pos,
                                     const uint32_t* offsets, uint32_t n_systems) {
     if (!c || !pos || !offsets || n_systems == 0) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (n_systems > c->cfg.max_systems)
         return fail(c, ASTRO_ERR_CAPACITY, "more systems than the context was created for");
     if (offsets[0] != 0) return fail(c, ASTRO_ERR_ARG, "offsets[0] must be 0");
@@ -530,9 +534,21 @@ extern "C" int astro_get_buffer(astro_ctx* c, int which, void** d_ptr, uint64_t*
             p = c->onodes;
             b = c->node_capacity * sizeof(OctreeNode);
             break;
-        case ASTRO_BUF_TREE_POS: p = c->tpos; b = c->node_capacity * 16ull; break;
-        case ASTRO_BUF_TREE_LINKS: p = c->tlink; b = c->node_capacity * sizeof(TLink); break;
-        case ASTRO_BUF_TREE_MOMENTS: p = c->qmom; b = c->qmom ? c->node_capacity * 32ull : 0; break;
+        case ASTRO_BUF_TREE_POS:
+            if (int rc = materialize_tree(c)) return rc;
+            p = c->tpos;
+            b = c->node_capacity * 16ull;
+            break;
+        case ASTRO_BUF_TREE_LINKS:
+            if (int rc = materialize_tree(c)) return rc;
+            p = c->tlink;
+            b = c->node_capacity * sizeof(TLink);
+            break;
+        case ASTRO_BUF_TREE_MOMENTS:
+            if (int rc = materialize_tree(c)) return rc;
+            p = c->qmom;
+            b = c->qmom ? c->node_capacity * 32ull : 0;
+            break;
         case ASTRO_BUF_LAYER_COUNTS: p = c->layer_counts; b = kMaxLayers * 4; break;
         case ASTRO_BUF_LEAF_OF_SORTED: p = c->leaf_of_sorted; b = NT * 4; break;
         case ASTRO_BUF_CUBES: p = c->cubes; b = c->cfg.max_systems * 16ull; break;
@@ -638,6 +654,7 @@ extern "C" int astro_radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_
                                 const uint32_t* d_count) {
     if (!c || !kin || !iin || !kout || !iout || !d_count) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (estimated_len == 0) return ASTRO_OK;  // RadixSort.swift:106
     if (estimated_len > c->cfg.max_particles)
         return fail(c, ASTRO_ERR_CAPACITY, "estimated_len beyond the sorter's max length");
@@ -656,6 +673,7 @@ extern "C" int astro_radix_sort(astro_ctx* c, const uint64_t* kin, const uint32_
 extern "C" int astro_scan(astro_ctx* c, const uint32_t* in, uint32_t* out, uint32_t len) {
     if (!c || (!in && len) || (!out && len)) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (len == 0) return ASTRO_OK;
     if (scan_tiles(len) > c->scratch_words)
         return fail(c, ASTRO_ERR_CAPACITY, "scan longer than the context supports");
@@ -669,6 +687,7 @@ extern "C" int astro_split_pass(astro_ctx* c, const uint64_t* kin, const uint32_
                                 uint32_t estimated_len, const uint32_t* d_count) {
     if (!c || !kin || !iin || !kout || !iout || !d_count || bit > 63) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (estimated_len == 0) return ASTRO_OK;
     if (estimated_len > c->cfg.max_particles)
         return fail(c, ASTRO_ERR_CAPACITY, "estimated_len beyond the sorter's max length");
@@ -702,6 +721,7 @@ static void enqueue_cube(astro_ctx* c, uint32_t count) {
 extern "C" int astro_generate_morton_codes(astro_ctx* c, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     if (count > c->nt) return fail(c, ASTRO_ERR_CAPACITY, "count beyond the tree sources");
     enqueue_cube(c, count);
@@ -714,6 +734,7 @@ extern "C" int astro_generate_morton_codes(astro_ctx* c, uint32_t count) {
 extern "C" int astro_fill_indices(astro_ctx* c, uint32_t* d_buf, uint32_t start, uint32_t end) {
     if (!c || !d_buf) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     launch_iota(d_buf, start, end, c->stream);
     LAUNCHCHK(c);
     return ASTRO_OK;
@@ -723,6 +744,7 @@ extern "C" int astro_clear_buffer(astro_ctx* c, void* d_buf, uint32_t count,
                                   uint32_t elem_bytes) {
     if (!c || !d_buf) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (elem_bytes == 8)
         launch_fill_u64(static_cast<uint64_t*>(d_buf), ASTRO_INVALID_MORTON_CODE, count,
                         c->stream);
@@ -749,6 +771,7 @@ extern "C" int astro_clear_octree_node_buffer(astro_ctx* c, OctreeNode* d_nodes,
 extern "C" int astro_count_unique_morton_codes(astro_ctx* c, int aggregate, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (count > c->cfg.max_particles) return fail(c, ASTRO_ERR_CAPACITY, "count too large");
     if (count == 0) return ASTRO_OK;
     hipStream_t s = c->stream;
@@ -771,6 +794,7 @@ static uint32_t fp64_from(const astro_ctx* c) {
 extern "C" int astro_aggregate_leaf_nodes(astro_ctx* c, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     if (count > c->nt) return fail(c, ASTRO_ERR_CAPACITY, "count beyond the tree sources");
     launch_aggregate_leaf(c->sorted_codes, c->sorted_idx, c->unique_idx, c->pos, c->parent_count,
@@ -786,6 +810,7 @@ extern "C" int astro_aggregate_internal_nodes(astro_ctx* c, uint32_t layer, uint
                                               uint32_t out_off, uint32_t node_count) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (layer == 0 || layer >= c->layers) return fail(c, ASTRO_ERR_ARG, "layer out of range");
     if ((uint64_t)out_off + node_count > c->node_capacity)
         return fail(c, ASTRO_ERR_CAPACITY, "output range beyond the node buffer");
@@ -804,6 +829,7 @@ extern "C" int astro_aggregate_internal_nodes(astro_ctx* c, uint32_t layer, uint
 extern "C" int astro_build_internal_nodes(astro_ctx* c, uint32_t count) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     for (uint32_t l = 1; l < c->layers; l++) {
         int rc;
@@ -839,6 +865,7 @@ static ForceArgs force_args(astro_ctx* c) {
     a.err = c->err_word;
     a.stats = nullptr;
     a.counts = c->layer_counts;
+    a.wlow = c->wlow;
     for (uint32_t l = 0; l < c->layers; l++) a.off[l] = c->off[l];
     a.layers = c->layers;
     a.top = c->force_top ? 1u : 0u;
@@ -872,6 +899,7 @@ static ForceArgs force_args(astro_ctx* c) {
 extern "C" int astro_barnes_hut(astro_ctx* c) {
     if (!c) return ASTRO_ERR_ARG;
     if (int rc = materialize_keys(c)) return rc;
+    if (int rc = materialize_tree(c)) return rc;
     if (int rc = need_particles(c)) return rc;
     if (c->nsys > 1) return fail(c, ASTRO_ERR_STATE, "stage traversal is single-system");
     if (c->params.multipole != 0)
@@ -933,6 +961,21 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.quirk_last_leaf = c->params.quirk_last_leaf;
     a.segmented = c->nsys > 1;
     a.pack = c->total < kLinkPackNodes && !c->force_wide ? 1u : 0u;
+    // layers no walk reaches are left unwritten (rebuilt in full when a caller reads the tree):
+    // one system, monopoles, the traversal's fast form, no OctreeNode array in the same build
+    ForceArgs fa{};
+    fa.p = c->params;
+    set_accept_threshold(fa);
+    a.skip_ok = c->skip_unreached && c->nsys <= 1 && c->params.multipole != 2 &&
+                        !c->params.quirk_last_leaf && fa.fast &&
+                        c->emit_mode != ASTRO_EMIT_EVERY_BUILD
+                    ? 1u
+                    : 0u;
+    a.acc = fa.acc;
+    a.eps2 = c->params.eps2;
+    a.side_shifted = c->params.side_rule == ASTRO_SIDE_SHIFTED ? 1u : 0u;
+    a.cubes = c->cubes;
+    a.wlow = c->wlow;
     return a;
 }
 
@@ -1021,9 +1064,11 @@ static void enqueue_emit(astro_ctx* c, hipStream_t es, bool copy_counts = true) 
 
 // emit: write the OctreeNode array right after the layers (on the main stream); false leaves
 // it to the caller (the pipelined step emits a tree while the next one is being sorted).
-static int enqueue_layers(astro_ctx* c, bool emit = true) {
+// full: every layer written (a caller reads the tree)
+static int enqueue_layers(astro_ctx* c, bool emit = true, bool full = false) {
     hipStream_t s = c->stream;
     TreeArgs a = tree_args(c);
+    if (full) a.skip_ok = 0;
     // scratch: all layers' supertile sums first (one clear), then per-layer tile counts
     uint64_t nsup = 0, ntile = 0;
     for (uint32_t l = 0; l < c->layers; l++) {
@@ -1072,7 +1117,19 @@ static int enqueue_layers(astro_ctx* c, bool emit = true) {
     // on read: the array of this tree is written when a caller reads it
     c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
     c->tree_built = true;
+    c->tree_partial = a.skip_ok != 0;
     LAUNCHCHK(c);
+    return ASTRO_OK;
+}
+
+// On read: the layers the last build left unwritten, by building the same tree again with every
+// layer (its inputs, the sorted keys and positions, are unchanged until the next sort).
+int materialize_tree(astro_ctx* c) {
+    if (!c->tree_partial || !c->tree_built) return ASTRO_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool pending = c->onodes_pending;
+    if (int rc = enqueue_layers(c, false, true)) return rc;
+    c->onodes_pending = pending;
     return ASTRO_OK;
 }
 
@@ -1105,6 +1162,7 @@ static int enqueue_forces(astro_ctx* c, unsigned long long* stats, bool sorted_p
 // (stream-ordered before whatever the caller enqueues next).
 int materialize_onodes(astro_ctx* c) {
     if (!c->onodes_pending) return ASTRO_OK;
+    if (int rc = materialize_tree(c)) return rc;  // every layer first
     c->onodes_pending = false;
     if (!c->tree_built) return ASTRO_OK;  // the layer table changed since: nothing to write
     HIPCHK(c, hipSetDevice(c->device));
@@ -1270,6 +1328,7 @@ extern "C" int astro_step(astro_ctx* c, float dt, uint32_t nsteps) {
     if (nsteps) {
         c->onodes_pending = c->emit_mode == ASTRO_EMIT_ON_READ;
         c->keys_pending = c->last_sort_seeded;
+        // tree_partial keeps the captured build's value
     }
     return ASTRO_OK;
 }
@@ -1382,6 +1441,25 @@ extern "C" int astro_sort_info(astro_ctx* c, int* seeded, uint32_t* overflow) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (seeded) *seeded = c->last_sort_seeded ? 1 : 0;
     if (overflow) *overflow = o;
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_skip_unreached_layers(astro_ctx* c, int on) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (int rc = materialize_tree(c)) return rc;
+    c->skip_unreached = on != 0;
+    invalidate_graph(c);
+    return ASTRO_OK;
+}
+
+extern "C" int astro_tree_info(astro_ctx* c, uint32_t* lowest_written, int* partial) {
+    if (!c) return ASTRO_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    uint32_t w = 0;
+    HIPCHK(c, hipMemcpyAsync(&w, c->wlow, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (lowest_written) *lowest_written = c->tree_partial ? w : 0u;
+    if (partial) *partial = c->tree_partial && w > 0 ? 1 : 0;
     return ASTRO_OK;
 }
 

@@ -74,6 +74,20 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Correctly rounded sqrt for normal x >= 2^-96: the hardware estimate corrected by one ulp
+// with two fused residuals -- the same correction the compiler emits for sqrtf, without the
+// rescaling it adds for tiny inputs.
+__device__ inline float sqrt_rn_normal(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, x);
+    const float rp = __builtin_fmaf(-sp, s, x);
+    s = rm <= 0.0f ? sm : s;
+    s = rp > 0.0f ? sp : s;
+    return s;
+}
+
 // Inclusive scan across the 64 lanes of a wave.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;
@@ -85,6 +99,20 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
+// The NW wave totals tmp[0 .. NW) (NW <= 64) replaced by their exclusive scan, and their sum
+// stored in tmp[NW], by the block's first wave in one shuffle scan.
+template <int NW>
+__device__ inline void scan_wave_totals(uint32_t* tmp) {
+    static_assert(NW >= 1 && NW <= 64, "one wave scans the totals");
+    if ((threadIdx.x >> 6) == 0) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t t = lane < (uint32_t)NW ? tmp[lane] : 0u;
+        const uint32_t inc = wave_inclusive_scan(t);
+        if (lane < (uint32_t)NW) tmp[lane] = inc - t;
+        if (lane == (uint32_t)NW - 1u) tmp[NW] = inc;
+    }
+}
+
 // Exclusive scan over a block of NT threads (NT multiple of 64, <= 1024).  `tmp` is an LDS
 // array of NT/64 + 1 words.  Returns the exclusive prefix; *total receives the block sum.
 template <int NT>
@@ -94,15 +122,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* tmp, uint3
     uint32_t inc = wave_inclusive_scan(v);
     if (lane == 63) tmp[wave] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int w = 0; w < NW; w++) {
-            uint32_t t = tmp[w];
-            tmp[w] = acc;
-            acc += t;
-        }
-        tmp[NW] = acc;
-    }
+    scan_wave_totals<NW>(tmp);
     __syncthreads();
     uint32_t r = tmp[wave] + inc - v;
     *total = tmp[NW];
@@ -123,16 +143,15 @@ __device__ inline uint32_t block_sum_and_scan(uint32_t r, uint32_t v, uint32_t* 
         tmp[NW + 1 + wave] = rinc;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0, racc = 0;
-        for (int w = 0; w < NW; w++) {
-            const uint32_t t = tmp[w];
-            tmp[w] = acc;
-            acc += t;
-            racc += tmp[NW + 1 + w];
+    if (wave == 0) {  // both totals by the first wave: a scan of v's, a sum of r's
+        const uint32_t t = lane < NW ? tmp[lane] : 0u;
+        const uint32_t rt = lane < NW ? tmp[NW + 1 + lane] : 0u;
+        const uint32_t ti = wave_inclusive_scan(t), ri = wave_inclusive_scan(rt);
+        if (lane < NW) tmp[lane] = ti - t;
+        if (lane == NW - 1) {
+            tmp[NW] = ti;
+            tmp[2 * NW + 1] = ri;
         }
-        tmp[NW] = acc;
-        tmp[2 * NW + 1] = racc;
     }
     __syncthreads();
     const uint32_t out = tmp[wave] + inc - v;

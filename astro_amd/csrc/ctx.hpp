@@ -89,6 +89,12 @@ struct astro_ctx {
     // build, and by the traversal of the same step)
     float4* spos = nullptr;
     uint32_t* stype = nullptr;
+    // layers no walk reaches are not written by the step's build (a switch, default on);
+    // wlow: the lowest layer the last build wrote (device word); tree_partial: the last build
+    // may have left layers out, so a caller's read of the tree builds it again in full
+    bool skip_unreached = true;
+    uint32_t* wlow = nullptr;
+    bool tree_partial = false;
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation
     uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms
     uint64_t scratch_words = 0;
@@ -145,3 +151,5 @@ int astro_enqueue_step(astro_ctx* c, float dt);
 int materialize_onodes(astro_ctx* c);
 // On read: writes the per-source key arrays of the last seeded sort (no-op otherwise).
 int materialize_keys(astro_ctx* c);
+// On read: writes the tree layers the last build left out (no-op otherwise).
+int materialize_tree(astro_ctx* c);

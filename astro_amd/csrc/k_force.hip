@@ -112,20 +112,6 @@ void set_accept_threshold(ForceArgs& a) {
     a.fast = ok ? 1u : 0u;
 }
 
-// Correctly rounded sqrt for normal x >= 2^-96: the hardware estimate corrected by one ulp
-// with two fused residuals -- the same correction the compiler emits for sqrtf, without the
-// rescaling it adds for tiny inputs.
-__device__ inline float sqrt_rn_normal(float x) {
-    float s = __builtin_amdgcn_sqrtf(x);
-    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-    const float rm = __builtin_fmaf(-sm, s, x);
-    const float rp = __builtin_fmaf(-sp, s, x);
-    s = rm <= 0.0f ? sm : s;
-    s = rp > 0.0f ? sp : s;
-    return s;
-}
-
 __device__ inline float clampf(float v, float c) {
     if (!(c > 0.0f)) return v;
     return v < -c ? -c : (v > c ? c : v);
@@ -380,7 +366,9 @@ __device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, fl
     const uint32_t L = a.layers, tid = threadIdx.x;
     if (tid == 0) {
         uint32_t lt = L, K = 0;
+        const uint32_t wl = a.wlow ? *a.wlow : 0u;  // the lowest layer the build wrote
         for (uint32_t l = L; l-- > 0;) {
+            if (l < wl) break;
             const uint32_t c = a.counts[l];
             if (K + c > slots) break;
             K += c;

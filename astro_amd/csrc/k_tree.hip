@@ -738,15 +738,7 @@ __device__ inline uint32_t lds_block_scan(uint32_t v, uint32_t* tmp, uint32_t* t
     const uint32_t inc = wave_inclusive_scan(v);
     if (lane == 63) tmp[wave] = inc;
     lds_barrier();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int w = 0; w < NW; w++) {
-            const uint32_t t = tmp[w];
-            tmp[w] = acc;
-            acc += t;
-        }
-        tmp[NW] = acc;
-    }
+    scan_wave_totals<NW>(tmp);
     lds_barrier();
     const uint32_t r = tmp[wave] + inc - v;
     *total = tmp[NW];
@@ -793,10 +785,19 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const 
     const uint32_t l = blockIdx.x;
     const uint32_t* in = fcnt + (uint64_t)l * tiles;
     uint32_t* out = fbase + (uint64_t)l * tiles;
+    // the block of layer lf - 1 also decides the lowest layer to write (block 0 without fused
+    // layers): k_fused_build's range of tile t - 1 ends at the first head at layer >= lf - 1
+    // in tile t, so a full tile t without one (no node of layer lf - 1 starts in it) leaves a
+    // node for k_fused_fix, which needs every layer below: then every layer is written
+    const bool decide = l == (lf >= 2 ? lf - 1 : 0u);
+    bool trunc = false;
     uint32_t carry = 0;
     for (uint32_t t0 = 0; t0 < tiles; t0 += kFusedThreads) {
         const uint32_t t = t0 + threadIdx.x;
         const uint32_t v = t < tiles ? in[t] : 0u;
+        if (decide && t >= 1 && t < tiles && v == 0u &&
+            (uint64_t)t * kFusedTile + kFusedTile < a.nt)
+            trunc = true;
         uint32_t total;
         const uint32_t ex = block_exclusive_scan<kFusedThreads>(v, s_tmp, &total);
         if (t < tiles) out[t] = carry + ex;
@@ -805,6 +806,26 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const 
     if (threadIdx.x == 0 && l < lf) {
         a.counts[l] = carry;
         if (carry > a.cap[l]) atomicOr(a.err, ERR_CAPACITY);
+    }
+    if (decide) {
+        trunc = __syncthreads_or(trunc) != 0;
+        if (threadIdx.x == 0) {
+            uint32_t W = 0;
+            if (a.skip_ok && lf >= 2 && !trunc) {
+                // layers 0 .. K-1 pass the traversal's opening test RN(sqrt(r2)) >= rmin0 2^l
+                // for every finite target (r2 >= eps2, and RN(sqrt) is monotone): their nodes
+                // are never opened, and the walks reach layer K - 1 at the lowest
+                const float side0 =
+                    a.side_shifted ? 2097152.0f : ldexpf(a.cubes[0].w, -(int)a.bits);
+                const float rmin0 = accept_rmin(side0, a.acc);
+                const float re = sqrt_rn_normal(a.eps2);
+                uint32_t K = 0;
+                while (K < lf && re >= ldexpf(rmin0, (int)K)) K++;
+                W = K > 0 ? K - 1 : 0u;
+            }
+            a.wlow[0] = W;
+            a.wlow[1] = trunc ? 1u : 0u;  // k_fused_fix has nodes to build
+        }
     }
 }
 
@@ -890,7 +911,7 @@ __device__ inline void mom_nodes(const float4* v, const float* sm, uint32_t c0, 
 // QUAD: the second moments too, aggregated in the per-layer kernels' order (emit_leaf,
 // emit_parent) from the layer below kept in LDS beside the centres.
 template <bool QUAD>
-__global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
+__global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_fused_build(TreeArgs a,
                                                                const uint8_t* __restrict__ plevel,
                                                                const uint32_t* __restrict__ fcnt,
                                                                const uint32_t* __restrict__ fbase,
@@ -930,6 +951,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
         h_end = plevel[e0];
     }
     const uint32_t np = e0 - a0;
+    const uint32_t W = *a.wlow;  // layers below W are not written; W's nodes end the walks
     const uint32_t q0 = 2 * tid, q1 = 2 * tid + 1;  // two adjacent positions per thread
     const uint32_t h0 = q0 < np ? plevel[a0 + q0] : kNoHead;
     const uint32_t h1 = q1 < np ? plevel[a0 + q1] : kNoHead;
@@ -1004,6 +1026,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
                 ffix[l * tiles + t] = l == 0 ? c0[i] : base_prev + c0[i];
                 continue;
             }
+            if (l < W) continue;  // no walk reaches this layer
             const uint32_t h = s_h[st[i]];
             const uint64_t nd = a.off[l] + gi;
             TNode tn;
@@ -1012,6 +1035,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_build(TreeArgs a,
             tn.z = cm[i][2];
             tn.m = cm[i][3];
             tn.last = l == 0 ? ASTRO_INVALID_CHILD : (uint32_t)(a.off[l - 1] + base_prev + c1[i] - 1u);
+            if (W > 0 && l == W) tn.last = kNextEnd;  // never opened by a finite target
             tn.info = tn_info(l, l == 0 ? 0u : c1[i] - c0[i], h);
             tn.next = a.pnext[a0 + st[i]];
             tn.start = a0 + st[i];
@@ -1036,6 +1060,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const u
                                                              const uint32_t* fbase,
                                                              const uint32_t* ffix, uint32_t lf,
                                                              uint32_t tiles) {
+    if (a.wlow[1] == 0u) return;  // no tile ran past its halo: nothing left over
     for (uint32_t l = 0; l < lf; l++) {
         for (uint32_t t = threadIdx.x; t < tiles; t += kFusedThreads) {
             const uint32_t v = ffix[(uint64_t)l * tiles + t];

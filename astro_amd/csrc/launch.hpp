@@ -143,6 +143,16 @@ struct TreeArgs {
     uint32_t quirk_last_leaf;
     uint32_t segmented;         // 1: heads also at system starts
     uint32_t pack;              // 1: fewer than kLinkPackNodes nodes (links in the moment rows)
+    // Layers no walk reaches are not written (skip_ok: one system, monopoles, the traversal's
+    // fast form): the scan of the fused layers finds the lowest layer W a finite target's walk
+    // can reach from the opening test (acc, eps2, the cube) and stores it in *wlow (0: every
+    // layer written); W's nodes get kNextEnd as last child, so that no walk goes below them.
+    uint32_t skip_ok;
+    AcceptRule acc;
+    float eps2;
+    uint32_t side_shifted;      // ASTRO_SIDE_SHIFTED: side0 = 2^21
+    const float4* cubes;
+    uint32_t* wlow;             // [2]: W, and whether any tile left a node for k_fused_fix
 };
 
 void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s);
@@ -234,6 +244,7 @@ struct ForceArgs {
     unsigned long long* stats;  // 4 counters, or null
     double* phi;                // per-particle FP64 potential, or null
     const uint32_t* counts;     // [layers] valid nodes per layer (device)
+    const uint32_t* wlow;       // [1] the lowest layer the build wrote (the LDS top stays above)
     uint64_t off[kMaxLayers];   // layer offsets
     uint32_t layers;
     uint32_t top;               // 1: the top layers from LDS (blocks of 512 or 1,024 threads)

@@ -216,6 +216,17 @@ int astro_set_force_top_cache(astro_ctx* ctx, int on);
  * offsets, no links in the moment rows), so that path can be compared with the oracle on small
  * trees.  Takes effect at the next build.  No counterpart in the reference. */
 int astro_set_wide_offsets(astro_ctx* ctx, int on);
+/* With one system, monopoles and the traversal's fast form (eps2 >= the floors), a build does not
+ * write the tree layers that no walk can reach: those whose every node passes the opening test
+ * for any finite target (RN(sqrt(eps2)) >= rmin0 2^l).  The lowest written layer's nodes get
+ * 0xFFFFFFFF as last child (a walk of a non-finite target ends there).  Reading the tree
+ * (astro_get_buffer of ASTRO_BUF_TREE_POS, _TREE_LINKS, _TREE_MOMENTS, _OCTREE_NODES, or
+ * astro_barnes_hut) first builds the same tree again with every layer.  Default on; off writes
+ * every layer in every build.  No counterpart in the reference, which writes every layer. */
+int astro_set_skip_unreached_layers(astro_ctx* ctx, int on);
+/* The lowest tree layer the last build wrote (0: all) and whether layers are left out until the
+ * tree is read; synchronises the stream. */
+int astro_tree_info(astro_ctx* ctx, uint32_t* lowest_written, int* partial);
 /* Key sort of steps and builds: ASTRO_SORT_RADIX or ASTRO_SORT_SEEDED (see above).  Replaces
  * nothing in the reference, whose every build sorts from scratch. */
 int astro_set_sort_mode(astro_ctx* ctx, int mode);
