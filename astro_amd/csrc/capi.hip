@@ -1100,8 +1100,8 @@ static int enqueue_layers(astro_ctx* c, bool emit = true, bool full = false) {
     // layers 0 .. lf-1 (where nodes hold a few particles each) in the fused launches; the
     // reference quirk keeps the per-layer kernels
     uint32_t lf = 0;
-    if (!a.quirk_last_leaf && c->layers > 10)
-        lf = std::min(std::min(l0, c->layers - 10), kFusedMaxLayers);
+    if (!a.quirk_last_leaf && c->layers > 10)  // 13 at B = 21 (14 runs past the halo)
+        lf = std::min(std::min(l0, c->layers - 9), kFusedMaxLayers);
     if (lf < 2) lf = 0;
     // the next pointers of every layer's nodes, before any node is written
     launch_heads_next(a, lf, c->plevel, c->fused_buf, s);
