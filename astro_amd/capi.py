@@ -121,6 +121,7 @@ SIGNATURES = [
     ("astro_set_wide_offsets", _I, [_P, _I]),
     ("astro_set_sort_mode", _I, [_P, _I]),
     ("astro_set_skip_unreached_layers", _I, [_P, _I]),
+    ("astro_set_force_walk", _I, [_P, _I]),
     ("astro_tree_info", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_I)]),
     ("astro_sort_info", _I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_U32)]),
     ("astro_autotune", _I, [_P, _U32, ctypes.POINTER(_U32), ctypes.POINTER(_U32),
@@ -479,6 +480,10 @@ class Context:
         """True (default): a build leaves out the tree layers no walk can reach (written in full
         when the tree is read)."""
         self._chk(self.lib.astro_set_skip_unreached_layers(self.h, 1 if on else 0))
+
+    def set_force_walk(self, mode):
+        """0: one walk per target (default); 1: one walk per wave (same forces)."""
+        self._chk(self.lib.astro_set_force_walk(self.h, int(mode)))
 
     def tree_info(self):
         """(lowest layer the last build wrote, whether layers are left out until read)."""

@@ -869,6 +869,7 @@ static ForceArgs force_args(astro_ctx* c) {
     for (uint32_t l = 0; l < c->layers; l++) a.off[l] = c->off[l];
     a.layers = c->layers;
     a.top = c->force_top ? 1u : 0u;
+    a.walk = c->force_walk;
     a.wide = c->force_wide ? 1u : 0u;
     a.off0 = c->off[0];
     a.root_off = c->off[c->layers - 1];
@@ -1460,6 +1461,14 @@ extern "C" int astro_tree_info(astro_ctx* c, uint32_t* lowest_written, int* part
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (lowest_written) *lowest_written = c->tree_partial ? w : 0u;
     if (partial) *partial = c->tree_partial && w > 0 ? 1 : 0;
+    return ASTRO_OK;
+}
+
+extern "C" int astro_set_force_walk(astro_ctx* c, int mode) {
+    if (!c) return ASTRO_ERR_ARG;
+    if (mode != 0 && mode != 1) return fail(c, ASTRO_ERR_ARG, "walk must be 0 (per lane) or 1 (per wave)");
+    c->force_walk = (uint32_t)mode;
+    invalidate_graph(c);
     return ASTRO_OK;
 }
 

@@ -68,6 +68,7 @@ struct astro_ctx {
     float guard_extent = 0.0f;         // |coordinate| bound of the guard (0: finiteness only)
     bool force_top = true;             // traversal reads the top layers from LDS (blocks >= 512)
     bool force_wide = false;           // 64-bit node offsets and unpacked moment rows on any tree
+    uint32_t force_walk = 0;           // 1: one walk per wave (monopoles, one system)
     // the step's sort seeded by the previous order (one system): its previous order (a
     // permutation of 0..nt-1 written by the last sort of these sources), bucket slots,
     // splitters (from the last sort's rows), bucket counts (+ the overflow count)

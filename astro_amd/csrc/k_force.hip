@@ -617,6 +617,121 @@ uint64_t check_reciprocal(unsigned long long* d_mismatch, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? (uint64_t)(hi - lo) : 0u;
 }
 
+// One walk per wave (monopoles, one system, the fast opening test).  The wave visits the union
+// of its lanes' walks in the walks' order: it opens a node when any lane that takes part there
+// opens it, and a lane takes part at the nodes of its own walk only -- having accepted a node
+// the wave opens for another lane, it sits out that node's subtree, until the wave reaches the
+// node's next.  So every lane meets its own nodes in its own order, and its sums are the
+// per-lane walk's, bit for bit.  The node index is uniform, so nodes come through scalar loads
+// (the constant address space); a lane that takes no part in a visit adds fma(0, d, F), which
+// leaves F as it is (F never holds -0).
+typedef uint32_t cu4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) lds_f4* const_f4;
+typedef const __attribute__((address_space(4))) cu4* const_u4;
+template <bool FDIV, bool G1>
+__device__ __forceinline__ void walk_wave(const ForceArgs& a, float4 p, uint32_t self, float rmin0,
+                                          bool lane_on, Acc3& F) {
+    constexpr uint32_t kOn = 0xFFFFFFFEu;  // taking part (otherwise: the node to resume at)
+    const const_f4 tp = (const_f4)(uintptr_t)a.tpos;
+    const const_u4 tl = (const_u4)(uintptr_t)a.tlink;
+    const float eps2 = a.p.eps2, G = a.p.G;
+    uint32_t resume = lane_on ? kOn : kNextEnd;
+    uint32_t n = (uint32_t)a.root_off;
+    while (n != kNextEnd) {
+        n = __builtin_amdgcn_readfirstlane(n);
+        const lds_f4 cm = tp[n];
+        const cu4 lk = tl[n];  // last, info, next, start
+        const uint32_t layer = lk.y & 31u;
+        if (resume == n) resume = kOn;
+        const bool on = resume == kOn;
+        const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
+        const float rr = sqrt_rn_normal(r2);
+        const bool far = rr >= ldexpf(rmin0, (int)layer);
+        const bool leaf = layer == 0u;
+        const float gm = G1 ? cm.w : G * cm.w;
+        const float D = r2 * rr;
+        const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
+        const bool interact = on && (leaf ? n != self : far);
+        const float sk = interact ? s : 0.0f;
+        F.x = __builtin_fmaf(sk, dx, F.x);
+        F.y = __builtin_fmaf(sk, dy, F.y);
+        F.z = __builtin_fmaf(sk, dz, F.z);
+        const bool want = on && !leaf && !far;
+        if (__ballot(want) == 0ull) {
+            n = lk.z;
+        } else if (lk.x == kNextEnd) {
+            // a node of the lowest written layer (its children were left out): a lane that
+            // opens it ends its walk there, as its own walk would; the wave goes on
+            if (want) resume = kNextEnd;
+            n = lk.z;
+        } else {
+            if (on && !leaf && far) resume = lk.z;  // sit out this subtree
+            n = lk.x;
+        }
+    }
+}
+
+template <int NT, bool G1>
+__global__ __launch_bounds__(NT) void k_force_wave(ForceArgs a) {
+    // the targets' tiles as in k_force_compact
+    const uint32_t nb = gridDim.x, b = blockIdx.x, K = a.xcd_chunk;
+    uint32_t tile;
+    if (K == 0u) {
+        const uint32_t xcd = b & 7u, q = nb >> 3, r = nb & 7u;
+        tile = xcd * q + (xcd < r ? xcd : r) + (b >> 3);
+    } else {
+        const uint32_t full = (nb / (8u * K)) * 8u * K, i = b >> 3;
+        tile = b < full ? ((i / K) * 8u + (b & 7u)) * K + (i % K) : b;
+    }
+    const uint32_t j = tile * NT + threadIdx.x;
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(a.err, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) &
+        ERR_CAPACITY)
+        return;
+    const bool active = j < a.ns;
+    const float4 cu = a.cubes[0];
+    const float side0 = a.p.side_rule == ASTRO_SIDE_SHIFTED
+                            ? 2097152.0f
+                            : ldexpf(cu.w, -(int)a.p.bits_per_axis);
+    const float rmin0 = accept_rmin(side0, a.acc);
+    const float L = a.div_lmax;
+    const bool fdiv = L > 0.0f && fabsf(cu.x) <= L && fabsf(cu.y) <= L && fabsf(cu.z) <= L &&
+                      fabsf(cu.x + cu.w) <= L && fabsf(cu.y + cu.w) <= L &&
+                      fabsf(cu.z + cu.w) <= L;
+    uint32_t pidx = 0, self = 0xFFFFFFFFu;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (active) {
+        pidx = j < a.nt ? a.sorted_idx[j] : j;
+        const bool sorted = a.spos && j < a.nt;
+        p = sorted ? a.spos[j] : a.pos[pidx];
+        const uint32_t type = sorted ? a.stype[j] : a.type[pidx];
+        if (type != 2u && j < a.nt) self = (uint32_t)(a.off0 + a.leaf_of_sorted[j]);
+    }
+    Acc3 F;
+    if (fdiv) walk_wave<true, G1>(a, p, self, rmin0, active, F);
+    else walk_wave<false, G1>(a, p, self, rmin0, active, F);
+    if (active) {
+        const float c = a.p.force_clamp;
+        a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
+    }
+}
+
+// The wave walk applies: force computations only, one system, monopoles, the fast opening
+// test without Kahan sums, 32-bit node offsets.
+static bool wave_walk_applies(const ForceArgs& a, int mode) {
+    return a.walk && mode == 0 && !a.sys_of && a.p.multipole != 2 && a.fast && !a.p.kahan &&
+           !force_off64(a);
+}
+template <int NT>
+static void launch_force_wave(const ForceArgs& a, hipStream_t s) {
+    const uint32_t blocks = (a.ns + NT - 1) / NT;
+    if (a.p.G == 1.0f)
+        hipLaunchKernelGGL((k_force_wave<NT, true>), dim3(blocks), dim3(NT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_force_wave<NT, false>), dim3(blocks), dim3(NT), 0, s, a);
+}
+
 template <int NT, bool KAHAN, bool QUAD, bool FAST, bool OFF64, bool G1, bool TOPC>
 static void launch_force_modes(const ForceArgs& a, int mode, hipStream_t s) {
     const uint32_t blocks = (a.ns + NT - 1) / NT;
@@ -683,6 +798,16 @@ bool force_block_supported(uint32_t nt) {
 void launch_force_compact(const ForceArgs& a, hipStream_t s) {
     if (!a.ns) return;
     const int mode = a.phi ? 2 : (a.stats ? 1 : 0);
+    if (wave_walk_applies(a, mode)) {
+        switch (a.block) {
+            case 64: launch_force_wave<64>(a, s); break;
+            case 128: launch_force_wave<128>(a, s); break;
+            case 512: launch_force_wave<512>(a, s); break;
+            case 1024: launch_force_wave<1024>(a, s); break;
+            default: launch_force_wave<256>(a, s); break;
+        }
+        return;
+    }
     switch (a.block) {
         case 64: launch_force_nt<64>(a, mode, s); break;
         case 128: launch_force_nt<128>(a, mode, s); break;

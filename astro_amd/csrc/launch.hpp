@@ -249,6 +249,7 @@ struct ForceArgs {
     uint32_t layers;
     uint32_t top;               // 1: the top layers from LDS (blocks of 512 or 1,024 threads)
     uint32_t wide;              // 1: 64-bit node offsets on any tree
+    uint32_t walk;              // 1: one walk per wave (k_force_wave) where it applies
     uint64_t off0;              // leaf layer offset
     uint64_t root_off;          // top layer offset (root of system s at root_off + s)
     uint64_t node_total;        // nodes in the current layer table (index bound)

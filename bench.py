@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--sort", choices=["seeded", "radix"], default="seeded",
                     help="the step's key sort: seeded by the previous step's order (default) or "
                          "a radix sort from scratch (for A/B profiles); the same order either way")
+    ap.add_argument("--walk", choices=["lane", "wave"], default="lane",
+                    help="the traversal: one walk per target (default) or one walk per wave over "
+                         "the union of its targets' walks (for A/B profiles); the same forces")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=4)
@@ -467,6 +470,7 @@ def main():
     ctx.check()
     ctx.set_force_top_cache(a.top_cache)
     ctx.set_sort_mode(astro_amd.SORT_SEEDED if a.sort == "seeded" else astro_amd.SORT_RADIX)
+    ctx.set_force_walk(1 if a.walk == "wave" else 0)
     if a.force_block and a.xcd_chunk >= 0:
         tuned_block, tuned_chunk = a.force_block, a.xcd_chunk
         ctx.set_force_block(tuned_block)

@@ -216,6 +216,12 @@ int astro_set_force_top_cache(astro_ctx* ctx, int on);
  * offsets, no links in the moment rows), so that path can be compared with the oracle on small
  * trees.  Takes effect at the next build.  No counterpart in the reference. */
 int astro_set_wide_offsets(astro_ctx* ctx, int on);
+/* Traversal form: 0 (default) one walk per target; 1 one walk per wave, which visits the union of
+ * its 64 targets' walks in the walks' order with scalar node loads, each target taking part at
+ * its own walk's nodes only (same forces, bit for bit).  The second applies to one system with
+ * monopoles and the fast form of the opening test (force computations only); elsewhere the
+ * first runs.  No counterpart in the reference. */
+int astro_set_force_walk(astro_ctx* ctx, int mode);
 /* With one system, monopoles and the traversal's fast form (eps2 >= the floors), a build does not
  * write the tree layers that no walk can reach: those whose every node passes the opening test
  * for any finite target (RN(sqrt(eps2)) >= rmin0 2^l).  The lowest written layer's nodes get
