@@ -150,10 +150,13 @@ struct Visits {
 
 // The top of the tree in LDS.  A block copies the nodes of the top layers lt .. L-1 (the most
 // that fit, chosen from the layer counts) into LDS before its walks start, and rewrites their
-// links: a top node is named kTopBit | layer << 24 | slot, with its LDS slot in the low 12
-// bits, so that its layer is known before its data arrives.  Every target walks the top
-// layers, and its 64 lanes read scattered nodes there: from LDS those reads leave the texture
-// address unit, which bounds the walk.
+// links to name LDS slots (bit 31 set).  Every target walks the top layers, and its 64 lanes
+// read scattered nodes there: from LDS those reads leave the texture address unit.
+//   Monopoles: a top node is named kTopBit | slot << 4, the byte offset of its rows in the
+//     two LDS arrays (centre and mass; last', info, next'), so that a visit reads either
+//     source into the same registers with one address operation.
+//   Quadrupoles: a top node is named kTopBit | layer << 24 | slot, so that its layer is known
+//     before its three rows arrive.
 //   last' of a top node of layer l > lt names its last child (a top node of layer l - 1); a
 //     node of layer lt keeps its child's global index (bit 31 clear); a top leaf (lt = 0) is
 //     never opened.
@@ -164,12 +167,17 @@ struct Visits {
 // next' (`resume`) and follows it wherever a global next points at or above off[lt].
 constexpr uint32_t kTopBit = 0x80000000u;
 constexpr uint32_t kTopSlotMask = 0x00000FFFu;
-// slots per block: 80 KB of LDS per 1,024 threads (two blocks per CU), 24 B per node for
-// monopoles (centre and mass, last', next'), 48 B with the moment rows
+constexpr uint32_t kTopAddrMask = 0x0000FFFFu;
+// slots per block: 80 KB of LDS per 1,024 threads (two blocks per CU), 32 B per node for
+// monopoles (centre and mass; last', info, next', one free word), 48 B with the moment rows
 template <int NT, bool QUAD>
 struct TopCap {
-    static constexpr uint32_t kSlots = (QUAD ? 1664u : 3328u) * (uint32_t)NT / 1024u;
+    static constexpr uint32_t kSlots = (QUAD ? 1664u : 2544u) * (uint32_t)NT / 1024u;
 };
+template <bool QUAD>
+__device__ inline uint32_t top_name(uint32_t layer, uint32_t slot) {
+    return QUAD ? (kTopBit | (layer << 24) | slot) : (kTopBit | (slot << 4));
+}
 // LDS pointers keep their address space, so that the walk's LDS and global reads stay
 // separate ds_read and global_load instructions (a generic pointer would turn both into flat
 // loads)
@@ -177,13 +185,16 @@ struct TopCap {
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
 typedef float lds_f2 __attribute__((ext_vector_type(2)));
 typedef uint32_t lds_u2 __attribute__((ext_vector_type(2)));
+typedef uint32_t lds_u3 __attribute__((ext_vector_type(3)));
 struct TopView {
     const ASTRO_LDS lds_f4* pos;  // [slots] centre and mass
-    const ASTRO_LDS lds_u2* link; // [slots] last', next'
+    const ASTRO_LDS lds_u2* link; // [slots] last', next' (quadrupoles)
+    const ASTRO_LDS uint4* lnk;   // [slots] last', info, next', 0 (monopoles)
     const ASTRO_LDS lds_f4* q0;   // [slots] Sxx, Syy, Szz, Sxy (quadrupoles)
     const ASTRO_LDS lds_f2* q1;   // [slots] Sxz, Syz
     uint32_t lt;         // first cached layer (L: none)
-    uint32_t top_lo;     // off[lt]: global indices at or above it are top nodes (0xFFFFFFFF: none)
+    uint32_t top_lo;     // off[lt]: global indices at or above it are top nodes (none:
+                         // 0xFFFFFFFF, or 0x7FFFFFFF for the monopoles' signed compare)
 };
 
 // One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
@@ -221,6 +232,35 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
     while (true) {
         float4 cm, qm0, qm1;
         uint32_t last, next, layer;
+        if constexpr (TOPC && !QUAD) {
+            // monopoles: one source or the other into the same registers; the LDS reads are
+            // issued first, so that the global loads wait at most for them (not the reverse)
+            uint32_t info;
+            if ((int32_t)n >= 0) {
+                const uint32_t o = n << 4;
+                cm = *reinterpret_cast<const float4*>(pos_base + o);
+                const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
+                last = li.last;
+                info = li.info;
+                next = li.next;
+            } else {
+                const uint32_t ad = n & kTopAddrMask;
+                const lds_f4 c = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.pos) + ad);
+                const lds_u3 k = *reinterpret_cast<const ASTRO_LDS lds_u3*>(
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.lnk) + ad);
+                cm = make_float4(c.x, c.y, c.z, c.w);
+                last = k.x;
+                info = k.y;
+                next = k.z;
+            }
+            layer = tn_layer(info);
+            // a global node's next at or above off[lt] is a top node: back to the top through
+            // the next' of the node of layer lt this lane opened last.  Top nodes' next' and
+            // kNextEnd are negative as int32 and never match (a global node with next kNextEnd
+            // lies on the root's first-child chain, where resume is kNextEnd too).
+            if ((int32_t)next >= (int32_t)tv.top_lo) next = resume;
+        } else {
         const bool top = TOPC && (n & kTopBit);
         lds_f4 tcm, tq0;
         lds_f2 tq1;
@@ -273,6 +313,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             layer = (n >> 24) & 31u;
         } else if (TOPC && next >= tv.top_lo) {
             next = resume;  // back to the top: the next of the opened node of layer lt
+        }
         }
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
@@ -360,8 +401,8 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
 // up (base[l]).  Thread 0 decides, every thread then copies its share of the nodes with their
 // links rewritten.  Returns the view the walks read.
 template <int NT, bool QUAD>
-__device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, float4* s_q0,
-                            float2* s_q1, uint32_t* s_base, uint32_t* s_meta) {
+__device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, uint4* s_lnk,
+                            float4* s_q0, float2* s_q1, uint32_t* s_base, uint32_t* s_meta) {
     constexpr uint32_t slots = TopCap<NT, QUAD>::kSlots;
     const uint32_t L = a.layers, tid = threadIdx.x;
     if (tid == 0) {
@@ -391,27 +432,30 @@ __device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, fl
         const TLink k = a.tlink[n];
         s_pos[s] = a.tpos[n];
         uint32_t last = k.last, next = k.next;
-        if (l > lt) last = kTopBit | ((l - 1u) << 24) | (last - (uint32_t)a.off[l - 1] + s_base[l - 1]);
+        if (l > lt) last = top_name<QUAD>(l - 1u, last - (uint32_t)a.off[l - 1] + s_base[l - 1]);
         else if (l == 0u) last = kNextEnd;  // a leaf: never opened
         if (next != kNextEnd) {
             const uint32_t h = tn_level(k.info);  // the layer of the next node
-            next = kTopBit | (h << 24) | (next - (uint32_t)a.off[h] + s_base[h]);
+            next = top_name<QUAD>(h, next - (uint32_t)a.off[h] + s_base[h]);
         }
-        s_link[s] = make_uint2(last, next);
         if (QUAD) {
+            s_link[s] = make_uint2(last, next);
             s_q0[s] = a.qmom[2 * (size_t)n];
             const float4 q = a.qmom[2 * (size_t)n + 1];
             s_q1[s] = make_float2(q.x, q.y);
+        } else {
+            s_lnk[s] = make_uint4(last, k.info, next, 0u);
         }
     }
     __syncthreads();
     TopView tv;
     tv.pos = (const ASTRO_LDS lds_f4*)s_pos;
     tv.link = (const ASTRO_LDS lds_u2*)s_link;
+    tv.lnk = (const ASTRO_LDS uint4*)s_lnk;
     tv.q0 = (const ASTRO_LDS lds_f4*)s_q0;
     tv.q1 = (const ASTRO_LDS lds_f2*)s_q1;
     tv.lt = lt;
-    tv.top_lo = lt < L ? (uint32_t)a.off[lt] : 0xFFFFFFFFu;
+    tv.top_lo = lt < L ? (uint32_t)a.off[lt] : (QUAD ? 0xFFFFFFFFu : 0x7FFFFFFFu);
     return tv;
 }
 
@@ -422,7 +466,8 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     static_assert(!(TOPC && OFF64), "the LDS top needs 32-bit node offsets");
     constexpr uint32_t kSl = TOPC ? TopCap<NT, QUAD>::kSlots : 1u;
     __shared__ float4 s_pos[kSl];
-    __shared__ uint2 s_link[kSl];
+    __shared__ uint2 s_link[TOPC && QUAD ? kSl : 1u];
+    __shared__ uint4 s_lnk[TOPC && !QUAD ? kSl : 1u];
     __shared__ float4 s_q0[TOPC && QUAD ? kSl : 1u];
     __shared__ float2 s_q1[TOPC && QUAD ? kSl : 1u];
     __shared__ uint32_t s_base[kMaxLayers];
@@ -449,7 +494,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         ERR_CAPACITY)
         return;
     TopView tv{};
-    if (TOPC) tv = fill_top<NT, QUAD>(a, s_pos, s_link, s_q0, s_q1, s_base, s_meta);
+    if (TOPC) tv = fill_top<NT, QUAD>(a, s_pos, s_link, s_lnk, s_q0, s_q1, s_base, s_meta);
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
@@ -482,9 +527,9 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         uint32_t root = (uint32_t)(a.root_off + sys);
         if (TOPC && tv.lt < a.layers) {
             // the roots are top nodes; so are the leaves when the whole tree is cached
-            root = kTopBit | ((a.layers - 1u) << 24) | (s_base[a.layers - 1u] + sys);
+            root = top_name<QUAD>(a.layers - 1u, s_base[a.layers - 1u] + sys);
             if (tv.lt == 0u && self != 0xFFFFFFFFu)
-                self = kTopBit | (self - (uint32_t)a.off0 + s_base[0]);
+                self = top_name<QUAD>(0u, self - (uint32_t)a.off0 + s_base[0]);
         }
         Acc3 F;
         double phi = 0.0;
