@@ -152,11 +152,12 @@ struct Visits {
 // that fit, chosen from the layer counts) into LDS before its walks start, and rewrites their
 // links to name LDS slots (bit 31 set).  Every target walks the top layers, and its 64 lanes
 // read scattered nodes there: from LDS those reads leave the texture address unit.
-//   Monopoles: a top node is named kTopBit | slot << 4, the byte offset of its rows in the
-//     two LDS arrays (centre and mass; last', info, next'), so that a visit reads either
-//     source into the same registers with one address operation.
-//   Quadrupoles: a top node is named kTopBit | layer << 24 | slot, so that its layer is known
-//     before its three rows arrive.
+//   A top node is named by the byte offset of its rows in the LDS arrays (slot << 4) with a
+//     marker bit that no global index has: bit 31 for monopoles (rows: centre and mass;
+//     last', info, next'), bit 26 with quadrupoles (trees of fewer than 2^26 nodes; rows:
+//     centre and mass, Sxx Syy Szz Sxy, and Sxz Syz (last' | layer << 27) next', the global
+//     moment rows' layout).  A visit reads either source into the same registers with one
+//     address operation and decodes both alike.
 //   last' of a top node of layer l > lt names its last child (a top node of layer l - 1); a
 //     node of layer lt keeps its child's global index (bit 31 clear); a top leaf (lt = 0) is
 //     never opened.
@@ -165,8 +166,8 @@ struct Visits {
 // A lane leaves the top only by opening a node P of layer lt, and returns to it only from
 // P's first-child chain, whose global next links all equal P's next: so a lane keeps P's
 // next' (`resume`) and follows it wherever a global next points at or above off[lt].
-constexpr uint32_t kTopBit = 0x80000000u;
-constexpr uint32_t kTopSlotMask = 0x00000FFFu;
+constexpr uint32_t kTopBit = 0x80000000u;   // monopoles
+constexpr uint32_t kTopQuad = 0x04000000u;  // quadrupoles (global indices below 2^26)
 constexpr uint32_t kTopAddrMask = 0x0000FFFFu;
 // slots per block: 80 KB of LDS per 1,024 threads (two blocks per CU), 32 B per node for
 // monopoles (centre and mass; last', info, next', one free word), 48 B with the moment rows
@@ -175,26 +176,23 @@ struct TopCap {
     static constexpr uint32_t kSlots = (QUAD ? 1664u : 2544u) * (uint32_t)NT / 1024u;
 };
 template <bool QUAD>
-__device__ inline uint32_t top_name(uint32_t layer, uint32_t slot) {
-    return QUAD ? (kTopBit | (layer << 24) | slot) : (kTopBit | (slot << 4));
+__device__ inline uint32_t top_name(uint32_t slot) {
+    return (QUAD ? kTopQuad : kTopBit) | (slot << 4);
 }
 // LDS pointers keep their address space, so that the walk's LDS and global reads stay
 // separate ds_read and global_load instructions (a generic pointer would turn both into flat
 // loads)
 #define ASTRO_LDS __attribute__((address_space(3)))
 typedef float lds_f4 __attribute__((ext_vector_type(4)));
-typedef float lds_f2 __attribute__((ext_vector_type(2)));
-typedef uint32_t lds_u2 __attribute__((ext_vector_type(2)));
 typedef uint32_t lds_u3 __attribute__((ext_vector_type(3)));
 struct TopView {
     const ASTRO_LDS lds_f4* pos;  // [slots] centre and mass
-    const ASTRO_LDS lds_u2* link; // [slots] last', next' (quadrupoles)
     const ASTRO_LDS uint4* lnk;   // [slots] last', info, next', 0 (monopoles)
     const ASTRO_LDS lds_f4* q0;   // [slots] Sxx, Syy, Szz, Sxy (quadrupoles)
-    const ASTRO_LDS lds_f2* q1;   // [slots] Sxz, Syz
+    const ASTRO_LDS lds_f4* q1;   // [slots] Sxz, Syz, last' | layer << 27, next' (quadrupoles)
     uint32_t lt;         // first cached layer (L: none)
     uint32_t top_lo;     // off[lt]: global indices at or above it are top nodes (none:
-                         // 0xFFFFFFFF, or 0x7FFFFFFF for the monopoles' signed compare)
+                         // 0x7FFFFFFF for monopoles, kTopQuad for quadrupoles)
 };
 
 // One target's traversal.  MODE: 0 forces, 1 forces + visit counters, 2 forces + FP64
@@ -260,14 +258,39 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             // kNextEnd are negative as int32 and never match (a global node with next kNextEnd
             // lies on the root's first-child chain, where resume is kNextEnd too).
             if ((int32_t)next >= (int32_t)tv.top_lo) next = resume;
+        } else if constexpr (TOPC && QUAD) {
+            // quadrupoles: the same, three rows from either source; a top node's rows carry
+            // the links packed as in the global moment rows
+            if ((int32_t)(n << 5) >= 0) {  // bit 26 clear: a global node
+                cm = *reinterpret_cast<const float4*>(pos_base + (n << 4));
+                const float4* qp = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const char*>(a.qmom) + (n << 5));
+                qm0 = qp[0];
+                qm1 = qp[1];
+            } else {
+                const uint32_t ad = n & kTopAddrMask;
+                const lds_f4 c = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.pos) + ad);
+                const lds_f4 u = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.q0) + ad);
+                const lds_f4 v = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.q1) + ad);
+                cm = make_float4(c.x, c.y, c.z, c.w);
+                qm0 = make_float4(u.x, u.y, u.z, u.w);
+                qm1 = make_float4(v.x, v.y, v.z, v.w);
+            }
+            const uint32_t w = __float_as_uint(qm1.z);
+            last = w & kPackedIndexMask;
+            layer = w >> 27;
+            next = __float_as_uint(qm1.w);
+            // a global node's next in [off[lt], 2^26) is a top node: back to the top through
+            // the next' of the node of layer lt this lane opened last.  Top nodes' next' lie
+            // at or above 2^26; kNextEnd is kept (a global node with next kNextEnd lies on the
+            // root's first-child chain, where resume is kNextEnd too).
+            if (next - tv.top_lo < kTopQuad - tv.top_lo) next = resume;
         } else {
-        const bool top = TOPC && (n & kTopBit);
-        lds_f4 tcm, tq0;
-        lds_f2 tq1;
-        lds_u2 tlk;
-        if (!top) {
-            // one byte offset (16 B per node) into both arrays, from uniform bases: 32-bit
-            // while the tree has fewer than 2^27 nodes
+            // every node from device memory: one byte offset (16 B per node) into both
+            // arrays, from uniform bases; 32-bit while the tree has fewer than 2^27 nodes
             const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
             cm = *reinterpret_cast<const float4*>(pos_base + o);
             // quadrupole mode: the moments are loaded with the node, not after the opening
@@ -293,28 +316,6 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
                 next = li.next;
             }
         }
-        if (top) {  // issued after the global loads above; both are waited for together
-            const uint32_t s = n & kTopSlotMask;
-            tcm = tv.pos[s];
-            tlk = tv.link[s];
-            if (QUAD) {
-                tq0 = tv.q0[s];
-                tq1 = tv.q1[s];
-            }
-        }
-        if (top) {
-            cm = make_float4(tcm.x, tcm.y, tcm.z, tcm.w);
-            if (QUAD) {
-                qm0 = make_float4(tq0.x, tq0.y, tq0.z, tq0.w);
-                qm1 = make_float4(tq1.x, tq1.y, 0.0f, 0.0f);
-            }
-            last = tlk.x;
-            next = tlk.y;
-            layer = (n >> 24) & 31u;
-        } else if (TOPC && next >= tv.top_lo) {
-            next = resume;  // back to the top: the next of the opened node of layer lt
-        }
-        }
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
         if (!FAST) r2 = fmaxf(r2, r2f);
@@ -328,7 +329,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             // second memory round trip on the loop's critical path.
             __asm__ volatile("" : : "v"(gm));
             const bool far = r2 >= ldexpf(r2min0, 2 * (int)layer);
-            const bool interact = leaf ? n != self : far;
+            const bool interact = (leaf & (n != self)) | (!leaf & far);
             open = !leaf & !far;
             if (interact) {
                 const float rr = sqrt_rn_normal(r2);
@@ -401,8 +402,8 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
 // up (base[l]).  Thread 0 decides, every thread then copies its share of the nodes with their
 // links rewritten.  Returns the view the walks read.
 template <int NT, bool QUAD>
-__device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, uint4* s_lnk,
-                            float4* s_q0, float2* s_q1, uint32_t* s_base, uint32_t* s_meta) {
+__device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint4* s_lnk, float4* s_q0,
+                            float4* s_q1, uint32_t* s_base, uint32_t* s_meta) {
     constexpr uint32_t slots = TopCap<NT, QUAD>::kSlots;
     const uint32_t L = a.layers, tid = threadIdx.x;
     if (tid == 0) {
@@ -432,17 +433,17 @@ __device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, ui
         const TLink k = a.tlink[n];
         s_pos[s] = a.tpos[n];
         uint32_t last = k.last, next = k.next;
-        if (l > lt) last = top_name<QUAD>(l - 1u, last - (uint32_t)a.off[l - 1] + s_base[l - 1]);
-        else if (l == 0u) last = kNextEnd;  // a leaf: never opened
+        if (l > lt) last = top_name<QUAD>(last - (uint32_t)a.off[l - 1] + s_base[l - 1]);
+        else if (l == 0u) last = QUAD ? kPackedNoChild : kNextEnd;  // a leaf: never opened
         if (next != kNextEnd) {
             const uint32_t h = tn_level(k.info);  // the layer of the next node
-            next = top_name<QUAD>(h, next - (uint32_t)a.off[h] + s_base[h]);
+            next = top_name<QUAD>(next - (uint32_t)a.off[h] + s_base[h]);
         }
         if (QUAD) {
-            s_link[s] = make_uint2(last, next);
             s_q0[s] = a.qmom[2 * (size_t)n];
             const float4 q = a.qmom[2 * (size_t)n + 1];
-            s_q1[s] = make_float2(q.x, q.y);
+            s_q1[s] = make_float4(q.x, q.y, __uint_as_float(last | (l << 27)),
+                                  __uint_as_float(next));
         } else {
             s_lnk[s] = make_uint4(last, k.info, next, 0u);
         }
@@ -450,12 +451,11 @@ __device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint2* s_link, ui
     __syncthreads();
     TopView tv;
     tv.pos = (const ASTRO_LDS lds_f4*)s_pos;
-    tv.link = (const ASTRO_LDS lds_u2*)s_link;
     tv.lnk = (const ASTRO_LDS uint4*)s_lnk;
     tv.q0 = (const ASTRO_LDS lds_f4*)s_q0;
-    tv.q1 = (const ASTRO_LDS lds_f2*)s_q1;
+    tv.q1 = (const ASTRO_LDS lds_f4*)s_q1;
     tv.lt = lt;
-    tv.top_lo = lt < L ? (uint32_t)a.off[lt] : (QUAD ? 0xFFFFFFFFu : 0x7FFFFFFFu);
+    tv.top_lo = lt < L ? (uint32_t)a.off[lt] : (QUAD ? kTopQuad : 0x7FFFFFFFu);
     return tv;
 }
 
@@ -466,10 +466,9 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     static_assert(!(TOPC && OFF64), "the LDS top needs 32-bit node offsets");
     constexpr uint32_t kSl = TOPC ? TopCap<NT, QUAD>::kSlots : 1u;
     __shared__ float4 s_pos[kSl];
-    __shared__ uint2 s_link[TOPC && QUAD ? kSl : 1u];
     __shared__ uint4 s_lnk[TOPC && !QUAD ? kSl : 1u];
     __shared__ float4 s_q0[TOPC && QUAD ? kSl : 1u];
-    __shared__ float2 s_q1[TOPC && QUAD ? kSl : 1u];
+    __shared__ float4 s_q1[TOPC && QUAD ? kSl : 1u];
     __shared__ uint32_t s_base[kMaxLayers];
     __shared__ uint32_t s_meta[2];
     // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
@@ -494,7 +493,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         ERR_CAPACITY)
         return;
     TopView tv{};
-    if (TOPC) tv = fill_top<NT, QUAD>(a, s_pos, s_link, s_lnk, s_q0, s_q1, s_base, s_meta);
+    if (TOPC) tv = fill_top<NT, QUAD>(a, s_pos, s_lnk, s_q0, s_q1, s_base, s_meta);
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
@@ -527,9 +526,9 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         uint32_t root = (uint32_t)(a.root_off + sys);
         if (TOPC && tv.lt < a.layers) {
             // the roots are top nodes; so are the leaves when the whole tree is cached
-            root = top_name<QUAD>(a.layers - 1u, s_base[a.layers - 1u] + sys);
+            root = top_name<QUAD>(s_base[a.layers - 1u] + sys);
             if (tv.lt == 0u && self != 0xFFFFFFFFu)
-                self = top_name<QUAD>(0u, self - (uint32_t)a.off0 + s_base[0]);
+                self = top_name<QUAD>(self - (uint32_t)a.off0 + s_base[0]);
         }
         Acc3 F;
         double phi = 0.0;
@@ -798,9 +797,11 @@ bool force_off64(const ForceArgs& a) { return a.node_total >= kLinkPackNodes || 
 template <int NT, bool KAHAN, bool QUAD, bool FAST, bool G1>
 static void launch_force_off(const ForceArgs& a, int mode, hipStream_t s) {
     constexpr bool kTopBlock = NT >= 512;  // the LDS top needs many waves per block
+    // quadrupole top nodes are marked by bit 26, which global indices below 2^26 leave clear
+    const bool top_fits = !QUAD || a.node_total < (uint64_t)kTopQuad;
     if (force_off64(a)) {
         launch_force_modes<NT, KAHAN, QUAD, FAST, true, G1, false>(a, mode, s);
-    } else if (kTopBlock && a.top) {
+    } else if (kTopBlock && a.top && top_fits) {
         launch_force_modes<NT, KAHAN, QUAD, FAST, false, G1, kTopBlock>(a, mode, s);
     } else {
         launch_force_modes<NT, KAHAN, QUAD, FAST, false, G1, false>(a, mode, s);
