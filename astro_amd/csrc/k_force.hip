@@ -46,6 +46,33 @@ __device__ inline void accumulate(Acc3& F, float s, float dx, float dy, float dz
     }
 }
 
+// Lane masks: a condition's lanes as a wave-uniform 64-bit mask (SGPRs), and back.  Boolean
+// combinations of conditions then run as scalar mask operations, where the compiler would
+// otherwise materialise each condition in a vector register and select between them.
+__device__ inline uint64_t lanes_of(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+__device__ inline bool lane_in(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// RN(1 / d) for d in [2^-45, 2^46): the hardware reciprocal refined by one Newton step.  The
+// raw estimate differs from the correctly rounded reciprocal on about 11% of that range, the
+// refined one on none (k_check_reciprocal checks every float of it).
+__device__ inline float rcp_rn_mid(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+}
+
+// q = RN(n / d) for d and n in [2^-45, 2^45] (n may also be 0), by Markstein's theorem: with
+// y = RN(1/d), q0 = RN(n y) lies within one ulp of n / d, the residual n - d q0 is exact
+// (one fma), and RN(q0 + residual y) = RN(n / d).  This replaces the compiler's IEEE division
+// sequence (operand scaling, two remainder corrections, special-case fix-up), which changes
+// nothing on that domain.
+__device__ inline float div_rn_mid(float n, float d) {
+    const float y = rcp_rn_mid(d);
+    const float q0 = n * y;
+    const float t = __builtin_fmaf(-d, q0, n);
+    return __builtin_fmaf(t, y, q0);
+}
+
 // Quadrupole term of an accepted internal node with second moments S (xx, yy, zz, xy, xz, yz)
 // about its centre of mass, d = COM - target, D = r2 * r:
 //   a_Q = u * q,  u = 3G v,  q = w d - (S d + T/2 d),  w = 2.5 (d.S.d) (v D),  T = tr S,
@@ -55,8 +82,9 @@ struct QuadTerm {
     float u, q0, q1, q2;
     double phi;  // -G (1.5 d.S.d - 0.5 T r2) / (D r2), for the potential diagnostics
 };
+template <bool QRCP>
 __device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, float dz, float r2,
-                                     float D, float G) {
+                                     float D, float G, bool qrcp) {
     const float Sxx = s0.x, Syy = s0.y, Szz = s0.z, Sxy = s0.w, Sxz = s1.x, Syz = s1.y;
     const float Sd0 = __builtin_fmaf(Sxz, dz, __builtin_fmaf(Sxy, dy, Sxx * dx));
     const float Sd1 = __builtin_fmaf(Syz, dz, __builtin_fmaf(Syy, dy, Sxy * dx));
@@ -64,7 +92,9 @@ __device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, f
     const float dSd = __builtin_fmaf(dz, Sd2, __builtin_fmaf(dy, Sd1, dx * Sd0));
     const float T = (Sxx + Syy) + Szz;
     const float ht = 0.5f * T;
-    const float v = 1.0f / (D * r2);
+    // qrcp: D r2 lies in [2^-45, 2^46), where rcp_rn_mid is RN(1 / x) (wave-uniform)
+    const float x = D * r2;
+    const float v = (QRCP && qrcp) ? rcp_rn_mid(x) : 1.0f / x;
     const float w = (2.5f * dSd) * (v * D);
     QuadTerm t;
     t.u = (3.0f * G) * v;
@@ -121,27 +151,6 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
-}
-
-// RN(1 / d) for d in [2^-45, 2^46): the hardware reciprocal refined by one Newton step.  The
-// raw estimate differs from the correctly rounded reciprocal on about 11% of that range, the
-// refined one on none (k_check_reciprocal checks every float of it).
-__device__ inline float rcp_rn_mid(float d) {
-    const float r = __builtin_amdgcn_rcpf(d);
-    const float e = __builtin_fmaf(-d, r, 1.0f);
-    return __builtin_fmaf(e, r, r);
-}
-
-// q = RN(n / d) for d and n in [2^-45, 2^45] (n may also be 0), by Markstein's theorem: with
-// y = RN(1/d), q0 = RN(n y) lies within one ulp of n / d, the residual n - d q0 is exact
-// (one fma), and RN(q0 + residual y) = RN(n / d).  This replaces the compiler's IEEE division
-// sequence (operand scaling, two remainder corrections, special-case fix-up), which changes
-// nothing on that domain.
-__device__ inline float div_rn_mid(float n, float d) {
-    const float y = rcp_rn_mid(d);
-    const float q0 = n * y;
-    const float t = __builtin_fmaf(-d, q0, n);
-    return __builtin_fmaf(t, y, q0);
 }
 
 struct Visits {
@@ -216,7 +225,7 @@ template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64, boo
           bool TOPC>
 __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, uint32_t root,
                                          float4 p, uint32_t self, float rmin0, float r2min0,
-                                         Acc3& F, double& phi, Visits& st) {
+                                         bool qrcp, Acc3& F, double& phi, Visits& st) {
     constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
@@ -328,20 +337,39 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             // keeps the mass in the node's one wide load: sunk into the branch it would be a
             // second memory round trip on the loop's critical path.
             __asm__ volatile("" : : "v"(gm));
-            const bool far = r2 >= ldexpf(r2min0, 2 * (int)layer);
-            const bool interact = (leaf & (n != self)) | (!leaf & far);
-            open = !leaf & !far;
+            const uint64_t mleaf = lanes_of(leaf), mfar = lanes_of(r2 >= ldexpf(r2min0, 2 * (int)layer));
+            const uint64_t mself = lanes_of(n != self);
+            const bool interact = lane_in((mleaf & mself) | (~mleaf & mfar));
+            open = lane_in(~mleaf & ~mfar);
             if (interact) {
                 const float rr = sqrt_rn_normal(r2);
                 const float D = r2 * rr;
                 const float s = div_rn_mid(gm, D);
-                if (QUAD && !leaf) {
-                    const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, D, G);
-                    accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
-                    if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+                if (KAHAN) {
+                    if (QUAD && !leaf) {
+                        const QuadTerm qt = quad_term<true>(qm0, qm1, dx, dy, dz, r2, D, G, qrcp);
+                        accumulate_quad<true>(F, s, dx, dy, dz, qt);
+                        if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+                    } else {
+                        accumulate<true>(F, s, dx, dy, dz);
+                        if (MODE == 2) phi -= (double)gm / (double)rr;
+                    }
                 } else {
-                    accumulate<KAHAN>(F, s, dx, dy, dz);
-                    if (MODE == 2) phi -= (double)gm / (double)rr;
+                    // the monopole sum first (all of a leaf's term), the quadrupole term on it
+                    float fx = __builtin_fmaf(s, dx, F.x), fy = __builtin_fmaf(s, dy, F.y),
+                          fz = __builtin_fmaf(s, dz, F.z);
+                    if (QUAD && !leaf) {
+                        const QuadTerm qt = quad_term<true>(qm0, qm1, dx, dy, dz, r2, D, G, qrcp);
+                        fx = __builtin_fmaf(qt.u, qt.q0, fx);
+                        fy = __builtin_fmaf(qt.u, qt.q1, fy);
+                        fz = __builtin_fmaf(qt.u, qt.q2, fz);
+                        if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+                    } else if (MODE == 2) {
+                        phi -= (double)gm / (double)rr;
+                    }
+                    F.x = fx;
+                    F.y = fy;
+                    F.z = fz;
                 }
             }
             if (MODE == 1) {
@@ -352,16 +380,16 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             }
         } else {
             const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
+            // (plain selects here: mask operations measured slower on this path)
             const bool far = rr >= ldexpf(rmin0, (int)layer);
             const bool hit = (n != self) & (FAST || r2 > cut);
-            const bool acc = far;
-            const bool interact = leaf ? hit : acc;
+            const bool interact = leaf ? hit : far;
             open = !leaf & !far;
             // the term is computed for every lane and kept only where it interacts
             const float D = r2 * rr;
             const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
             if (QUAD && interact && !leaf) {
-                const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, D, G);
+                const QuadTerm qt = quad_term<false>(qm0, qm1, dx, dy, dz, r2, D, G, false);
                 accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
                 if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
             } else if (KAHAN) {
@@ -375,8 +403,8 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
                 F.cz = interact ? T.cz : F.cz;
             } else {
                 // F never holds -0 (it starts at +0, and x + (-x) rounds to +0), so the zero
-                // term fma(0, d, F) leaves it unchanged for finite d; the selects take lane masks
-                const float s_hit = hit ? s : 0.0f, s_acc = acc ? s : 0.0f;
+                // term fma(0, d, F) leaves it unchanged for finite d
+                const float s_hit = hit ? s : 0.0f, s_acc = far ? s : 0.0f;
                 const float sk = leaf ? s_hit : s_acc;
                 F.x = __builtin_fmaf(sk, dx, F.x);
                 F.y = __builtin_fmaf(sk, dy, F.y);
@@ -497,10 +525,11 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
-    // +-div_lmax (then |d| <= 2 div_lmax per axis; masses are bounded on the host)
+    // +-div_lmax (then |d| <= 2 div_lmax per axis; masses are bounded on the host); the
+    // quadrupole term's fast reciprocal when they lie within +-qrcp_lmax
     // a node at layer l is accepted iff r >= rmin0 * 2^l (accept.hpp)
     float rmin0 = 0.0f;
-    bool fdiv_lane = true;
+    bool fdiv_lane = true, qrcp_lane = true;
     if (active) {
         const float4 cu = a.cubes[sys];
         float side0;
@@ -511,8 +540,15 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         fdiv_lane = L > 0.0f && fabsf(cu.x) <= L && fabsf(cu.y) <= L && fabsf(cu.z) <= L &&
                     fabsf(cu.x + cu.w) <= L && fabsf(cu.y + cu.w) <= L && fabsf(cu.z + cu.w) <= L;
         if (FAST && QUAD) fdiv_lane = fdiv_lane && accept_r2_ok(rmin0);
+        if (QUAD) {
+            const float Q = a.qrcp_lmax;
+            qrcp_lane = Q > 0.0f && fabsf(cu.x) <= Q && fabsf(cu.y) <= Q && fabsf(cu.z) <= Q &&
+                        fabsf(cu.x + cu.w) <= Q && fabsf(cu.y + cu.w) <= Q &&
+                        fabsf(cu.z + cu.w) <= Q;
+        }
     }
     const bool fdiv = __builtin_amdgcn_readfirstlane(__all(fdiv_lane ? 1 : 0)) != 0;
+    const bool qrcp = QUAD && __builtin_amdgcn_readfirstlane(__all(qrcp_lane ? 1 : 0)) != 0;
 
     Visits st;
     if (active) {
@@ -534,10 +570,11 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         double phi = 0.0;
         if (fdiv)
             traverse<KAHAN, MODE, QUAD, FAST, true, OFF64, G1, TOPC>(
-                a, tv, root, p, self, rmin0, QUAD ? accept_r2min(rmin0) : 0.0f, F, phi, st);
+                a, tv, root, p, self, rmin0, QUAD ? accept_r2min(rmin0) : 0.0f, qrcp, F, phi,
+                st);
         else
             traverse<KAHAN, MODE, QUAD, FAST, false, OFF64, G1, TOPC>(a, tv, root, p, self, rmin0,
-                                                                      0.0f, F, phi, st);
+                                                                      0.0f, false, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
@@ -921,7 +958,7 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
                 // second moments sit in emittedColor (xx, yy, zz, xy) / emittedColorCenter
                 const float4 s0 = *reinterpret_cast<const float4*>(nd->emittedColor);
                 const float4 s1 = *reinterpret_cast<const float4*>(nd->emittedColorCenter);
-                const QuadTerm qt = quad_term(s0, s1, dx, dy, dz, r2, D, a.p.G);
+                const QuadTerm qt = quad_term<false>(s0, s1, dx, dy, dz, r2, D, a.p.G, false);
                 if (a.p.kahan) accumulate_quad<true>(F, s, dx, dy, dz, qt);
                 else accumulate_quad<false>(F, s, dx, dy, dz, qt);
             } else if (a.p.kahan) {
