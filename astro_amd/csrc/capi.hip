@@ -68,6 +68,20 @@ static int refresh_layers(astro_ctx* c) {
     compute_layers(sizes, c->params.bits_per_axis, &c->layers, c->cap, c->off, &c->total);
     if (c->total > c->node_capacity)
         return fail(c, ASTRO_ERR_CAPACITY, "tree needs more nodes than the context holds");
+    // the seeded sort's buckets: a batch gives every system its own
+    if (c->nsys <= 1) {
+        c->rs_nb = resort_buckets(c->nt);
+    } else {
+        std::vector<uint32_t> b0, bs;
+        c->rs_nb = resort_buckets_batch(c->sys_off, &b0, &bs);
+        if (c->rs_nb) {
+            HIPCHK(c, hipMemcpyAsync(c->d_sys_b0, b0.data(), b0.size() * sizeof(uint32_t),
+                                     hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->d_bkt_sys, bs.data(), bs.size() * sizeof(uint32_t),
+                                     hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+        }
+    }
     invalidate_graph(c);
     c->onodes_dirty = true;  // stale-tail clearing assumes an unchanged layer table
     return ASTRO_OK;
@@ -136,7 +150,8 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->err_word, c->stats, c->scratch,
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom, c->order,
                     c->rs_keys, c->rs_idx, c->spl_keys, c->spl_idx, c->rs_count, c->spl_cube,
-                    c->spos, c->stype, c->rs_pos, c->rs_type, c->wlow};
+                    c->spos, c->stype, c->rs_pos, c->rs_type, c->wlow, c->d_sys_b0,
+                    c->d_bkt_sys};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -244,7 +259,9 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->diag_partial, (uint64_t)c->cfg.max_systems * kDiagChunks * 9);
     // the seeded sort: previous order, bucket slots (kResortCap per bucket), splitters, counts
     // + overflow count
-    c->rs_slots = resort_slots(NP);
+    // (batches: at most one bucket more per system)
+    c->rs_slots = std::min<uint64_t>(resort_slots(NP) + (uint64_t)c->cfg.max_systems * kResortCap,
+                                     (uint64_t)kResortMaxBuckets * kResortCap);
     e = e ? e : dalloc(c->device_bytes, &c->order, NP);
     e = e ? e : dalloc(c->device_bytes, &c->rs_keys, c->rs_slots);
     e = e ? e : dalloc(c->device_bytes, &c->rs_idx, c->rs_slots);
@@ -253,7 +270,9 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->spl_keys, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->spl_idx, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->rs_count, 2ull * kResortMaxBuckets + 1);
-    e = e ? e : dalloc(c->device_bytes, &c->spl_cube, 1);
+    e = e ? e : dalloc(c->device_bytes, &c->spl_cube, c->cfg.max_systems);
+    e = e ? e : dalloc(c->device_bytes, &c->d_sys_b0, c->cfg.max_systems + 1ull);
+    e = e ? e : dalloc(c->device_bytes, &c->d_bkt_sys, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->wlow, 2);
     e = e ? e : dalloc(c->device_bytes, &c->spos, NT);  // the sort's positions and types in
     e = e ? e : dalloc(c->device_bytes, &c->stype, NT); // sorted order
@@ -986,10 +1005,11 @@ static TreeArgs tree_args(astro_ctx* c) {
     return a;
 }
 
-// The seeded sort applies: one system, within its bucket limit.
+// The seeded sort applies: within its bucket limit (one system, or a batch whose systems each
+// get their own buckets).
 static bool seeded_sort_applies(const astro_ctx* c) {
-    const uint64_t nb = resort_buckets(c->nt);
-    return c->sort_mode == ASTRO_SORT_SEEDED && c->nsys <= 1 && c->nt > 0 && nb > 0 &&
+    const uint64_t nb = c->rs_nb;
+    return c->sort_mode == ASTRO_SORT_SEEDED && c->nt > 0 && nb > 0 &&
            nb * kResortCap <= c->rs_slots && c->nt <= c->cfg.max_particles;
 }
 // This build's sort is the seeded one: it applies and the previous order is there.
@@ -1030,6 +1050,12 @@ static ResortArgs resort_args(astro_ctx* c) {
     r.type = c->type;
     r.spos = c->spos;
     r.stype = c->stype;
+    r.nb = c->rs_nb;
+    r.nsys = c->nsys > 1 ? c->nsys : 1u;
+    r.sys_of = c->sys_of;
+    r.sys_off = c->d_sys_off;
+    r.sys_b0 = c->d_sys_b0;
+    r.bkt_sys = c->d_bkt_sys;
     return r;
 }
 

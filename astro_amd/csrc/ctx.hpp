@@ -84,8 +84,11 @@ struct astro_ctx {
     uint64_t* spl_keys = nullptr;
     uint32_t* spl_idx = nullptr;
     uint32_t* rs_count = nullptr;
-    float4* spl_cube = nullptr;        // the cube of the splitters' keys
+    float4* spl_cube = nullptr;        // [max_systems] the cubes of the splitters' keys
     uint64_t rs_slots = 0;             // rows of rs_keys / rs_idx
+    uint32_t rs_nb = 0;                // buckets of the current layout (0: no seeded sort)
+    uint32_t* d_sys_b0 = nullptr;      // [max_systems + 1] first bucket of each system (batch)
+    uint32_t* d_bkt_sys = nullptr;     // [kResortMaxBuckets] system of each bucket (batch)
     // positions and types in sorted order, written by every sort of a build (read by the fused
     // build, and by the traversal of the same step)
     float4* spos = nullptr;

@@ -152,19 +152,40 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t m) {
     return P;
 }
 
+// ------------------------------------------------------------------------------ bucket map
+// One system's positions [off, off + n) and buckets [b0, b0 + nb): bucket b0 + c starts at
+// position off + c n / nb (rounded down).  One system: the whole sort.
+struct SysBuckets {
+    uint32_t s, off, n, b0, nb;
+};
+__device__ __forceinline__ SysBuckets sys_buckets(const ResortArgs& a, uint32_t s, uint32_t nb) {
+    if (a.nsys <= 1) return SysBuckets{0u, 0u, a.n, 0u, nb};
+    const uint32_t off = a.sys_off[s], b0 = a.sys_b0[s];
+    return SysBuckets{s, off, a.sys_off[s + 1] - off, b0, a.sys_b0[s + 1] - b0};
+}
+// Position where the system's bucket c (0 .. nb) starts.
+__device__ __forceinline__ uint32_t bucket_pos(const SysBuckets& m, uint32_t c) {
+    return m.off + (uint32_t)((uint64_t)c * m.n / m.nb);
+}
+// The system's bucket holding position q (the largest c with bucket_pos(c) <= q).
+__device__ __forceinline__ uint32_t bucket_of_pos(const SysBuckets& m, uint32_t q) {
+    return (uint32_t)(((uint64_t)(q - m.off + 1) * m.nb + m.n - 1) / m.n - 1);
+}
+
 // --------------------------------------------------------------------------------- samples
-__device__ __forceinline__ SortRow sample_row(const ResortArgs& a, uint32_t j, uint32_t ns) {
-    const uint32_t p = a.order[(uint64_t)j * a.n / ns];
-    return SortRow{morton_key(a.pos[p], a.cubes, 0u, a.params), p};
+__device__ __forceinline__ SortRow sample_row(const ResortArgs& a, const SysBuckets& m,
+                                              uint32_t j, uint32_t ns) {
+    const uint32_t p = a.order[m.off + (uint64_t)j * m.n / ns];
+    return SortRow{morton_key(a.pos[p], a.cubes, m.s, a.params), p};
 }
 
 // The last sort's splitters still divide this sort's keys evenly: the key grid has the same cell
 // size and moved by less than kResortShift cells along each axis (or does not depend on the
 // cube).
 constexpr float kResortShift = 1024.0f;
-__device__ inline bool splitters_current(const ResortArgs& a) {
+__device__ inline bool splitters_current(const ResortArgs& a, uint32_t s) {
     if (a.params.quantizer == ASTRO_QUANT_OFFSET_WRAP) return true;
-    const float4 c = a.cubes[0], o = *a.spl_cube;
+    const float4 c = a.cubes[s], o = a.spl_cube[s];
     int e, eo;
     frexpf(c.w, &e);
     frexpf(o.w, &eo);
@@ -174,39 +195,45 @@ __device__ inline bool splitters_current(const ResortArgs& a) {
            fabsf(c.z - o.z) * scale < kResortShift;
 }
 
-__global__ __launch_bounds__(kResortThreads) void k_resort_sample(ResortArgs a, uint32_t nb,
-                                                                  uint32_t over) {
+// One block per system: zeroes its buckets' counts and, when its key grid changed, samples its
+// previous order (over samples per bucket) and sorts the samples for its splitters.
+__global__ __launch_bounds__(kResortThreads) void k_resort_sample(ResortArgs a, uint32_t nb) {
     __shared__ uint64_t s_k[kResortMaxSamples];
     __shared__ uint32_t s_i[kResortMaxSamples];
     const uint32_t tid = threadIdx.x;
-    const uint32_t ns = nb * over;
+    const SysBuckets m = sys_buckets(a, blockIdx.x, nb);
     // the counts and the overflow list of this sort start empty
-    for (uint32_t b = tid; b < nb; b += kResortThreads) {
-        a.moved_in[b] = 0;
-        a.moved_out[b] = 0;
+    for (uint32_t b = tid; b < m.nb; b += kResortThreads) {
+        a.moved_in[m.b0 + b] = 0;
+        a.moved_out[m.b0 + b] = 0;
     }
-    if (tid == 0) *a.ovf = 0;
-    if (splitters_current(a)) return;  // the last sort's splitters (block-uniform)
-    if (tid == 0) *a.spl_cube = a.cubes[0];
+    if (blockIdx.x == 0 && tid == 0) *a.ovf = 0;
+    if (splitters_current(a, m.s)) return;  // the last sort's splitters (block-uniform)
+    if (tid == 0) a.spl_cube[m.s] = a.cubes[m.s];
+    uint32_t over = kResortMaxSamples / m.nb;  // samples per bucket
+    if (over > kResortOversample) over = kResortOversample;
+    const uint32_t ns = m.nb * over;
+    uint64_t* spl_k = a.spl_k + m.b0;  // the system's splitters: nb - 1 of them
+    uint32_t* spl_i = a.spl_i + m.b0;
     if (ns <= 2 * kResortThreads) {
         SortRow r[2];
         for (uint32_t s = 0; s < 2; s++) {
             const uint32_t j = 2 * tid + s;
-            r[s] = j < ns ? sample_row(a, j, ns) : row_max();
+            r[s] = j < ns ? sample_row(a, m, j, ns) : row_max();
         }
         bitonic_pairs<kResortThreads>(r, pow2_at_least(ns), s_k, s_i);
         // splitter t is sample (t + 1) * over
         for (uint32_t s = 0; s < 2; s++) {
             const uint32_t j = 2 * tid + s;
             if (j < ns && j > 0 && j % over == 0) {
-                a.spl_k[j / over - 1] = r[s].k;
-                a.spl_i[j / over - 1] = r[s].i;
+                spl_k[j / over - 1] = r[s].k;
+                spl_i[j / over - 1] = r[s].i;
             }
         }
         return;
     }
     for (uint32_t j = tid; j < ns; j += kResortThreads) {
-        const SortRow v = sample_row(a, j, ns);
+        const SortRow v = sample_row(a, m, j, ns);
         s_k[j] = v.k;
         s_i[j] = v.i;
     }
@@ -224,40 +251,41 @@ __global__ __launch_bounds__(kResortThreads) void k_resort_sample(ResortArgs a, 
             }
         },
         [] { lds_barrier(); });
-    for (uint32_t t = tid; t + 1 < nb; t += kResortThreads) {
-        a.spl_k[t] = s_k[(t + 1) * over];
-        a.spl_i[t] = s_i[(t + 1) * over];
+    for (uint32_t t = tid; t + 1 < m.nb; t += kResortThreads) {
+        spl_k[t] = s_k[(t + 1) * over];
+        spl_i[t] = s_i[(t + 1) * over];
     }
 }
 
 // ----------------------------------------------------------------------------------- place
-// Splitter t lies at or before source (key, p).
-__device__ __forceinline__ bool splitter_le(const ResortArgs& a, int t, uint64_t key, uint32_t p) {
-    const uint64_t sk = a.spl_k[t];
-    return sk < key || (sk == key && a.spl_i[t] <= p);
+// Splitter t (of the system's, spl_k / spl_i) lies at or before source (key, p).
+__device__ __forceinline__ bool splitter_le(const uint64_t* spl_k, const uint32_t* spl_i, int t,
+                                            uint64_t key, uint32_t p) {
+    const uint64_t sk = spl_k[t];
+    return sk < key || (sk == key && spl_i[t] <= p);
 }
 
-// The bucket of (key, p): the number of splitters (nb - 1, ascending) at or before it.  A
-// galloping search from the guess g: two loads when the guess is right.
-__device__ inline uint32_t find_bucket(const ResortArgs& a, uint64_t key, uint32_t p, uint32_t g,
-                                       uint32_t nb) {
+// The system's bucket of (key, p): the number of its splitters (nb - 1, ascending) at or
+// before it.  A galloping search from the guess g: two loads when the guess is right.
+__device__ inline uint32_t find_bucket(const uint64_t* spl_k, const uint32_t* spl_i,
+                                       uint64_t key, uint32_t p, uint32_t g, uint32_t nb) {
     const int ns = (int)nb - 1, gg = (int)g;
     int lo, hi;  // splitter lo at or before (p), or lo = -1; splitter hi after, or hi = ns
-    if (gg > 0 && !splitter_le(a, gg - 1, key, p)) {
+    if (gg > 0 && !splitter_le(spl_k, spl_i, gg - 1, key, p)) {
         hi = gg - 1;
         int step = 1;
         lo = hi - step;
-        while (lo >= 0 && !splitter_le(a, lo, key, p)) {
+        while (lo >= 0 && !splitter_le(spl_k, spl_i, lo, key, p)) {
             hi = lo;
             step <<= 1;
             lo = hi - step;
         }
         if (lo < -1) lo = -1;
-    } else if (gg < ns && splitter_le(a, gg, key, p)) {
+    } else if (gg < ns && splitter_le(spl_k, spl_i, gg, key, p)) {
         lo = gg;
         int step = 1;
         hi = lo + step;
-        while (hi < ns && splitter_le(a, hi, key, p)) {
+        while (hi < ns && splitter_le(spl_k, spl_i, hi, key, p)) {
             lo = hi;
             step <<= 1;
             hi = lo + step;
@@ -268,45 +296,39 @@ __device__ inline uint32_t find_bucket(const ResortArgs& a, uint64_t key, uint32
     }
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (splitter_le(a, mid, key, p)) lo = mid;
+        if (splitter_le(spl_k, spl_i, mid, key, p)) lo = mid;
         else hi = mid;
     }
     return (uint32_t)hi;
 }
 
-// Row c n / nb (rounded down) of the previous order, c = 0 .. nb: where the positions of
-// bucket c start.
-__device__ __forceinline__ uint32_t bucket_pos(const ResortArgs& a, uint32_t nb, uint32_t c) {
-    return (uint32_t)((uint64_t)c * a.n / nb);
-}
-
 // A source whose bucket is the one its previous position lies in (a "stayer") is stored at that
 // position's slot: bucket g's slots 0 .. S_g - 1 are its positions.  A source that changed
 // bucket leaves a hole (a row that sorts last) at its position's slot, counts itself out of
-// the old bucket and takes a slot past the positions of the new one with one atomic.
+// the old bucket and takes a slot past the positions of the new one with one atomic.  Sources
+// never change system: a batch's source looks for its bucket among its own system's.
 __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs a, uint32_t nb) {
     const uint32_t i = blockIdx.x * kResortPlaceThreads + threadIdx.x;
-    // The guess: the bucket whose positions hold i, i.e. the number of splitter rows
-    // q(c) = c n / nb (c = 1 .. nb - 1) at or before i.  Buckets span at least n / nb >= 448
-    // positions (or there is one), so the block's 256 positions meet at most one q(c): the
-    // guess is g0 (the first position's) or g0 + 1 from q(g0 + 1) on.
-    const uint32_t i0 = blockIdx.x * kResortPlaceThreads;
-    const uint32_t g0 = (uint32_t)(((uint64_t)(i0 + 1) * nb + a.n - 1) / a.n - 1);
-    const uint32_t q0 = bucket_pos(a, nb, g0), q1 = bucket_pos(a, nb, g0 + 1);
     if (i >= a.n) return;
-    const uint32_t g = i >= q1 ? g0 + 1 : g0;
-    const uint32_t qs = i >= q1 ? q1 : q0;
+    // the guess: the bucket whose positions hold i
+    const uint32_t sy = a.nsys > 1 ? a.sys_of[i] : 0u;
+    const SysBuckets m = sys_buckets(a, sy, nb);
+    const uint32_t c = bucket_of_pos(m, i);
+    const uint32_t g = m.b0 + c;
+    const uint32_t qs = bucket_pos(m, c);
+    const uint64_t* spl_k = a.spl_k + m.b0;
+    const uint32_t* spl_i = a.spl_i + m.b0;
     // the splitters around the guessed bucket load beside the source's own chain
     SortRow s0 = SortRow{0ull, 0u}, s1 = row_max();
-    if (g > 0) s0 = SortRow{a.spl_k[g - 1], a.spl_i[g - 1]};
-    if (g + 1 < nb) s1 = SortRow{a.spl_k[g], a.spl_i[g]};
+    if (c > 0) s0 = SortRow{spl_k[c - 1], spl_i[c - 1]};
+    if (c + 1 < m.nb) s1 = SortRow{spl_k[c], spl_i[c]};
     const uint32_t p = a.order[i];
     const float4 xp = a.pos[p];
     const uint32_t ty = a.type[p];
-    const uint64_t key = morton_key(xp, a.cubes, 0u, a.params);
+    const uint64_t key = morton_key(xp, a.cubes, sy, a.params);
     const SortRow x{key, p};
-    const bool at_or_after = g == 0 || !row_less(x, s0);
-    const bool before = g + 1 >= nb || row_less(x, s1);
+    const bool at_or_after = c == 0 || !row_less(x, s0);
+    const bool before = c + 1 >= m.nb || row_less(x, s1);
     const uint64_t stay = (uint64_t)g * kResortCap + (i - qs);
     if (at_or_after && before) {
         a.rk[stay] = key;
@@ -315,12 +337,13 @@ __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs
         a.rt[stay] = ty;
         return;
     }
-    const uint32_t b = find_bucket(a, key, p, g, nb);
+    const uint32_t bc = find_bucket(spl_k, spl_i, key, p, c, m.nb);
+    const uint32_t b = m.b0 + bc;
     a.rk[stay] = ~0ull;  // a hole
     a.ri[stay] = ~0u;
     atomicAdd(&a.moved_out[g], 1u);
     const uint32_t slot =
-        bucket_pos(a, nb, b + 1) - bucket_pos(a, nb, b) + atomicAdd(&a.moved_in[b], 1u);
+        bucket_pos(m, bc + 1) - bucket_pos(m, bc) + atomicAdd(&a.moved_in[b], 1u);
     if (slot < kResortCap) {
         const uint64_t q = (uint64_t)b * kResortCap + slot;
         a.rk[q] = key;
@@ -336,20 +359,21 @@ __global__ __launch_bounds__(kResortPlaceThreads) void k_resort_place(ResortArgs
 }
 
 // ---------------------------------------------------------------------------------- bucket
-// Splitter t of the next sort is the row at sorted position (t + 1) n / nb; row position q writes
-// the one it is, if any.
-__device__ __forceinline__ void emit_splitter(const ResortArgs& a, uint32_t nb, uint32_t q,
+// The next sort's splitter c of a system (c = 1 .. nb - 1, stored at b0 + c - 1) is the row at
+// its bucket c's first position; row position q writes the one it is, if any.
+__device__ __forceinline__ void emit_splitter(const ResortArgs& a, const SysBuckets& m, uint32_t q,
                                               uint64_t k, uint32_t i) {
-    const uint64_t t1 = ((uint64_t)q * nb + a.n - 1) / a.n;  // the smallest t + 1 reaching q
-    if (t1 >= 1 && t1 < nb && t1 * a.n / nb == q) {
-        a.spl_k[t1 - 1] = k;
-        a.spl_i[t1 - 1] = i;
+    const uint32_t x = q - m.off;
+    const uint64_t c = ((uint64_t)x * m.nb + m.n - 1) / m.n;  // the smallest c reaching x
+    if (c >= 1 && c < m.nb && c * m.n / m.nb == x) {
+        a.spl_k[m.b0 + c - 1] = k;
+        a.spl_i[m.b0 + c - 1] = i;
     }
 }
 
-// The smallest c >= 1 whose splitter position c n / nb (rounded down) is at or past q0.
-__device__ __forceinline__ uint32_t first_splitter(const ResortArgs& a, uint32_t nb, uint32_t q0) {
-    const uint32_t c = (uint32_t)(((uint64_t)q0 * nb + a.n - 1) / a.n);
+// The smallest c >= 1 whose bucket starts at or past position q0.
+__device__ __forceinline__ uint32_t first_splitter(const SysBuckets& m, uint32_t q0) {
+    const uint32_t c = (uint32_t)(((uint64_t)(q0 - m.off) * m.nb + m.n - 1) / m.n);
     return c ? c : 1u;
 }
 
@@ -361,12 +385,15 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
     __shared__ uint32_t s_fill;
     static_assert(kResortCap >= 4 * NT && kResortRegRows == 2 * NT, "row layout");
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    const SysBuckets sb = sys_buckets(a, a.nsys > 1 ? a.bkt_sys[b] : 0u, nb);
+    const uint32_t c = b - sb.b0;  // the bucket within its system
     // the bucket's positions, sources that left and sources that came: its rows are its
-    // positions' slots (stayers and holes) and then those that came
-    const uint32_t q_b = bucket_pos(a, nb, b), npos = bucket_pos(a, nb, b + 1) - q_b;
+    // positions' slots (stayers and holes) and then those that came.  Sources stay in their
+    // system, so the system's buckets before this one place it.
+    const uint32_t q_b = bucket_pos(sb, c), npos = bucket_pos(sb, c + 1) - q_b;
     const uint32_t came = a.moved_in[b], left = a.moved_out[b];
     uint32_t part_in = 0, part_out = 0;
-    for (uint32_t t = tid; t < b; t += NT) {
+    for (uint32_t t = sb.b0 + tid; t < b; t += NT) {
         part_in += a.moved_in[t];
         part_out += a.moved_out[t];
     }
@@ -374,8 +401,9 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
     block_sum_and_scan<NT>(part_out, part_in, s_tmp, &out_before, &in_before);
     const uint32_t start = __builtin_amdgcn_readfirstlane(q_b + in_before - out_before);
     const uint32_t m = npos + came - left;  // the bucket's sources
-    if (b == 0 && tid == 0) *a.spl_cube = a.cubes[0];  // the next sort's splitters' cube
-    if (left > npos || start > a.n || m > a.n - start) return;  // never, on counts of a pass
+    if (c == 0 && tid == 0) a.spl_cube[sb.s] = a.cubes[sb.s];  // the next sort's splitters' cube
+    // never, on counts of a pass
+    if (left > npos || start < sb.off || start > sb.off + sb.n || m > sb.off + sb.n - start) return;
     const uint64_t slots = (uint64_t)b * kResortCap;
     const uint32_t R = npos + came;  // rows in the slots when none overflowed
     if (R <= kResortRegRows) {  // holes sort after every source
@@ -414,13 +442,13 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
             }
         }
         // the next sort's splitters among these rows (a few per bucket, found once per block)
-        for (uint32_t c = first_splitter(a, nb, start); c < nb; c++) {
-            const uint32_t q = (uint32_t)((uint64_t)c * a.n / nb) - start;
+        for (uint32_t cs = first_splitter(sb, start); cs < sb.nb; cs++) {
+            const uint32_t q = bucket_pos(sb, cs) - start;
             if (q >= m) break;
             if ((q >> 1) == tid) {
                 const SortRow v = (q & 1u) ? r[1] : r[0];
-                a.spl_k[c - 1] = v.k;
-                a.spl_i[c - 1] = v.i;
+                a.spl_k[sb.b0 + cs - 1] = v.k;
+                a.spl_i[sb.b0 + cs - 1] = v.i;
             }
         }
         return;
@@ -450,7 +478,7 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
             a.order[start + j] = s_i[j];
             a.spos[start + j] = a.pos[s_i[j]];
             a.stype[start + j] = a.type[s_i[j]];
-            emit_splitter(a, nb, start + j, s_k[j], s_i[j]);
+            emit_splitter(a, sb, start + j, s_k[j], s_i[j]);
         }
         return;
     }
@@ -497,7 +525,7 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
         a.order[start + j] = gi[j];
         a.spos[start + j] = a.pos[gi[j]];
         a.stype[start + j] = a.type[gi[j]];
-        emit_splitter(a, nb, start + j, gk[j], gi[j]);
+        emit_splitter(a, sb, start + j, gk[j], gi[j]);
     }
 }
 
@@ -505,13 +533,15 @@ __global__ __launch_bounds__(kResortBucketThreads) void k_resort_bucket(ResortAr
 // the splitters, and this build's cube as theirs.
 __global__ void k_resort_seed(const uint64_t* __restrict__ sk, const uint32_t* __restrict__ si,
                               ResortArgs a, uint32_t nb) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.spl_cube = a.cubes[0];
+    const uint32_t ns = a.nsys > 1 ? a.nsys : 1u;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x)
+        a.spl_cube[s] = a.cubes[s];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < a.n; q += gridDim.x * blockDim.x) {
         const uint32_t i = si[q];
         a.order[q] = i;
         a.spos[q] = a.pos[i];
         a.stype[q] = a.type[i];
-        emit_splitter(a, nb, q, sk[q], i);
+        emit_splitter(a, sys_buckets(a, a.nsys > 1 ? a.sys_of[q] : 0u, nb), q, sk[q], i);
     }
 }
 
@@ -534,7 +564,7 @@ void launch_sorted_gather(const uint32_t* si, const float4* pos, const uint32_t*
 }
 void launch_resort_seed(const uint64_t* sk, const uint32_t* si, const ResortArgs& a,
                         hipStream_t s) {
-    const uint32_t nb = resort_buckets(a.n);
+    const uint32_t nb = a.nb;
     if (!a.n || !nb) return;
     uint32_t blocks = (a.n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
@@ -567,12 +597,28 @@ uint32_t resort_buckets(uint32_t n) {
     return nb <= kResortMaxBuckets ? nb : 0u;
 }
 
+uint32_t resort_buckets_batch(const std::vector<uint32_t>& off, std::vector<uint32_t>* b0,
+                              std::vector<uint32_t>* bkt_sys) {
+    const size_t ns = off.empty() ? 0 : off.size() - 1;
+    b0->assign(ns + 1, 0u);
+    bkt_sys->clear();
+    uint64_t nb = 0;
+    for (size_t s = 0; s < ns; s++) {
+        (*b0)[s] = (uint32_t)nb;
+        const uint64_t k = ((uint64_t)(off[s + 1] - off[s]) + kResortMean - 1) / kResortMean;
+        nb += k;
+        if (nb > kResortMaxBuckets) return 0u;
+        bkt_sys->insert(bkt_sys->end(), (size_t)k, (uint32_t)s);
+    }
+    (*b0)[ns] = (uint32_t)nb;
+    return (uint32_t)nb;
+}
+
 void launch_resort(const ResortArgs& a, hipStream_t s) {
-    const uint32_t nb = resort_buckets(a.n);
+    const uint32_t nb = a.nb;
     if (!a.n || !nb) return;
-    uint32_t over = kResortMaxSamples / nb;  // samples per bucket
-    if (over > kResortOversample) over = kResortOversample;
-    hipLaunchKernelGGL(k_resort_sample, dim3(1), dim3(kResortThreads), 0, s, a, nb, over);
+    hipLaunchKernelGGL(k_resort_sample, dim3(a.nsys > 1 ? a.nsys : 1u), dim3(kResortThreads), 0,
+                       s, a, nb);
     hipLaunchKernelGGL(k_resort_place, dim3((a.n + kResortPlaceThreads - 1) / kResortPlaceThreads),
                        dim3(kResortPlaceThreads), 0, s, a, nb);
     hipLaunchKernelGGL(k_resort_bucket, dim3(nb), dim3(kResortBucketThreads), 0, s, a, nb);

@@ -1,6 +1,8 @@
 // launch.hpp -- host-side launchers of the libastro kernels (defined in the k_*.hip files).
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -72,8 +74,8 @@ constexpr uint32_t kResortOversample = 2;      // samples per bucket (at most)
 // Up to this many sources the register sort's rows carry index << 10 | slot (slot < 1,024)
 constexpr uint32_t kResortPackMax = 1u << 22;
 struct ResortArgs {
-    const float4* pos;       // [n] source positions (one system)
-    const float4* cubes;     // [1] its cube
+    const float4* pos;       // [n] source positions
+    const float4* cubes;     // [nsys] the systems' cubes
     astro_params params;     // quantizer and key bits
     uint32_t* order;         // [n] previous sorted order in, this sort's order out
     uint32_t n;
@@ -83,7 +85,7 @@ struct ResortArgs {
     uint32_t* rt;
     uint64_t* spl_k;         // [kResortMaxBuckets] splitters (the bucket kernel leaves the next
     uint32_t* spl_i;         //  sort's, from its own sorted rows)
-    float4* spl_cube;        // [1] the cube the splitters' keys were made with
+    float4* spl_cube;        // [nsys] the cubes the splitters' keys were made with
     uint32_t* moved_in;      // [kResortMaxBuckets] sources that came into each bucket
     uint32_t* moved_out;     // [kResortMaxBuckets] sources that left each bucket's positions
     uint32_t* ovf;           // [1] overflow entries
@@ -95,9 +97,23 @@ struct ResortArgs {
     const uint32_t* type;    // [n] source types
     float4* spos;            // [n] positions and types in sorted order (out)
     uint32_t* stype;
+    // Batches (nsys > 1): system s holds positions and sources [sys_off[s], sys_off[s+1]) and
+    // buckets [sys_b0[s], sys_b0[s+1]); a bucket never spans two systems, so that rows of one
+    // bucket compare by (key, index) alone.  One system: nsys 1, the arrays unused.
+    uint32_t nb;             // buckets in all
+    uint32_t nsys;
+    const uint32_t* sys_of;  // [n] system of each source (and of each position)
+    const uint32_t* sys_off; // [nsys + 1]
+    const uint32_t* sys_b0;  // [nsys + 1]
+    const uint32_t* bkt_sys; // [buckets] the system of each bucket
 };
-// Buckets of a seeded sort of n sources; 0 when n is beyond it.
+// Buckets of a seeded sort of n sources (of one system); 0 when n is beyond it.
 uint32_t resort_buckets(uint32_t n);
+// Buckets of a seeded sort of the systems [off[s], off[s+1]) (each gets its own, at least one
+// per source-holding system), their first buckets b0 (nsys + 1 entries) and the system of
+// every bucket; 0 when beyond the sort's bucket limit.
+uint32_t resort_buckets_batch(const std::vector<uint32_t>& off, std::vector<uint32_t>* b0,
+                              std::vector<uint32_t>* bkt_sys);
 // Bucket slots (rows of rk / ri) a seeded sort of up to nmax sources needs.
 inline uint64_t resort_slots(uint32_t nmax) {
     uint64_t nb = ((uint64_t)nmax + kResortMean - 1) / kResortMean;

@@ -163,3 +163,103 @@ def test_seeded_bucket_limit(astro, n, seeded):
         ctx.check()
         assert ctx.sort_info() == (seeded, 0)
         check_sorted(ctx, n)
+
+
+# ---- batches: every system gets its own buckets; the order is (system, key, index) ----------
+def check_sorted_batch(ctx, sys_of):
+    codes = ctx.read("MORTON_CODES", np.uint64, len(sys_of))
+    perm = ctx.read("SORTED_INDICES", np.uint32, len(sys_of))
+    keys = ctx.read("SORTED_CODES", np.uint64, len(sys_of))
+    want = np.lexsort((codes, sys_of)).astype(np.uint32)  # stable: ties by index
+    assert np.array_equal(perm, want)
+    assert np.array_equal(keys, codes[want])
+
+
+def batch(astro, sizes, seed0):
+    from test_gpu_batched import make_batch
+    parts, pos, vel, col, off = make_batch(astro, sizes, seed0=seed0)
+    sys_of = np.repeat(np.arange(len(sizes)), sizes)
+    return pos, vel, col, off, sys_of
+
+
+BATCH_SIZES = [1, 500, 896, 897, 3000, 20000, 1793, 70001, 2]
+
+
+def test_seeded_batch_steps_and_permutations(astro):
+    # the first build sorts with the radix sort and seeds the next; steps and a permutation of
+    # every system's positions in place (its previous order unrelated to the new keys) sort
+    # from the previous order, each system among its own buckets
+    say("seeded sort: batch of", len(BATCH_SIZES), "systems", BATCH_SIZES)
+    pos, vel, col, off, sys_of = batch(astro, BATCH_SIZES, 40)
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=16) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload_systems(pos, vel, col, off)
+        ctx.build_tree()
+        ctx.check()
+        assert ctx.sort_info() == (False, 0)
+        check_sorted_batch(ctx, sys_of)
+        for k in range(3):
+            ctx.step(1e-3, 1)
+            ctx.check()
+            assert ctx.sort_info()[0], k
+            check_sorted_batch(ctx, sys_of)
+        rng = np.random.default_rng(4)
+        p2, _, _ = ctx.download()
+        for s in range(len(BATCH_SIZES)):
+            a, b = int(off[s]), int(off[s + 1])
+            p2[a:b] = p2[a:b][rng.permutation(b - a)]
+        ctx.write("POSITION_MASS", rows(p2))
+        ctx.build_tree()
+        ctx.check()
+        assert ctx.sort_info()[0]
+        check_sorted_batch(ctx, sys_of)
+
+
+@pytest.mark.parametrize("change", ["double_one", "shift_one"])
+def test_seeded_batch_one_cube_changes(astro, change):
+    # one system's cube changes (its splitters are sampled again, its buckets overflow when
+    # doubled); the others keep theirs
+    say("seeded sort: batch, one system's cube", change)
+    sizes = [4000, 50000, 9000]
+    pos, vel, col, off, sys_of = batch(astro, sizes, 70)
+    with astro.Context(len(pos), bits_per_axis=21, max_systems=4) as ctx:
+        ctx.set_params(astro.params_physics(21))
+        ctx.upload_systems(pos, vel, col, off)
+        ctx.build_tree()
+        ctx.build_tree()
+        ctx.check()
+        assert ctx.sort_info() == (True, 0)
+        p2 = pos.copy()
+        a, b = int(off[1]), int(off[2])
+        seg = p2[a:b][np.random.default_rng(2).permutation(b - a)]
+        if change == "double_one":
+            seg["position"] *= np.float32(2.0)
+        else:
+            seg["position"] += np.float32(0.3) * np.ptp(seg["position"], axis=0).max()
+        p2[a:b] = seg
+        ctx.write("POSITION_MASS", rows(p2))
+        ctx.build_tree()
+        ctx.check()
+        seeded, ovf = ctx.sort_info()
+        assert seeded
+        if change == "double_one":
+            assert ovf > 0
+        check_sorted_batch(ctx, sys_of)
+
+
+def test_seeded_and_radix_batch_steps_same_bytes(astro):
+    # six graph steps of a batch with each sort: the same positions and velocities
+    say("seeded vs radix: batch of", len(BATCH_SIZES), "systems, 6 steps")
+    pos, vel, col, off, _ = batch(astro, BATCH_SIZES, 90)
+    out = {}
+    for mode in (astro.SORT_RADIX, astro.SORT_SEEDED):
+        with astro.Context(len(pos), bits_per_axis=21, max_systems=16) as ctx:
+            ctx.set_params(astro.params_physics(21))
+            ctx.set_sort_mode(mode)
+            ctx.upload_systems(pos, vel, col, off)
+            ctx.step(1e-3, 6)
+            ctx.check()
+            assert ctx.sort_info()[0] == (mode == astro.SORT_SEEDED)
+            out[mode] = ctx.download()[:2]
+    assert same_bytes(out[0][0], out[1][0])
+    assert same_bytes(out[0][1], out[1][1])
