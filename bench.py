@@ -532,12 +532,13 @@ def main():
                    st1["monopole_terms"])
     achieved = FLOP_PER_INTERACTION * inter / (force_ms * 1e-3) / 1e12
     traffic = None
-    valu_frac = None
+    valu_frac = ta_frac = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic = tj.get("force_compact_bytes_per_launch")
         valu_frac = tj.get("valu_issue_frac")
+        ta_frac = tj.get("ta_busy_frac")
     # the OctreeNode array is written on read only
     sb = stage_bytes(n, V, emit=False, seeded=a.sort == "seeded")
     stages = {}
@@ -576,9 +577,11 @@ def main():
                      "valu_issue_what": "SQ_INSTS_VALU x 4 cycles per wave64 instruction over "
                                         "1024 SIMDs and the kernel's busy cycles "
                                         "(GRBM_GUI_ACTIVE per XCD; rocprofv3, profiles/)",
-                     "limiter": "VALU issue (89% of the SIMD cycles) with the top tree "
-                                "layers read from LDS; the texture address unit, busy 66%, "
-                                "bound it before (83%) "
+                     "ta_busy_frac": ta_frac,
+                     "limiter": "VALU issue with the top tree layers read from LDS (the "
+                                "texture address unit bound it while every node came from "
+                                "memory: busy 83%); issue and address unit both below "
+                                "saturation: the walk's dependent chain per visit "
                                 "(profiles/force_memory_pipeline_two_disks_500k.md)",
                      "interactions_per_step": inter, "flop_per_interaction": FLOP_PER_INTERACTION,
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
