@@ -273,7 +273,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->spl_cube, c->cfg.max_systems);
     e = e ? e : dalloc(c->device_bytes, &c->d_sys_b0, c->cfg.max_systems + 1ull);
     e = e ? e : dalloc(c->device_bytes, &c->d_bkt_sys, kResortMaxBuckets);
-    e = e ? e : dalloc(c->device_bytes, &c->wlow, 2);
+    e = e ? e : dalloc(c->device_bytes, &c->wlow, 3);
     e = e ? e : dalloc(c->device_bytes, &c->spos, NT);  // the sort's positions and types in
     e = e ? e : dalloc(c->device_bytes, &c->stype, NT); // sorted order
     if (e != hipSuccess) {
@@ -285,7 +285,7 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     hipStream_t s = c->stream;
     launch_fill_u32(c->err_word, 0, 1, s);
     launch_fill_u32(c->cube_counter, 0, 1, s);
-    launch_fill_u32(c->wlow, 0, 2, s);
+    launch_fill_u32(c->wlow, 0, 3, s);
     launch_fill_u32(c->layer_counts, 0, kMaxLayers, s);
     launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
     launch_fill_u32(c->sys_of, 0, NP, s);
@@ -1117,8 +1117,6 @@ static int enqueue_layers(astro_ctx* c, bool emit = true, bool full = false) {
         launch_fill_u32(c->layer_counts_prev, 0, kMaxLayers, s);
         c->onodes_dirty = false;
     }
-    // supertile sums and the tile counts of layers 1.. are accumulated: clear them
-    launch_fill_u32(c->scratch, 0, nsup + ntile, s);
     std::vector<uint32_t*> sup(c->layers), tc(c->layers);
     uint32_t* p = c->scratch;
     for (uint32_t l = 0; l < c->layers; l++) {
@@ -1136,8 +1134,9 @@ static int enqueue_layers(astro_ctx* c, bool emit = true, bool full = false) {
     if (!a.quirk_last_leaf && c->layers > 10)  // 13 at B = 21 (14 runs past the halo)
         lf = std::min(std::min(l0, c->layers - 9), kFusedMaxLayers);
     if (lf < 2) lf = 0;
-    // the next pointers of every layer's nodes, before any node is written
-    launch_heads_next(a, lf, c->plevel, c->fused_buf, s);
+    // the next pointers of every layer's nodes, before any node is written; the supertile sums
+    // and tile counts the layer kernels accumulate start at zero (cleared there too)
+    launch_heads_next(a, lf, c->plevel, c->fused_buf, c->scratch, (uint32_t)(nsup + ntile), s);
     if (lf) launch_fused_layers(a, lf, c->plevel, c->fused_buf, s);
     if (lf < l0) launch_layer_count(a, lf, tc[lf], sup[lf], s);
     for (uint32_t l = lf; l < l0; l++) {

@@ -718,7 +718,8 @@ __global__ __launch_bounds__(kTopThreads) void k_layers_top(TreeArgs a, uint32_t
 //    the particles and each layer from the one below, and writes the nodes that start in its
 //    tile.  It reads past its tile up to the next head at layer lf-1 (at most kFusedHalo
 //    positions), so the nodes it owns are complete; a node that runs past that halo is left to
-//  * k_fused_fix: one block, layer by layer, builds those nodes from the finished layer below.
+//  * fused_fix: the fused build's last block to finish builds those nodes, layer by layer,
+//    from the finished layer below.
 // Aggregation order and arithmetic are those of the per-layer kernels (emit_leaf, emit_parent),
 // so the nodes are bit-identical.
 constexpr uint32_t kFusedTile = 1024;
@@ -751,8 +752,12 @@ __device__ inline uint32_t lds_block_scan(uint32_t v, uint32_t* tmp, uint32_t* t
 __global__ __launch_bounds__(kFusedThreads) void k_fused_heads(TreeArgs a, uint8_t* plevel,
                                                                uint32_t* fcnt, uint32_t* ffix,
                                                                uint32_t lf, uint32_t nl,
-                                                               uint32_t tiles) {
+                                                               uint32_t tiles, uint32_t* clr,
+                                                               uint32_t nclr) {
     __shared__ uint32_t s_hist[kMaxLayers];
+    // the per-layer kernels' tile counts and supertile sums start at zero
+    for (uint32_t i = blockIdx.x * kFusedThreads + threadIdx.x; i < nclr; i += tiles * kFusedThreads)
+        clr[i] = 0u;
     if (threadIdx.x < kMaxLayers) s_hist[threadIdx.x] = 0;
     if (threadIdx.x < lf) ffix[threadIdx.x * tiles + blockIdx.x] = 0xFFFFFFFFu;  // no leftover
     __syncthreads();
@@ -788,7 +793,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const 
     // the block of layer lf - 1 also decides the lowest layer to write (block 0 without fused
     // layers): k_fused_build's range of tile t - 1 ends at the first head at layer >= lf - 1
     // in tile t, so a full tile t without one (no node of layer lf - 1 starts in it) leaves a
-    // node for k_fused_fix, which needs every layer below: then every layer is written
+    // node for fused_fix, which needs every layer below: then every layer is written
     const bool decide = l == (lf >= 2 ? lf - 1 : 0u);
     bool trunc = false;
     uint32_t carry = 0;
@@ -824,7 +829,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_scan(TreeArgs a, const 
                 W = K > 0 ? K - 1 : 0u;
             }
             a.wlow[0] = W;
-            a.wlow[1] = trunc ? 1u : 0u;  // k_fused_fix has nodes to build
+            a.wlow[1] = trunc ? 1u : 0u;  // fused_fix has nodes to build
         }
     }
 }
@@ -910,6 +915,35 @@ __device__ inline void mom_nodes(const float4* v, const float* sm, uint32_t c0, 
 
 // QUAD: the second moments too, aggregated in the per-layer kernels' order (emit_leaf,
 // emit_parent) from the layer below kept in LDS beside the centres.
+// The nodes k_fused_build left (they run past a halo), layer by layer: each tile's last node
+// of the layer, from the layer below, which is complete by then.  Run by the fused build's last
+// block to finish, after every other block's nodes are visible.
+template <bool QUAD>
+__device__ void fused_fix(const TreeArgs& a, const uint32_t* fcnt, const uint32_t* fbase,
+                          const uint32_t* ffix, uint32_t lf, uint32_t tiles) {
+    for (uint32_t l = 0; l < lf; l++) {
+        for (uint32_t t = threadIdx.x; t < tiles; t += kFusedThreads) {
+            const uint32_t v = ffix[(uint64_t)l * tiles + t];
+            if (v == 0xFFFFFFFFu) continue;
+            const uint64_t lt = (uint64_t)l * tiles + t;
+            const uint32_t p = fbase[lt] + fcnt[lt] - 1u;
+            if (p >= a.cap[l]) continue;
+            if (l == 0) {
+                if (a.fp64_from == 0) emit_leaf<Agg64, QUAD>(a, v, p);
+                else emit_leaf<Agg32, QUAD>(a, v, p);
+            } else {
+                const uint64_t cb = a.off[l - 1];
+                const uint32_t capc = (uint32_t)a.cap[l - 1];
+                uint32_t lv[8];
+                for (uint32_t j = 0; j < 8; j++) lv[j] = v + j < capc ? a.levels[cb + v + j] : 0u;
+                const TNode u0 = load_node(a, cb + v);
+                emit_parent<QUAD>(a, l, v, p, lv, u0, a.counts[l - 1]);
+            }
+        }
+        __syncthreads();  // layer l is complete before layer l + 1 reads it
+    }
+}
+
 template <bool QUAD>
 __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_fused_build(TreeArgs a,
                                                                const uint8_t* __restrict__ plevel,
@@ -1022,7 +1056,7 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
             if (st[i] >= own) continue;  // a halo node: owned by a later tile
             const uint32_t gi = base + k;
             if (gi >= a.cap[l]) continue;  // ERR_CAPACITY is raised by the scan
-            if (!done[i]) {  // runs past the halo: k_fused_fix builds it
+            if (!done[i]) {  // runs past the halo: fused_fix builds it
                 ffix[l * tiles + t] = l == 0 ? c0[i] : base_prev + c0[i];
                 continue;
             }
@@ -1051,36 +1085,23 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
         // s_rank or s_pos, and its first LDS write (s_tmp) was last read before this layer's
         // own later barriers
     }
-}
-
-// The nodes k_fused_build left (they run past a halo), layer by layer: each tile's last node
-// of the layer, from the layer below, which is complete by then.
-template <bool QUAD>
-__global__ __launch_bounds__(kFusedThreads) void k_fused_fix(TreeArgs a, const uint32_t* fcnt,
-                                                             const uint32_t* fbase,
-                                                             const uint32_t* ffix, uint32_t lf,
-                                                             uint32_t tiles) {
-    if (a.wlow[1] == 0u) return;  // no tile ran past its halo: nothing left over
-    for (uint32_t l = 0; l < lf; l++) {
-        for (uint32_t t = threadIdx.x; t < tiles; t += kFusedThreads) {
-            const uint32_t v = ffix[(uint64_t)l * tiles + t];
-            if (v == 0xFFFFFFFFu) continue;
-            const uint64_t lt = (uint64_t)l * tiles + t;
-            const uint32_t p = fbase[lt] + fcnt[lt] - 1u;
-            if (p >= a.cap[l]) continue;
-            if (l == 0) {
-                if (a.fp64_from == 0) emit_leaf<Agg64, QUAD>(a, v, p);
-                else emit_leaf<Agg32, QUAD>(a, v, p);
-            } else {
-                const uint64_t cb = a.off[l - 1];
-                const uint32_t capc = (uint32_t)a.cap[l - 1];
-                uint32_t lv[8];
-                for (uint32_t j = 0; j < 8; j++) lv[j] = v + j < capc ? a.levels[cb + v + j] : 0u;
-                const TNode u0 = load_node(a, cb + v);
-                emit_parent<QUAD>(a, l, v, p, lv, u0, a.counts[l - 1]);
-            }
+    // Leftovers (a tile ran past its halo; the scan raised wlow[1] before this launch): the
+    // last block to finish builds them.  Each block's nodes are released by its count
+    // (agent-scope acq_rel after the barrier); the last block acquires them all.
+    if (a.wlow[1] != 0u) {
+        __shared__ uint32_t s_last;
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t done = __hip_atomic_fetch_add(&a.wlow[2], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            s_last = done + 1u == gridDim.x ? 1u : 0u;
         }
-        __syncthreads();  // layer l is complete before layer l + 1 reads it
+        __syncthreads();
+        if (s_last) {
+            __threadfence();
+            fused_fix<QUAD>(a, fcnt, fbase, ffix, lf, tiles);
+            if (tid == 0) __hip_atomic_store(&a.wlow[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -1095,12 +1116,16 @@ static uint32_t* fused_ffix(uint32_t* fbuf, uint32_t tiles) {
 }
 
 void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
-                       hipStream_t s) {
+                       uint32_t* clr, uint32_t nclr, hipStream_t s) {
     const uint32_t tiles = fused_tiles(a.nt);
-    if (!a.nt) return;
+    if (!a.nt) {
+        launch_fill_u32(clr, 0, nclr, s);
+        return;
+    }
     const uint32_t nl = a.bits;
     hipLaunchKernelGGL(k_fused_heads, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
-                       fused_fcnt(fbuf, tiles), fused_ffix(fbuf, tiles), lf, nl, tiles);
+                       fused_fcnt(fbuf, tiles), fused_ffix(fbuf, tiles), lf, nl, tiles, clr,
+                       nclr);
     hipLaunchKernelGGL(k_fused_scan, dim3(nl), dim3(kFusedThreads), 0, s, a,
                        fused_fcnt(fbuf, tiles), fused_fbase(fbuf, tiles), lf, tiles);
     hipLaunchKernelGGL(k_next_fill, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
@@ -1117,13 +1142,9 @@ void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, 
     if (a.quad) {
         hipLaunchKernelGGL(k_fused_build<true>, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
                            fcnt, fbase, ffix, lf, tiles);
-        hipLaunchKernelGGL(k_fused_fix<true>, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase,
-                           ffix, lf, tiles);
     } else {
         hipLaunchKernelGGL(k_fused_build<false>, dim3(tiles), dim3(kFusedThreads), 0, s, a,
                            plevel, fcnt, fbase, ffix, lf, tiles);
-        hipLaunchKernelGGL(k_fused_fix<false>, dim3(1), dim3(kFusedThreads), 0, s, a, fcnt, fbase,
-                           ffix, lf, tiles);
     }
 }
 

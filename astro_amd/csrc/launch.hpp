@@ -168,7 +168,8 @@ struct TreeArgs {
     float eps2;
     uint32_t side_shifted;      // ASTRO_SIDE_SHIFTED: side0 = 2^21
     const float4* cubes;
-    uint32_t* wlow;             // [2]: W, and whether any tile left a node for k_fused_fix
+    uint32_t* wlow;             // [3]: W, whether any tile left a node for fused_fix, and
+                                //      the fused build's finished-block count (left 0)
 };
 
 void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s);
@@ -202,8 +203,9 @@ uint32_t fused_tiles(uint32_t nt);
 inline uint64_t fused_buf_words(uint32_t nt) {
     return (2ull * kMaxLayers + kFusedMaxLayers) * fused_tiles(nt);
 }
+// Also zeroes clr[0 .. nclr) (the per-layer kernels' counts) before any layer kernel runs.
 void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
-                       hipStream_t s);
+                       uint32_t* clr, uint32_t nclr, hipStream_t s);
 void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, uint32_t* fbuf,
                          hipStream_t s);
 // The top layers with at most kTopCandidates candidates each in one single-block launch:
