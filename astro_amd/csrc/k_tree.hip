@@ -22,6 +22,9 @@ void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s) {
 }
 
 constexpr uint32_t kBoxTile = 4096;
+// positions per block of the single-system cube (1,024 and 2,048 measured slower: the
+// finishing atomics of more blocks cost more than the read they spread)
+constexpr uint32_t kCubeTile = 4096;
 
 __device__ inline float wave_min(float v) {
 #pragma unroll
@@ -162,10 +165,11 @@ __global__ __launch_bounds__(256) void k_cube_single(const float4* __restrict__ 
                                                      float4* cubes) {
     __shared__ uint32_t s_red[6][4];
     __shared__ uint32_t s_last;
-    const uint64_t base = (uint64_t)blockIdx.x * kBoxTile;
-    const uint64_t end = base + kBoxTile < n ? base + kBoxTile : n;
+    const uint64_t base = (uint64_t)blockIdx.x * kCubeTile;
+    const uint64_t end = base + kCubeTile < n ? base + kCubeTile : n;
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-#pragma unroll 4
+    // every load of the tile in flight at once (the kernel is one memory round trip long)
+#pragma unroll 16
     for (uint64_t e = base + threadIdx.x; e < end; e += 256) {
         const float4 p = pos[e];
         const uint32_t x = f2ord(p.x), y = f2ord(p.y), z = f2ord(p.z);
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(256) void k_cube_single(const float4* __restrict__ 
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
-uint32_t cube_single_blocks(uint32_t n) { return n ? (n + kBoxTile - 1) / kBoxTile : 1u; }
+uint32_t cube_single_blocks(uint32_t n) { return n ? (n + kCubeTile - 1) / kCubeTile : 1u; }
 void launch_cube_single(const float4* pos, uint32_t n, uint32_t* part, uint32_t* counter,
                         float4* cubes, hipStream_t s) {
     hipLaunchKernelGGL(k_cube_single, dim3(cube_single_blocks(n)), dim3(256), 0, s, pos, n, part,
