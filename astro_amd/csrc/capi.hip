@@ -1137,8 +1137,10 @@ static int enqueue_layers(astro_ctx* c, bool emit = true, bool full = false) {
     // the next pointers of every layer's nodes, before any node is written; the supertile sums
     // and tile counts the layer kernels accumulate start at zero (cleared there too)
     launch_heads_next(a, lf, c->plevel, c->fused_buf, c->scratch, (uint32_t)(nsup + ntile), s);
-    if (lf) launch_fused_layers(a, lf, c->plevel, c->fused_buf, s);
-    if (lf < l0) launch_layer_count(a, lf, tc[lf], sup[lf], s);
+    // the fused build counts layer lf's heads itself; without fused layers a launch does
+    if (lf) launch_fused_layers(a, lf, c->plevel, c->fused_buf, lf < l0 ? tc[lf] : nullptr,
+                                lf < l0 ? sup[lf] : nullptr, s);
+    else if (lf < l0) launch_layer_count(a, lf, tc[lf], sup[lf], s);
     for (uint32_t l = lf; l < l0; l++) {
         const bool last = l + 1 == l0;
         launch_layer_emit(a, l, tc[l], sup[l], last ? nullptr : tc[l + 1],

@@ -924,7 +924,8 @@ __device__ inline void mom_nodes(const float4* v, const float* sm, uint32_t c0, 
 // block to finish, after every other block's nodes are visible.
 template <bool QUAD>
 __device__ void fused_fix(const TreeArgs& a, const uint32_t* fcnt, const uint32_t* fbase,
-                          const uint32_t* ffix, uint32_t lf, uint32_t tiles) {
+                          const uint32_t* ffix, uint32_t lf, uint32_t tiles, uint32_t* lcnt,
+                          uint32_t* lsup) {
     for (uint32_t l = 0; l < lf; l++) {
         for (uint32_t t = threadIdx.x; t < tiles; t += kFusedThreads) {
             const uint32_t v = ffix[(uint64_t)l * tiles + t];
@@ -941,7 +942,11 @@ __device__ void fused_fix(const TreeArgs& a, const uint32_t* fcnt, const uint32_
                 uint32_t lv[8];
                 for (uint32_t j = 0; j < 8; j++) lv[j] = v + j < capc ? a.levels[cb + v + j] : 0u;
                 const TNode u0 = load_node(a, cb + v);
-                emit_parent<QUAD>(a, l, v, p, lv, u0, a.counts[l - 1]);
+                const uint32_t h = emit_parent<QUAD>(a, l, v, p, lv, u0, a.counts[l - 1]);
+                if (l + 1 == lf && lcnt && h >= lf) {  // a head of layer lf (as counted above)
+                    atomicAdd(&lcnt[p / kLayerTile], 1u);
+                    atomicAdd(&lsup[(p / kLayerTile / kSuper) * kSuperStride], 1u);
+                }
             }
         }
         __syncthreads();  // layer l is complete before layer l + 1 reads it
@@ -954,8 +959,10 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
                                                                const uint32_t* __restrict__ fcnt,
                                                                const uint32_t* __restrict__ fbase,
                                                                uint32_t* ffix, uint32_t lf,
-                                                               uint32_t tiles) {
+                                                               uint32_t tiles, uint32_t* lcnt,
+                                                               uint32_t* lsup) {
     __shared__ float4 s_val[kFusedRange];     // the current layer's nodes (centre, mass)
+    __shared__ uint32_t s_lc[8];              // heads of layer lf per candidate tile
     __shared__ float s_mom[QUAD ? 6 * kFusedRange : 1];  // and their second moments
     __shared__ uint16_t s_rank[kFusedRange];  // per position: nodes of the layer before it
     __shared__ uint16_t s_pos[kFusedRange];   // per node of the layer: its start position
@@ -965,6 +972,7 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
     const uint32_t a0 = t * kFusedTile;
     const uint32_t b0 = min(a0 + kFusedTile, a.nt);
     const uint32_t own = b0 - a0;  // positions [0, own) are this tile's
+    if (tid < 8) s_lc[tid] = 0u;   // read after the layer loop's barriers
     // the range ends at the next head at layer lf-1 after the tile (every node of the lower
     // layers ends there), or at the halo's end
     uint32_t cand = kFusedHalo;
@@ -1081,6 +1089,24 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
             a.levels[nd] = (uint8_t)h;
             if (QUAD) store_moments(a, nd, S[i], tn.last, l, tn.next);
         }
+        if (l + 1 == lf && lcnt) {
+            // layer lf's heads among this block's nodes of layer lf - 1 (at most 5 candidate
+            // tiles of kLayerTile), added to the tile counts and supertile sums the first
+            // per-layer kernel reads (in place of a counting launch)
+            const uint32_t t0 = base / kLayerTile;
+#pragma unroll
+            for (uint32_t i = 0; i < 2; i++) {
+                const uint32_t k = tid + i * kFusedThreads;
+                if (k >= tot || st[i] >= own || !done[i] || base + k >= a.cap[l]) continue;
+                if (s_h[st[i]] >= lf) atomicAdd(&s_lc[(base + k) / kLayerTile - t0], 1u);
+            }
+            lds_barrier();
+            if (tid < 8 && s_lc[tid]) {
+                const uint32_t tt = t0 + tid;
+                atomicAdd(&lcnt[tt], s_lc[tid]);
+                atomicAdd(&lsup[(tt / kSuper) * kSuperStride], s_lc[tid]);
+            }
+        }
         // per position, the nodes of this layer before it (the next layer's child ranks)
         if (q0 < np) s_rank[q0] = (uint16_t)r0;
         if (q1 < np) s_rank[q1] = (uint16_t)r1;
@@ -1103,7 +1129,7 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
         __syncthreads();
         if (s_last) {
             __threadfence();
-            fused_fix<QUAD>(a, fcnt, fbase, ffix, lf, tiles);
+            fused_fix<QUAD>(a, fcnt, fbase, ffix, lf, tiles, lcnt, lsup);
             if (tid == 0) __hip_atomic_store(&a.wlow[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -1137,7 +1163,7 @@ void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t
 }
 
 void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, uint32_t* fbuf,
-                         hipStream_t s) {
+                         uint32_t* lcnt, uint32_t* lsup, hipStream_t s) {
     const uint32_t tiles = fused_tiles(a.nt);
     if (!a.nt || lf == 0) return;
     uint32_t* fcnt = fused_fcnt(fbuf, tiles);
@@ -1145,10 +1171,10 @@ void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, 
     uint32_t* ffix = fused_ffix(fbuf, tiles);
     if (a.quad) {
         hipLaunchKernelGGL(k_fused_build<true>, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
-                           fcnt, fbase, ffix, lf, tiles);
+                           fcnt, fbase, ffix, lf, tiles, lcnt, lsup);
     } else {
         hipLaunchKernelGGL(k_fused_build<false>, dim3(tiles), dim3(kFusedThreads), 0, s, a,
-                           plevel, fcnt, fbase, ffix, lf, tiles);
+                           plevel, fcnt, fbase, ffix, lf, tiles, lcnt, lsup);
     }
 }
 

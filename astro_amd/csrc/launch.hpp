@@ -65,7 +65,7 @@ void launch_radix_pass(const uint64_t* kin, const uint32_t* iin, uint64_t* kout,
 constexpr uint32_t kResortThreads = 1024;      // the sample block
 constexpr uint32_t kResortBucketThreads = 512; // bucket blocks
 constexpr uint32_t kResortPlaceThreads = 256;
-constexpr uint32_t kResortMean = 896;          // buckets = ceil(n / kResortMean)
+constexpr uint32_t kResortMean = 980;          // buckets = ceil(n / kResortMean)
 constexpr uint32_t kResortRegRows = 2 * kResortBucketThreads;  // sorted in registers up to this
 constexpr uint32_t kResortCap = 2048;          // slots per bucket (sorted in LDS up to this)
 constexpr uint32_t kResortMaxBuckets = 8192;
@@ -206,8 +206,10 @@ inline uint64_t fused_buf_words(uint32_t nt) {
 // Also zeroes clr[0 .. nclr) (the per-layer kernels' counts) before any layer kernel runs.
 void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t* fbuf,
                        uint32_t* clr, uint32_t nclr, hipStream_t s);
+// lcnt / lsup (or null): the tile counts and supertile sums of layer lf's candidates, which the
+// fused build adds to (zeroed before it), so that no counting launch precedes layer lf.
 void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, uint32_t* fbuf,
-                         hipStream_t s);
+                         uint32_t* lcnt, uint32_t* lsup, hipStream_t s);
 // The top layers with at most kTopCandidates candidates each in one single-block launch:
 // layers_top_first(a) is the first of them (a.layers if none; layer 0 never), and the layer
 // before it is emitted without next-layer counts.

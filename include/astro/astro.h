@@ -58,8 +58,8 @@ enum { ASTRO_EMIT_NEVER = 0, ASTRO_EMIT_EVERY_BUILD = 1, ASTRO_EMIT_ON_READ = 2 
  *   RADIX   an LSD radix sort from scratch, three launches per 8-bit digit;
  *   SEEDED  buckets between splitters sampled from the previous sort's order, each sorted in
  *           LDS (default; after a first radix sort of the current sources, while the buckets
- *           number at most 8,192: one system of at most 7,340,032 sources, or a batch, whose
- *           systems get ceil(n_s / 896) buckets each; otherwise RADIX). */
+ *           number at most 8,192: one system of at most 8,028,160 sources, or a batch, whose
+ *           systems get ceil(n_s / 980) buckets each; otherwise RADIX). */
 enum { ASTRO_SORT_RADIX = 0, ASTRO_SORT_SEEDED = 1 };
 
 /* Buffers reachable through astro_get_buffer (names follow Renderer.swift:56-70). */

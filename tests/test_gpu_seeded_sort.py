@@ -11,7 +11,7 @@ for bit:
   gathered from the overflow list and sorted in device memory);
 * with many equal keys (sources on a few hundred points);
 * with fewer tree sources than particles, and at sizes around the bucket size and the bucket
-  count limit (7,340,032 sources; one more falls back to the radix sort);
+  count limit (8,028,160 sources; one more falls back to the radix sort);
 and whole steps with either sort give the same bytes.  Each test prints its case before
 touching the GPU."""
 import sys
@@ -105,7 +105,7 @@ def test_seeded_after_permuting_positions(astro, dups, change):
         assert list(ref.layer_counts()) == list(counts)
 
 
-@pytest.mark.parametrize("n,nt", [(895, 0), (896, 0), (897, 0), (1024, 0), (1025, 0), (1793, 0),
+@pytest.mark.parametrize("n,nt", [(979, 0), (980, 0), (981, 0), (1024, 0), (1025, 0), (1961, 0),
                                   (2049, 0), (5000, 0), (70001, 0), (2000, 205), (100000, 33333)])
 def test_seeded_sizes(astro, n, nt):
     # one bucket, a bucket boundary, tree sources fewer than particles; a permuted previous
@@ -150,9 +150,9 @@ def test_seeded_and_radix_steps_same_bytes(astro):
     assert same_bytes(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("n,seeded", [(7340032, True), (7340033, False)])
+@pytest.mark.parametrize("n,seeded", [(8028160, True), (8028161, False)])
 def test_seeded_bucket_limit(astro, n, seeded):
-    # 8,192 buckets of 896 sources at most: one source more sorts with the radix sort
+    # 8,192 buckets of 980 sources at most: one source more sorts with the radix sort
     say("seeded sort: bucket limit, n", n)
     pos, vel, col = astro.ic_plummer(n, 5)
     with astro.Context(n, bits_per_axis=21, emit_octree_nodes=False) as ctx:
@@ -182,7 +182,7 @@ def batch(astro, sizes, seed0):
     return pos, vel, col, off, sys_of
 
 
-BATCH_SIZES = [1, 500, 896, 897, 3000, 20000, 1793, 70001, 2]
+BATCH_SIZES = [1, 500, 980, 981, 3000, 20000, 1961, 70001, 2]
 
 
 def test_seeded_batch_steps_and_permutations(astro):
