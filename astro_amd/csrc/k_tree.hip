@@ -546,31 +546,50 @@ __global__ __launch_bounds__(256) void k_leaf_emit(TreeArgs a, const uint32_t* _
 // c + 7 (read as 0 past the layer's capacity); n is the candidate count.  The children are c
 // and the candidates after it up to the next head (at most 8); they are aggregated serially in
 // order (Octree.metal:157-172).  Returns the parent's head level.
-template <bool QUAD>
+template <bool QUAD, bool PRE = false>
 __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, uint32_t c,
                                                 uint32_t p, const uint32_t* lv, const TNode& u0,
-                                                uint32_t n) {
+                                                uint32_t n, const float4* pre = nullptr) {
     const uint64_t cb = a.off[l - 1];
     const bool fp64 = l >= a.fp64_from;
     uint32_t kc = 1;
 #pragma unroll
     for (uint32_t j = 1; j < 8; j++) kc += (kc == j && c + j < n && lv[j] < l) ? 1u : 0u;
-    const float4* ch = a.tpos + cb + c;
+    // the children's centres: preloaded by the caller (PRE: pre[j], j < 8, indexed with
+    // constants once the loops unroll, so the array stays in registers), or read here
+    const float4* gch = a.tpos + cb + c;
+    auto ch = [&](uint32_t j) { return PRE ? pre[j] : gch[j]; };  // (constant j when PRE)
     float cm[4];
     if (fp64) {
         Agg64 g;
         g.add(u0.x, u0.y, u0.z, u0.m);
-        for (uint32_t j = 1; j < kc; j++) {
-            const float4 v = ch[j];
-            g.add(v.x, v.y, v.z, v.w);
+        if constexpr (PRE) {
+#pragma unroll
+            for (uint32_t j = 1; j < 8; j++) {  // unrolled: the array stays in registers
+                if (j >= kc) break;
+                g.add(pre[j].x, pre[j].y, pre[j].z, pre[j].w);
+            }
+        } else {
+            for (uint32_t j = 1; j < kc; j++) {
+                const float4 v = gch[j];
+                g.add(v.x, v.y, v.z, v.w);
+            }
         }
         g.fin(cm);
     } else {
         Agg32 g;
         g.add(u0.x, u0.y, u0.z, u0.m);
-        for (uint32_t j = 1; j < kc; j++) {
-            const float4 v = ch[j];
-            g.add(v.x, v.y, v.z, v.w);
+        if constexpr (PRE) {
+#pragma unroll
+            for (uint32_t j = 1; j < 8; j++) {  // unrolled: the array stays in registers
+                if (j >= kc) break;
+                g.add(pre[j].x, pre[j].y, pre[j].z, pre[j].w);
+            }
+        } else {
+            for (uint32_t j = 1; j < kc; j++) {
+                const float4 v = gch[j];
+                g.add(v.x, v.y, v.z, v.w);
+            }
         }
         g.fin(cm);
     }
@@ -583,7 +602,7 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
     t.m = cm[3];
     t.last = (uint32_t)(cb + c + kc - 1);
     t.info = tn_info(l, kc, level);
-    t.next = a.pnext[u0.start];
+    t.next = u0.next;  // every node starting at u0.start has the same next (pnext of it)
     t.start = u0.start;
     store_node(a, nd, t);
     a.levels[nd] = (uint8_t)level;
@@ -592,19 +611,23 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
         const float4* cq = a.qmom + 2 * (cb + c);
         if (fp64) {
             Mom64 q;
-            for (uint32_t j = 0; j < kc; j++) {
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                if (j >= kc) break;  // (unrolled; kc <= 8)
                 const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
                 const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                const float4 v = ch[j];
+                const float4 v = ch(j);
                 q.add(v.x, v.y, v.z, v.w, cm, Sc);
             }
             q.fin(S);
         } else {
             Mom32 q;
-            for (uint32_t j = 0; j < kc; j++) {
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                if (j >= kc) break;  // (unrolled; kc <= 8)
                 const float4 q0 = cq[2 * j], q1 = cq[2 * j + 1];
                 const float Sc[6] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y};
-                const float4 v = ch[j];
+                const float4 v = ch(j);
                 q.add(v.x, v.y, v.z, v.w, cm, Sc);
             }
             q.fin(S);
@@ -689,15 +712,24 @@ __global__ __launch_bounds__(kTopThreads) void k_layers_top(TreeArgs a, uint32_t
         uint32_t carry = 0;
         for (uint32_t base = 0; base < n; base += kTopThreads) {
             const uint32_t c = base + threadIdx.x;
+            // every load of the round up front: levels, the candidate and (with quadrupoles, whose
+            // children's rows make a long chain otherwise) the seven nodes after it, so a layer
+            // waits on one round of latency; only candidates below the count are read.
+            // Monopoles measured faster loading the further children in emit_parent.
             uint32_t lv[8];
+            float4 ch[8];
 #pragma unroll
-            for (uint32_t j = 0; j < 8; j++) lv[j] = c + j < capc ? a.levels[cb + c + j] : 0u;
-            const TNode u0 = load_node(a, cb + (c < capc ? c : 0u));
+            for (uint32_t j = 0; j < 8; j++) {
+                const bool in = c + j < n && c + j < capc;  // capc: a count past capacity
+                lv[j] = in ? a.levels[cb + c + j] : 0u;
+                if (QUAD) ch[j] = in ? a.tpos[cb + c + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            const TNode u0 = load_node(a, cb + (c < n && c < capc ? c : 0u));
             const bool head = c < n && lv[0] >= l;
             uint32_t total;
             const uint32_t p = carry + block_exclusive_scan<kTopThreads>(head ? 1u : 0u, s_tmp,
                                                                         &total);
-            if (head && p < a.cap[l]) emit_parent<QUAD>(a, l, c, p, lv, u0, n);
+            if (head && p < a.cap[l]) emit_parent<QUAD, QUAD>(a, l, c, p, lv, u0, n, ch);
             carry += total;
         }
         if (threadIdx.x == 0) {
