@@ -88,6 +88,18 @@ __device__ inline float sqrt_rn_normal(float x) {
     return s;
 }
 
+// RN(sqrt(x)) for finite x >= 2^-96 in five operations: one Goldschmidt step from the hardware
+// reciprocal square root y ~ 1/sqrt(x) (g = x y, residual x - g^2 in one fma, g + residual y/2).
+// Not an error bound but an exhaustive result: on gfx950 it equals the correctly rounded square
+// root for every float of [2^-96, FLT_MAX] (k_check_sqrt checks all of them).  x = +inf gives
+// NaN, so callers use it only where x is bounded.
+__device__ inline float sqrt_rn_rsq(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float g = x * y, h = 0.5f * y;
+    const float r = __builtin_fmaf(-g, g, x);
+    return __builtin_fmaf(r, h, g);
+}
+
 // Inclusive scan across the 64 lanes of a wave.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;

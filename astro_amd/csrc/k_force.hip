@@ -161,12 +161,13 @@ struct Visits {
 // that fit, chosen from the layer counts) into LDS before its walks start, and rewrites their
 // links to name LDS slots (bit 31 set).  Every target walks the top layers, and its 64 lanes
 // read scattered nodes there: from LDS those reads leave the texture address unit.
-//   A top node is named by the byte offset of its rows in the LDS arrays (slot << 4) with a
-//     marker bit that no global index has: bit 31 for monopoles (rows: centre and mass;
-//     last', info, next'), bit 26 with quadrupoles (trees of fewer than 2^26 nodes; rows:
-//     centre and mass, Sxx Syy Szz Sxy, and Sxz Syz (last' | layer << 27) next', the global
-//     moment rows' layout).  A visit reads either source into the same registers with one
-//     address operation and decodes both alike.
+//   A top node is named with a marker bit that no global index has.  Monopoles: its slot with
+//     bit 31 set (rows: centre and mass; last', info, next'), so that name << 4, the byte
+//     offset of a global node's rows, is also the byte offset of its LDS rows (the marker
+//     shifts out).  Quadrupoles: the byte offset of its LDS rows (slot << 4) with bit 26 set
+//     (trees of fewer than 2^26 nodes; rows: centre and mass, Sxx Syy Szz Sxy, and Sxz Syz
+//     (last' | layer << 27) next', the global moment rows' layout).  A visit reads either
+//     source into the same registers and decodes both alike.
 //   last' of a top node of layer l > lt names its last child (a top node of layer l - 1); a
 //     node of layer lt keeps its child's global index (bit 31 clear); a top leaf (lt = 0) is
 //     never opened.
@@ -177,7 +178,7 @@ struct Visits {
 // next' (`resume`) and follows it wherever a global next points at or above off[lt].
 constexpr uint32_t kTopBit = 0x80000000u;   // monopoles
 constexpr uint32_t kTopQuad = 0x04000000u;  // quadrupoles (global indices below 2^26)
-constexpr uint32_t kTopAddrMask = 0x0000FFFFu;
+constexpr uint32_t kTopAddrMask = 0x0000FFFFu;  // quadrupoles: a top name's LDS byte offset
 // slots per block: 80 KB of LDS per 1,024 threads (two blocks per CU), 32 B per node for
 // monopoles (centre and mass; last', info, next', one free word), 48 B with the moment rows
 template <int NT, bool QUAD>
@@ -186,7 +187,7 @@ struct TopCap {
 };
 template <bool QUAD>
 __device__ inline uint32_t top_name(uint32_t slot) {
-    return (QUAD ? kTopQuad : kTopBit) | (slot << 4);
+    return QUAD ? (kTopQuad | (slot << 4)) : (kTopBit | slot);
 }
 // LDS pointers keep their address space, so that the walk's LDS and global reads stay
 // separate ds_read and global_load instructions (a generic pointer would turn both into flat
@@ -243,19 +244,18 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             // monopoles: one source or the other into the same registers; the LDS reads are
             // issued first, so that the global loads wait at most for them (not the reverse)
             uint32_t info;
+            const uint32_t o = n << 4;  // the rows' byte offset in either source
             if ((int32_t)n >= 0) {
-                const uint32_t o = n << 4;
                 cm = *reinterpret_cast<const float4*>(pos_base + o);
                 const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
                 last = li.last;
                 info = li.info;
                 next = li.next;
             } else {
-                const uint32_t ad = n & kTopAddrMask;
                 const lds_f4 c = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
-                    reinterpret_cast<const ASTRO_LDS char*>(tv.pos) + ad);
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.pos) + o);
                 const lds_u3 k = *reinterpret_cast<const ASTRO_LDS lds_u3*>(
-                    reinterpret_cast<const ASTRO_LDS char*>(tv.lnk) + ad);
+                    reinterpret_cast<const ASTRO_LDS char*>(tv.lnk) + o);
                 cm = make_float4(c.x, c.y, c.z, c.w);
                 last = k.x;
                 info = k.y;
@@ -342,7 +342,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             const bool interact = lane_in((mleaf & mself) | (~mleaf & mfar));
             open = lane_in(~mleaf & ~mfar);
             if (interact) {
-                const float rr = sqrt_rn_normal(r2);
+                const float rr = sqrt_rn_rsq(r2);  // FDIV: r2 is bounded
                 const float D = r2 * rr;
                 const float s = div_rn_mid(gm, D);
                 if (KAHAN) {
@@ -379,7 +379,9 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
                 st.opens += open;
             }
         } else {
-            const float rr = FAST ? sqrt_rn_normal(r2) : fmaxf(sqrtf(r2), rf);
+            // FDIV bounds r2 (every cube within +-div_lmax), as sqrt_rn_rsq needs
+            const float rr = FAST ? (FDIV ? sqrt_rn_rsq(r2) : sqrt_rn_normal(r2))
+                                  : fmaxf(sqrtf(r2), rf);
             // (plain selects here: mask operations measured slower on this path)
             const bool far = rr >= ldexpf(rmin0, (int)layer);
             const bool hit = (n != self) & (FAST || r2 > cut);
@@ -658,9 +660,10 @@ int check_open_test(uint64_t n, uint64_t seed, unsigned long long* d_mismatch, h
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// Exhaustive self-check of sqrt_rn_normal: every float x in [2^-96, FLT_MAX] against the
-// double square root rounded to float (correctly rounded: 53 >= 2 * 24 + 2 bits make the
-// double rounding innocuous).  The raw hardware estimate alone differs on about 15% of them.
+// Exhaustive self-check of both traversal square roots, sqrt_rn_normal and sqrt_rn_rsq: every
+// float x in [2^-96, FLT_MAX] against the double square root rounded to float (correctly
+// rounded: 53 >= 2 * 24 + 2 bits make the double rounding innocuous); a mismatch of either
+// counts.  The raw hardware estimate alone differs on about 15% of them.
 __global__ void k_check_sqrt(uint32_t lo, uint32_t hi, unsigned long long* mismatch) {
     unsigned long long bad = 0;
     for (uint64_t u = lo + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < hi;
@@ -668,6 +671,7 @@ __global__ void k_check_sqrt(uint32_t lo, uint32_t hi, unsigned long long* misma
         const float x = __uint_as_float((uint32_t)u);
         const float ref = (float)__builtin_sqrt((double)x);
         bad += __float_as_uint(sqrt_rn_normal(x)) != __float_as_uint(ref);
+        bad += __float_as_uint(sqrt_rn_rsq(x)) != __float_as_uint(ref);
     }
     bad = wave_sum_u64(bad);
     if ((threadIdx.x & 63u) == 0u && bad) atomicAdd(mismatch, bad);
@@ -727,7 +731,7 @@ __device__ __forceinline__ void walk_wave(const ForceArgs& a, float4 p, uint32_t
         const bool on = resume == kOn;
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
-        const float rr = sqrt_rn_normal(r2);
+        const float rr = FDIV ? sqrt_rn_rsq(r2) : sqrt_rn_normal(r2);
         const bool far = rr >= ldexpf(rmin0, (int)layer);
         const bool leaf = layer == 0u;
         const float gm = G1 ? cm.w : G * cm.w;

@@ -270,10 +270,11 @@ int astro_check_division(astro_ctx* ctx, uint64_t n, uint64_t seed, uint64_t* mi
  * rmin0.  Compares both forms on n pseudo-random cases near the threshold; *mismatches
  * receives the count of disagreements (0 expected).  No reference counterpart.  Synchronous. */
 int astro_check_open_test(astro_ctx* ctx, uint64_t n, uint64_t seed, uint64_t* mismatches);
-/* Exhaustive self-check of the traversal's correctly rounded square root (the fast path's
- * hardware estimate plus a one-ulp correction) on every float in [2^-96, FLT_MAX]: *checked
- * receives the count, *mismatches the disagreements with the double-precision square root
- * rounded to float (0 expected).  No reference counterpart.  Synchronous, about 0.1 s. */
+/* Exhaustive self-check of the traversal's correctly rounded square roots (the hardware
+ * estimate plus a one-ulp correction, and the hardware reciprocal square root plus one
+ * Goldschmidt step) on every float in [2^-96, FLT_MAX]: *checked receives the count,
+ * *mismatches the disagreements of either with the double-precision square root rounded to
+ * float (0 expected).  No reference counterpart.  Synchronous, about 0.1 s. */
 int astro_check_sqrt(astro_ctx* ctx, uint64_t* checked, uint64_t* mismatches);
 /* Exhaustive self-check of the reciprocal behind the traversal's fast division (the hardware
  * estimate refined by one Newton step, which the division then uses once, by Markstein's

@@ -487,8 +487,9 @@ def test_fast_division_matches_ieee(astro, small_ctx):
 
 
 def test_square_root_exhaustive(astro, small_ctx):
-    # the fast path's square root (hardware estimate + one-ulp correction) on every float in
-    # [2^-96, FLT_MAX] against the double square root rounded to float
+    # the fast paths' square roots (hardware estimate + one-ulp correction; reciprocal square
+    # root + one Goldschmidt step) on every float in [2^-96, FLT_MAX] against the double square
+    # root rounded to float
     say("exhaustive square-root self-check")
     n, bad = small_ctx.check_sqrt()
     assert n == (0x7F800000 - ((127 - 96) << 23)) and bad == 0
