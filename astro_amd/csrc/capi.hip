@@ -913,12 +913,6 @@ static ForceArgs force_args(astro_ctx* c) {
                     (c->mass_min_pos == INFINITY || G * c->mass_min_pos >= lo) &&
                     G * c->mass_total_max <= hi && 12.0 * L * L + p.eps2 <= 0x1p30;
     a.div_lmax = ok ? (float)L : 0.0f;
-    // The quadrupole term's 1 / x, x = RN(RN(r2 RN(sqrt(r2))) r2), by rcp_rn_mid (exact on
-    // [2^-45, 2^46)): with every cube within +-64 and eps2 <= 2^15.6, r2 <= 1.5 * 2^16 and
-    // x < 2^42 (rounding is monotone); r2 >= 2^-18 gives x >= 2^-45.
-    const double Lq = 64.0;
-    a.qrcp_lmax = ok && r2min >= 0x1p-18 && 12.0 * Lq * Lq + p.eps2 <= 0x1p16 + 0x1p15
-                      ? (float)Lq : 0.0f;
     return a;
 }
 

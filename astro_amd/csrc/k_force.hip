@@ -66,25 +66,25 @@ __device__ inline float rcp_rn_mid(float d) {
 // (one fma), and RN(q0 + residual y) = RN(n / d).  This replaces the compiler's IEEE division
 // sequence (operand scaling, two remainder corrections, special-case fix-up), which changes
 // nothing on that domain.
-__device__ inline float div_rn_mid(float n, float d) {
-    const float y = rcp_rn_mid(d);
+__device__ inline float div_rn_mid_y(float n, float d, float y) {  // y = RN(1 / d)
     const float q0 = n * y;
     const float t = __builtin_fmaf(-d, q0, n);
     return __builtin_fmaf(t, y, q0);
 }
+__device__ inline float div_rn_mid(float n, float d) { return div_rn_mid_y(n, d, rcp_rn_mid(d)); }
 
 // Quadrupole term of an accepted internal node with second moments S (xx, yy, zz, xy, xz, yz)
 // about its centre of mass, d = COM - target, D = r2 * r:
 //   a_Q = u * q,  u = 3G v,  q = w d - (S d + T/2 d),  w = 2.5 (d.S.d) (v D),  T = tr S,
-// with v = 1 / (D r2) the one division (v D is 1/r2 to within a rounding).
+// with v = 1/r^5 = (y y) r from y = RN(1 / D), the reciprocal the monopole term's division
+// already computed (v D is 1/r2 to within a few roundings).
 // The trace term is what Plummer softening adds to the expansion of  -G m / sqrt(|d|^2 + eps2).
 struct QuadTerm {
     float u, q0, q1, q2;
     double phi;  // -G (1.5 d.S.d - 0.5 T r2) / (D r2), for the potential diagnostics
 };
-template <bool QRCP>
 __device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, float dz, float r2,
-                                     float D, float G, bool qrcp) {
+                                     float rr, float D, float y, float G) {
     const float Sxx = s0.x, Syy = s0.y, Szz = s0.z, Sxy = s0.w, Sxz = s1.x, Syz = s1.y;
     const float Sd0 = __builtin_fmaf(Sxz, dz, __builtin_fmaf(Sxy, dy, Sxx * dx));
     const float Sd1 = __builtin_fmaf(Syz, dz, __builtin_fmaf(Syy, dy, Sxy * dx));
@@ -92,9 +92,7 @@ __device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, f
     const float dSd = __builtin_fmaf(dz, Sd2, __builtin_fmaf(dy, Sd1, dx * Sd0));
     const float T = (Sxx + Syy) + Szz;
     const float ht = 0.5f * T;
-    // qrcp: D r2 lies in [2^-45, 2^46), where rcp_rn_mid is RN(1 / x) (wave-uniform)
-    const float x = D * r2;
-    const float v = (QRCP && qrcp) ? rcp_rn_mid(x) : 1.0f / x;
+    const float v = (y * y) * rr;
     const float w = (2.5f * dSd) * (v * D);
     QuadTerm t;
     t.u = (3.0f * G) * v;
@@ -226,7 +224,7 @@ template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64, boo
           bool TOPC>
 __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, uint32_t root,
                                          float4 p, uint32_t self, float rmin0, float r2min0,
-                                         bool qrcp, Acc3& F, double& phi, Visits& st) {
+                                         Acc3& F, double& phi, Visits& st) {
     constexpr bool R2 = FAST && FDIV && QUAD;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
@@ -344,10 +342,11 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             if (interact) {
                 const float rr = sqrt_rn_rsq(r2);  // FDIV: r2 is bounded
                 const float D = r2 * rr;
-                const float s = div_rn_mid(gm, D);
+                const float y = rcp_rn_mid(D);
+                const float s = div_rn_mid_y(gm, D, y);
                 if (KAHAN) {
                     if (QUAD && !leaf) {
-                        const QuadTerm qt = quad_term<true>(qm0, qm1, dx, dy, dz, r2, D, G, qrcp);
+                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, G);
                         accumulate_quad<true>(F, s, dx, dy, dz, qt);
                         if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
                     } else {
@@ -359,7 +358,7 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
                     float fx = __builtin_fmaf(s, dx, F.x), fy = __builtin_fmaf(s, dy, F.y),
                           fz = __builtin_fmaf(s, dz, F.z);
                     if (QUAD && !leaf) {
-                        const QuadTerm qt = quad_term<true>(qm0, qm1, dx, dy, dz, r2, D, G, qrcp);
+                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, G);
                         fx = __builtin_fmaf(qt.u, qt.q0, fx);
                         fy = __builtin_fmaf(qt.u, qt.q1, fy);
                         fz = __builtin_fmaf(qt.u, qt.q2, fz);
@@ -391,7 +390,8 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             const float D = r2 * rr;
             const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
             if (QUAD && interact && !leaf) {
-                const QuadTerm qt = quad_term<false>(qm0, qm1, dx, dy, dz, r2, D, G, false);
+                const float y = FDIV ? rcp_rn_mid(D) : 1.0f / D;
+                const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, G);
                 accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
                 if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
             } else if (KAHAN) {
@@ -527,11 +527,10 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
-    // +-div_lmax (then |d| <= 2 div_lmax per axis; masses are bounded on the host); the
-    // quadrupole term's fast reciprocal when they lie within +-qrcp_lmax
+    // +-div_lmax (then |d| <= 2 div_lmax per axis; masses are bounded on the host)
     // a node at layer l is accepted iff r >= rmin0 * 2^l (accept.hpp)
     float rmin0 = 0.0f;
-    bool fdiv_lane = true, qrcp_lane = true;
+    bool fdiv_lane = true;
     if (active) {
         const float4 cu = a.cubes[sys];
         float side0;
@@ -542,15 +541,8 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         fdiv_lane = L > 0.0f && fabsf(cu.x) <= L && fabsf(cu.y) <= L && fabsf(cu.z) <= L &&
                     fabsf(cu.x + cu.w) <= L && fabsf(cu.y + cu.w) <= L && fabsf(cu.z + cu.w) <= L;
         if (FAST && QUAD) fdiv_lane = fdiv_lane && accept_r2_ok(rmin0);
-        if (QUAD) {
-            const float Q = a.qrcp_lmax;
-            qrcp_lane = Q > 0.0f && fabsf(cu.x) <= Q && fabsf(cu.y) <= Q && fabsf(cu.z) <= Q &&
-                        fabsf(cu.x + cu.w) <= Q && fabsf(cu.y + cu.w) <= Q &&
-                        fabsf(cu.z + cu.w) <= Q;
-        }
     }
     const bool fdiv = __builtin_amdgcn_readfirstlane(__all(fdiv_lane ? 1 : 0)) != 0;
-    const bool qrcp = QUAD && __builtin_amdgcn_readfirstlane(__all(qrcp_lane ? 1 : 0)) != 0;
 
     Visits st;
     if (active) {
@@ -572,11 +564,10 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
         double phi = 0.0;
         if (fdiv)
             traverse<KAHAN, MODE, QUAD, FAST, true, OFF64, G1, TOPC>(
-                a, tv, root, p, self, rmin0, QUAD ? accept_r2min(rmin0) : 0.0f, qrcp, F, phi,
-                st);
+                a, tv, root, p, self, rmin0, QUAD ? accept_r2min(rmin0) : 0.0f, F, phi, st);
         else
             traverse<KAHAN, MODE, QUAD, FAST, false, OFF64, G1, TOPC>(a, tv, root, p, self, rmin0,
-                                                                      0.0f, false, F, phi, st);
+                                                                      0.0f, F, phi, st);
         const float c = a.p.force_clamp;
         a.force[pidx] = make_float4(clampf(F.x, c), clampf(F.y, c), clampf(F.z, c), 0.0f);
         if (MODE == 2) a.phi[pidx] = phi;
@@ -962,7 +953,7 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
                 // second moments sit in emittedColor (xx, yy, zz, xy) / emittedColorCenter
                 const float4 s0 = *reinterpret_cast<const float4*>(nd->emittedColor);
                 const float4 s1 = *reinterpret_cast<const float4*>(nd->emittedColorCenter);
-                const QuadTerm qt = quad_term<false>(s0, s1, dx, dy, dz, r2, D, a.p.G, false);
+                const QuadTerm qt = quad_term(s0, s1, dx, dy, dz, r2, rr, D, 1.0f / D, a.p.G);
                 if (a.p.kahan) accumulate_quad<true>(F, s, dx, dy, dz, qt);
                 else accumulate_quad<false>(F, s, dx, dy, dz, qt);
             } else if (a.p.kahan) {

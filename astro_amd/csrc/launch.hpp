@@ -282,8 +282,6 @@ struct ForceArgs {
     float div_lmax;             // > 0: when a system's cube lies within [-div_lmax, div_lmax]^3,
                                 //    every G M / (r2 r) of its targets is in the domain of
                                 //    div_rn_mid; 0: always use the full division
-    float qrcp_lmax;            // > 0: when a system's cube lies within +-qrcp_lmax, every
-                                //    quadrupole term's 1 / (D r2) is in rcp_rn_mid's domain
     astro_params p;
 };
 // Self-check of div_rn_mid against the IEEE division on n pseudo-random pairs of its domain.

@@ -519,7 +519,8 @@ void orc_barnes_hut(const OctreeNode* nodes, uint64_t root, const PositionMass* 
                 float q[3] = {0.0f, 0.0f, 0.0f}, u = 0.0f;
                 int quad = p->multipole == 2 && !leaf;
                 if (quad) {
-                    /* a_Q = 3G v (w d - (S d + T/2 d)), v = 1/(D r2), w = 2.5 (d.S.d) v D */
+                    /* a_Q = 3G v (w d - (S d + T/2 d)), v = 1/r^5 = (y y) r with y = 1/D,
+                     * w = 2.5 (d.S.d) v D */
                     float S[6];
                     mom_load(nd, S);
                     float Sd0 = fmaf(S[4], dz, fmaf(S[3], dy, S[0] * dx));
@@ -528,7 +529,8 @@ void orc_barnes_hut(const OctreeNode* nodes, uint64_t root, const PositionMass* 
                     float dSd = fmaf(dz, Sd2, fmaf(dy, Sd1, dx * Sd0));
                     float T = (S[0] + S[1]) + S[2];
                     float ht = 0.5f * T;
-                    float v = 1.0f / (D * r2); /* the one division; v D = 1/r2 to a rounding */
+                    float y = 1.0f / D;
+                    float v = (y * y) * r; /* v D = 1/r2 to within a few roundings */
                     float w = (2.5f * dSd) * (v * D);
                     u = (3.0f * p->G) * v;
                     q[0] = fmaf(w, dx, -fmaf(ht, dx, Sd0));
