@@ -74,57 +74,58 @@ __device__ inline float div_rn_mid_y(float n, float d, float y) {  // y = RN(1 /
 __device__ inline float div_rn_mid(float n, float d) { return div_rn_mid_y(n, d, rcp_rn_mid(d)); }
 
 // Quadrupole term of an accepted internal node with second moments S (xx, yy, zz, xy, xz, yz)
-// about its centre of mass, d = COM - target, D = r2 * r:
-//   a_Q = u * q,  u = 3G v,  q = w d - (S d + T/2 d),  w = 2.5 (d.S.d) (v D),  T = tr S,
+// about its centre of mass, d = COM - target, D = r2 * r, monopole coefficient s = G M / D:
+//   a = s d + u (w d - (S d + T/2 d)) = c d - u S d,
+//   c = s + u (w - T/2),  u = 3G v,  w = 2.5 (d.S.d) (v D),  T = tr S,
 // with v = 1/r^5 = (y y) r from y = RN(1 / D), the reciprocal the monopole term's division
-// already computed (v D is 1/r2 to within a few roundings).
+// already computed (v D is 1/r2 to within a few roundings).  Folding the terms along d into
+// one coefficient leaves one fma per component for them.
 // The trace term is what Plummer softening adds to the expansion of  -G m / sqrt(|d|^2 + eps2).
 struct QuadTerm {
-    float u, q0, q1, q2;
+    float c, u, sd0, sd1, sd2;  // the coefficient along d, u, and S d
     double phi;  // -G (1.5 d.S.d - 0.5 T r2) / (D r2), for the potential diagnostics
 };
 __device__ inline QuadTerm quad_term(float4 s0, float4 s1, float dx, float dy, float dz, float r2,
-                                     float rr, float D, float y, float G) {
+                                     float rr, float D, float y, float s, float G) {
     const float Sxx = s0.x, Syy = s0.y, Szz = s0.z, Sxy = s0.w, Sxz = s1.x, Syz = s1.y;
     const float Sd0 = __builtin_fmaf(Sxz, dz, __builtin_fmaf(Sxy, dy, Sxx * dx));
     const float Sd1 = __builtin_fmaf(Syz, dz, __builtin_fmaf(Syy, dy, Sxy * dx));
     const float Sd2 = __builtin_fmaf(Szz, dz, __builtin_fmaf(Syz, dy, Sxz * dx));
     const float dSd = __builtin_fmaf(dz, Sd2, __builtin_fmaf(dy, Sd1, dx * Sd0));
     const float T = (Sxx + Syy) + Szz;
-    const float ht = 0.5f * T;
     const float v = (y * y) * rr;
     const float w = (2.5f * dSd) * (v * D);
     QuadTerm t;
     t.u = (3.0f * G) * v;
-    t.q0 = __builtin_fmaf(w, dx, -__builtin_fmaf(ht, dx, Sd0));
-    t.q1 = __builtin_fmaf(w, dy, -__builtin_fmaf(ht, dy, Sd1));
-    t.q2 = __builtin_fmaf(w, dz, -__builtin_fmaf(ht, dz, Sd2));
+    t.c = __builtin_fmaf(t.u, __builtin_fmaf(-0.5f, T, w), s);
+    t.sd0 = Sd0;
+    t.sd1 = Sd1;
+    t.sd2 = Sd2;
     t.phi = -(double)G * (1.5 * (double)dSd - 0.5 * (double)T * (double)r2) /
             ((double)D * (double)r2);
     return t;
 }
 
-// F += s d (+ u q), with the oracle's order of roundings for both accumulation modes.
+// F += c d - u S d, with the oracle's order of roundings for both accumulation modes.
 template <bool KAHAN>
-__device__ inline void accumulate_quad(Acc3& F, float s, float dx, float dy, float dz,
-                                       const QuadTerm& t) {
+__device__ inline void accumulate_quad(Acc3& F, float dx, float dy, float dz, const QuadTerm& t) {
     if (KAHAN) {
-        float y = __builtin_fmaf(s, dx, __builtin_fmaf(t.u, t.q0, -F.cx));
+        float y = __builtin_fmaf(t.c, dx, __builtin_fmaf(-t.u, t.sd0, -F.cx));
         float v = F.x + y;
         F.cx = (v - F.x) - y;
         F.x = v;
-        y = __builtin_fmaf(s, dy, __builtin_fmaf(t.u, t.q1, -F.cy));
+        y = __builtin_fmaf(t.c, dy, __builtin_fmaf(-t.u, t.sd1, -F.cy));
         v = F.y + y;
         F.cy = (v - F.y) - y;
         F.y = v;
-        y = __builtin_fmaf(s, dz, __builtin_fmaf(t.u, t.q2, -F.cz));
+        y = __builtin_fmaf(t.c, dz, __builtin_fmaf(-t.u, t.sd2, -F.cz));
         v = F.z + y;
         F.cz = (v - F.z) - y;
         F.z = v;
     } else {
-        F.x = __builtin_fmaf(t.u, t.q0, __builtin_fmaf(s, dx, F.x));
-        F.y = __builtin_fmaf(t.u, t.q1, __builtin_fmaf(s, dy, F.y));
-        F.z = __builtin_fmaf(t.u, t.q2, __builtin_fmaf(s, dz, F.z));
+        F.x = __builtin_fmaf(t.c, dx, __builtin_fmaf(-t.u, t.sd0, F.x));
+        F.y = __builtin_fmaf(t.c, dy, __builtin_fmaf(-t.u, t.sd1, F.y));
+        F.z = __builtin_fmaf(t.c, dz, __builtin_fmaf(-t.u, t.sd2, F.z));
     }
 }
 
@@ -346,29 +347,30 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
                 const float s = div_rn_mid_y(gm, D, y);
                 if (KAHAN) {
                     if (QUAD && !leaf) {
-                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, G);
-                        accumulate_quad<true>(F, s, dx, dy, dz, qt);
+                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, s, G);
+                        accumulate_quad<true>(F, dx, dy, dz, qt);
                         if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
                     } else {
                         accumulate<true>(F, s, dx, dy, dz);
                         if (MODE == 2) phi -= (double)gm / (double)rr;
                     }
                 } else {
-                    // the monopole sum first (all of a leaf's term), the quadrupole term on it
-                    float fx = __builtin_fmaf(s, dx, F.x), fy = __builtin_fmaf(s, dy, F.y),
-                          fz = __builtin_fmaf(s, dz, F.z);
+                    // an internal node subtracts u S d and turns s into c; then every
+                    // interacting lane adds its coefficient times d (all of a leaf's term)
+                    float c = s, fx = F.x, fy = F.y, fz = F.z;
                     if (QUAD && !leaf) {
-                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, G);
-                        fx = __builtin_fmaf(qt.u, qt.q0, fx);
-                        fy = __builtin_fmaf(qt.u, qt.q1, fy);
-                        fz = __builtin_fmaf(qt.u, qt.q2, fz);
+                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, s, G);
+                        c = qt.c;
+                        fx = __builtin_fmaf(-qt.u, qt.sd0, fx);
+                        fy = __builtin_fmaf(-qt.u, qt.sd1, fy);
+                        fz = __builtin_fmaf(-qt.u, qt.sd2, fz);
                         if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
                     } else if (MODE == 2) {
                         phi -= (double)gm / (double)rr;
                     }
-                    F.x = fx;
-                    F.y = fy;
-                    F.z = fz;
+                    F.x = __builtin_fmaf(c, dx, fx);
+                    F.y = __builtin_fmaf(c, dy, fy);
+                    F.z = __builtin_fmaf(c, dz, fz);
                 }
             }
             if (MODE == 1) {
@@ -391,8 +393,8 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
             if (QUAD && interact && !leaf) {
                 const float y = FDIV ? rcp_rn_mid(D) : 1.0f / D;
-                const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, G);
-                accumulate_quad<KAHAN>(F, s, dx, dy, dz, qt);
+                const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, s, G);
+                accumulate_quad<KAHAN>(F, dx, dy, dz, qt);
                 if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
             } else if (KAHAN) {
                 Acc3 T = F;
@@ -953,9 +955,9 @@ __global__ __launch_bounds__(64) void k_force_reference(ForceArgs a, uint32_t de
                 // second moments sit in emittedColor (xx, yy, zz, xy) / emittedColorCenter
                 const float4 s0 = *reinterpret_cast<const float4*>(nd->emittedColor);
                 const float4 s1 = *reinterpret_cast<const float4*>(nd->emittedColorCenter);
-                const QuadTerm qt = quad_term(s0, s1, dx, dy, dz, r2, rr, D, 1.0f / D, a.p.G);
-                if (a.p.kahan) accumulate_quad<true>(F, s, dx, dy, dz, qt);
-                else accumulate_quad<false>(F, s, dx, dy, dz, qt);
+                const QuadTerm qt = quad_term(s0, s1, dx, dy, dz, r2, rr, D, 1.0f / D, s, a.p.G);
+                if (a.p.kahan) accumulate_quad<true>(F, dx, dy, dz, qt);
+                else accumulate_quad<false>(F, dx, dy, dz, qt);
             } else if (a.p.kahan) {
                 accumulate<true>(F, s, dx, dy, dz);
             } else {

@@ -516,10 +516,11 @@ void orc_barnes_hut(const OctreeNode* nodes, uint64_t root, const PositionMass* 
                 float gm = p->G * nd->totalMass;
                 float D = r2 * r;
                 float s = gm / D;
-                float q[3] = {0.0f, 0.0f, 0.0f}, u = 0.0f;
+                float sd[3] = {0.0f, 0.0f, 0.0f}, u = 0.0f, c = s;
                 int quad = p->multipole == 2 && !leaf;
                 if (quad) {
-                    /* a_Q = 3G v (w d - (S d + T/2 d)), v = 1/r^5 = (y y) r with y = 1/D,
+                    /* a = s d + 3G v (w d - (S d + T/2 d)) = c d - u S d with
+                     * c = s + u (w - T/2), u = 3G v, v = 1/r^5 = (y y) r with y = 1/D,
                      * w = 2.5 (d.S.d) v D */
                     float S[6];
                     mom_load(nd, S);
@@ -528,26 +529,26 @@ void orc_barnes_hut(const OctreeNode* nodes, uint64_t root, const PositionMass* 
                     float Sd2 = fmaf(S[2], dz, fmaf(S[5], dy, S[4] * dx));
                     float dSd = fmaf(dz, Sd2, fmaf(dy, Sd1, dx * Sd0));
                     float T = (S[0] + S[1]) + S[2];
-                    float ht = 0.5f * T;
                     float y = 1.0f / D;
                     float v = (y * y) * r; /* v D = 1/r2 to within a few roundings */
                     float w = (2.5f * dSd) * (v * D);
                     u = (3.0f * p->G) * v;
-                    q[0] = fmaf(w, dx, -fmaf(ht, dx, Sd0));
-                    q[1] = fmaf(w, dy, -fmaf(ht, dy, Sd1));
-                    q[2] = fmaf(w, dz, -fmaf(ht, dz, Sd2));
+                    c = fmaf(u, fmaf(-0.5f, T, w), s);
+                    sd[0] = Sd0;
+                    sd[1] = Sd1;
+                    sd[2] = Sd2;
                 }
                 float d[3] = {dx, dy, dz};
                 if (p->kahan) {
                     for (int k = 0; k < 3; k++) {
-                        float y = quad ? fmaf(s, d[k], fmaf(u, q[k], -C[k])) : fmaf(s, d[k], -C[k]);
+                        float y = quad ? fmaf(c, d[k], fmaf(-u, sd[k], -C[k])) : fmaf(s, d[k], -C[k]);
                         float t = F[k] + y;
                         C[k] = (t - F[k]) - y;
                         F[k] = t;
                     }
                 } else {
                     for (int k = 0; k < 3; k++)
-                        F[k] = quad ? fmaf(u, q[k], fmaf(s, d[k], F[k])) : fmaf(s, d[k], F[k]);
+                        F[k] = quad ? fmaf(c, d[k], fmaf(-u, sd[k], F[k])) : fmaf(s, d[k], F[k]);
                 }
             }
         }
