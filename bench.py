@@ -578,11 +578,12 @@ def main():
                                         "1024 SIMDs and the kernel's busy cycles "
                                         "(GRBM_GUI_ACTIVE per XCD; rocprofv3, profiles/)",
                      "ta_busy_frac": ta_frac,
-                     "limiter": "VALU issue with the top tree layers read from LDS (the "
-                                "texture address unit bound it while every node came from "
-                                "memory: busy 83%); issue and address unit both below "
-                                "saturation: the walk's dependent chain per visit "
-                                "(profiles/force_memory_pipeline_two_disks_500k.md)",
+                     "limiter": "the walk's dependent chain per visit (node load, distance, "
+                                "square root, division, next link): VALU issue and the "
+                                "texture address unit both busy about three quarters of the "
+                                "kernel's cycles with the top tree layers read from LDS (the "
+                                "address unit bound it while every node came from memory: "
+                                "busy 83%; profiles/force_memory_pipeline_two_disks_500k.md)",
                      "interactions_per_step": inter, "flop_per_interaction": FLOP_PER_INTERACTION,
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
         "stages": stages,
