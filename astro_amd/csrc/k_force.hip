@@ -221,12 +221,19 @@ struct TopView {
 // opened node goes to its last child and any other node to its next, until the root's next
 // (kNextEnd).  The visiting order, and with it the order of the force sums, is the
 // reference's LIFO traversal (children pushed 0..7, the last popped first).
+//
+// A visit decides where the walk goes (distance, opening test, links) and issues the next
+// node's loads before it computes its own interaction, which then runs while they are in
+// flight.  Two register sets take turns (the loop visits twice per iteration), so the loads
+// need no copies of the node they replace.
 template <bool KAHAN, int MODE, bool QUAD, bool FAST, bool FDIV, bool OFF64, bool G1,
           bool TOPC>
 __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, uint32_t root,
                                          float4 p, uint32_t self, float rmin0, float r2min0,
                                          Acc3& F, double& phi, Visits& st) {
     constexpr bool R2 = FAST && FDIV && QUAD;
+    // quadrupole trees below 2^27 nodes keep the links in the second moment row
+    constexpr bool QLINKS = QUAD && !OFF64;
     const float eps2 = a.p.eps2, r2f = a.p.r2_floor, rf = a.p.r_floor;
     const float cut = a.p.leaf_cutoff_r2, G = a.p.G;
     const char* pos_base = reinterpret_cast<const char*>(a.tpos);
@@ -234,102 +241,114 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
     // Node indices are not checked per visit: the host traverses only a tree built on the
     // current layer table, and the kernel none whose build overflowed a layer, so every index
     // is the root, a written node's last child, or a written node's next.
-    uint32_t n = root;
     uint32_t resume = kNextEnd;  // TOPC: next' of the last node of layer lt this lane opened
-    while (true) {
+    // a node as loaded (decoded where it is visited, so that no wait for the loads comes
+    // before the previous node's interaction)
+    struct Node {
         float4 cm, qm0, qm1;
-        uint32_t last, next, layer;
+        uint32_t last, info, next;  // monopoles, and quadrupole trees of 2^27 nodes or more
+        uint32_t n;
+    };
+    auto fetch = [&](uint32_t n, Node& v) {
+        v.n = n;
         if constexpr (TOPC && !QUAD) {
             // monopoles: one source or the other into the same registers; the LDS reads are
             // issued first, so that the global loads wait at most for them (not the reverse)
-            uint32_t info;
             const uint32_t o = n << 4;  // the rows' byte offset in either source
             if ((int32_t)n >= 0) {
-                cm = *reinterpret_cast<const float4*>(pos_base + o);
+                v.cm = *reinterpret_cast<const float4*>(pos_base + o);
                 const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
-                last = li.last;
-                info = li.info;
-                next = li.next;
+                v.last = li.last;
+                v.info = li.info;
+                v.next = li.next;
             } else {
                 const lds_f4 c = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
                     reinterpret_cast<const ASTRO_LDS char*>(tv.pos) + o);
                 const lds_u3 k = *reinterpret_cast<const ASTRO_LDS lds_u3*>(
                     reinterpret_cast<const ASTRO_LDS char*>(tv.lnk) + o);
-                cm = make_float4(c.x, c.y, c.z, c.w);
-                last = k.x;
-                info = k.y;
-                next = k.z;
+                v.cm = make_float4(c.x, c.y, c.z, c.w);
+                v.last = k.x;
+                v.info = k.y;
+                v.next = k.z;
             }
-            layer = tn_layer(info);
-            // a global node's next at or above off[lt] is a top node: back to the top through
-            // the next' of the node of layer lt this lane opened last.  Top nodes' next' and
-            // kNextEnd are negative as int32 and never match (a global node with next kNextEnd
-            // lies on the root's first-child chain, where resume is kNextEnd too).
-            if ((int32_t)next >= (int32_t)tv.top_lo) next = resume;
         } else if constexpr (TOPC && QUAD) {
             // quadrupoles: the same, three rows from either source; a top node's rows carry
             // the links packed as in the global moment rows
             if ((int32_t)(n << 5) >= 0) {  // bit 26 clear: a global node
-                cm = *reinterpret_cast<const float4*>(pos_base + (n << 4));
+                v.cm = *reinterpret_cast<const float4*>(pos_base + (n << 4));
                 const float4* qp = reinterpret_cast<const float4*>(
                     reinterpret_cast<const char*>(a.qmom) + (n << 5));
-                qm0 = qp[0];
-                qm1 = qp[1];
+                v.qm0 = qp[0];
+                v.qm1 = qp[1];
             } else {
                 const uint32_t ad = n & kTopAddrMask;
                 const lds_f4 c = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
                     reinterpret_cast<const ASTRO_LDS char*>(tv.pos) + ad);
                 const lds_f4 u = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
                     reinterpret_cast<const ASTRO_LDS char*>(tv.q0) + ad);
-                const lds_f4 v = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
+                const lds_f4 w = *reinterpret_cast<const ASTRO_LDS lds_f4*>(
                     reinterpret_cast<const ASTRO_LDS char*>(tv.q1) + ad);
-                cm = make_float4(c.x, c.y, c.z, c.w);
-                qm0 = make_float4(u.x, u.y, u.z, u.w);
-                qm1 = make_float4(v.x, v.y, v.z, v.w);
+                v.cm = make_float4(c.x, c.y, c.z, c.w);
+                v.qm0 = make_float4(u.x, u.y, u.z, u.w);
+                v.qm1 = make_float4(w.x, w.y, w.z, w.w);
             }
-            const uint32_t w = __float_as_uint(qm1.z);
-            last = w & kPackedIndexMask;
-            layer = w >> 27;
-            next = __float_as_uint(qm1.w);
+        } else {
+            // every node from device memory: one byte offset (16 B per node) into both
+            // arrays, from uniform bases; 32-bit while the tree has fewer than 2^27 nodes
+            const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
+            v.cm = *reinterpret_cast<const float4*>(pos_base + o);
+            // quadrupole mode: the moments are loaded with the node, not after the opening
+            // test; below 2^27 nodes their second row also holds the links
+            // (last | layer << 27, next)
+            if (QUAD) {
+                const float4* qp = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const char*>(a.qmom) +
+                    (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
+                v.qm0 = qp[0];
+                v.qm1 = qp[1];
+            }
+            if (!QLINKS) {
+                const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
+                v.last = li.last;
+                v.info = li.info;
+                v.next = li.next;
+            }
+        }
+    };
+    // Visits the node in v, loads the next one into w; returns whether the walk goes on.
+    auto visit = [&](const Node& v, Node& w) -> bool {
+        const float4 cm = v.cm;
+        uint32_t last, next, layer;
+        if constexpr (QLINKS) {
+            const uint32_t lw = __float_as_uint(v.qm1.z);
+            last = lw & kPackedIndexMask;
+            layer = lw >> 27;
+            next = __float_as_uint(v.qm1.w);
+        } else {
+            last = v.last;
+            layer = tn_layer(v.info);
+            next = v.next;
+        }
+        if constexpr (TOPC && !QUAD) {
+            // a global node's next at or above off[lt] is a top node: back to the top through
+            // the next' of the node of layer lt this lane opened last.  Top nodes' next' and
+            // kNextEnd are negative as int32 and never match (a global node with next kNextEnd
+            // lies on the root's first-child chain, where resume is kNextEnd too).
+            if ((int32_t)next >= (int32_t)tv.top_lo) next = resume;
+        } else if constexpr (TOPC && QUAD) {
             // a global node's next in [off[lt], 2^26) is a top node: back to the top through
             // the next' of the node of layer lt this lane opened last.  Top nodes' next' lie
             // at or above 2^26; kNextEnd is kept (a global node with next kNextEnd lies on the
             // root's first-child chain, where resume is kNextEnd too).
             if (next - tv.top_lo < kTopQuad - tv.top_lo) next = resume;
-        } else {
-            // every node from device memory: one byte offset (16 B per node) into both
-            // arrays, from uniform bases; 32-bit while the tree has fewer than 2^27 nodes
-            const size_t o = OFF64 ? ((size_t)n << 4) : (size_t)(n << 4);
-            cm = *reinterpret_cast<const float4*>(pos_base + o);
-            // quadrupole mode: the moments are loaded with the node, not after the opening
-            // test; below 2^27 nodes their second row also holds the links
-            // (last | layer << 27, next)
-            constexpr bool QLINKS = QUAD && !OFF64;
-            if (QUAD) {
-                const float4* qp = reinterpret_cast<const float4*>(
-                    reinterpret_cast<const char*>(a.qmom) +
-                    (OFF64 ? ((size_t)n << 5) : (size_t)(n << 5)));
-                qm0 = qp[0];
-                qm1 = qp[1];
-            }
-            if (QLINKS) {
-                const uint32_t w = __float_as_uint(qm1.z);
-                last = w & kPackedIndexMask;
-                layer = w >> 27;
-                next = __float_as_uint(qm1.w);
-            } else {
-                const TLink li = *reinterpret_cast<const TLink*>(link_base + o);
-                last = li.last;
-                layer = tn_layer(li.info);
-                next = li.next;
-            }
         }
         const float dx = cm.x - p.x, dy = cm.y - p.y, dz = cm.z - p.z;
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) + eps2;
         if (!FAST) r2 = fmaxf(r2, r2f);
         const bool leaf = layer == 0u;
         const float gm = G1 ? cm.w : G * cm.w;  // G1: G = 1, so RN(G M) = M
-        bool open;
+        bool open, interact, hit = false, far = false;
+        float rr = 0.0f;
         if constexpr (R2) {
             // RN(sqrt(r2)) >= rmin0 2^l  <=>  r2 >= r2min0 4^l (accept.hpp): the square root
             // and the division are only computed where some lane interacts.  The empty asm
@@ -337,63 +356,64 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
             // second memory round trip on the loop's critical path.
             __asm__ volatile("" : : "v"(gm));
             const uint64_t mleaf = lanes_of(leaf), mfar = lanes_of(r2 >= ldexpf(r2min0, 2 * (int)layer));
-            const uint64_t mself = lanes_of(n != self);
-            const bool interact = lane_in((mleaf & mself) | (~mleaf & mfar));
+            const uint64_t mself = lanes_of(v.n != self);
+            interact = lane_in((mleaf & mself) | (~mleaf & mfar));
             open = lane_in(~mleaf & ~mfar);
+        } else {
+            // FDIV bounds r2 (every cube within +-div_lmax), as sqrt_rn_rsq needs
+            rr = FAST ? (FDIV ? sqrt_rn_rsq(r2) : sqrt_rn_normal(r2)) : fmaxf(sqrtf(r2), rf);
+            // (plain selects here: mask operations measured slower on this path)
+            far = rr >= ldexpf(rmin0, (int)layer);
+            hit = (v.n != self) & (FAST || r2 > cut);
+            interact = leaf ? hit : far;
+            open = !leaf & !far;
+        }
+        // only top nodes have layer lt (the rest of the tree lies below it)
+        if (TOPC && open && layer == tv.lt) resume = next;
+        const uint32_t nn = open ? last : next;
+        const bool more = nn != kNextEnd;
+        fetch(more ? nn : v.n, w);  // a lane about to finish reads its node again
+        if constexpr (R2) {
             if (interact) {
-                const float rr = sqrt_rn_rsq(r2);  // FDIV: r2 is bounded
-                const float D = r2 * rr;
+                const float rq = sqrt_rn_rsq(r2);  // FDIV: r2 is bounded
+                const float D = r2 * rq;
                 const float y = rcp_rn_mid(D);
                 const float s = div_rn_mid_y(gm, D, y);
                 if (KAHAN) {
                     if (QUAD && !leaf) {
-                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, s, G);
+                        const QuadTerm qt = quad_term(v.qm0, v.qm1, dx, dy, dz, r2, rq, D, y, s, G);
                         accumulate_quad<true>(F, dx, dy, dz, qt);
-                        if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+                        if (MODE == 2) phi += qt.phi - (double)gm / (double)rq;
                     } else {
                         accumulate<true>(F, s, dx, dy, dz);
-                        if (MODE == 2) phi -= (double)gm / (double)rr;
+                        if (MODE == 2) phi -= (double)gm / (double)rq;
                     }
                 } else {
                     // an internal node subtracts u S d and turns s into c; then every
                     // interacting lane adds its coefficient times d (all of a leaf's term)
                     float c = s, fx = F.x, fy = F.y, fz = F.z;
                     if (QUAD && !leaf) {
-                        const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, s, G);
+                        const QuadTerm qt = quad_term(v.qm0, v.qm1, dx, dy, dz, r2, rq, D, y, s, G);
                         c = qt.c;
                         fx = __builtin_fmaf(-qt.u, qt.sd0, fx);
                         fy = __builtin_fmaf(-qt.u, qt.sd1, fy);
                         fz = __builtin_fmaf(-qt.u, qt.sd2, fz);
-                        if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
+                        if (MODE == 2) phi += qt.phi - (double)gm / (double)rq;
                     } else if (MODE == 2) {
-                        phi -= (double)gm / (double)rr;
+                        phi -= (double)gm / (double)rq;
                     }
                     F.x = __builtin_fmaf(c, dx, fx);
                     F.y = __builtin_fmaf(c, dy, fy);
                     F.z = __builtin_fmaf(c, dz, fz);
                 }
             }
-            if (MODE == 1) {
-                st.visits += 1;
-                st.leafs += leaf && interact;
-                st.monos += !leaf && interact;
-                st.opens += open;
-            }
         } else {
-            // FDIV bounds r2 (every cube within +-div_lmax), as sqrt_rn_rsq needs
-            const float rr = FAST ? (FDIV ? sqrt_rn_rsq(r2) : sqrt_rn_normal(r2))
-                                  : fmaxf(sqrtf(r2), rf);
-            // (plain selects here: mask operations measured slower on this path)
-            const bool far = rr >= ldexpf(rmin0, (int)layer);
-            const bool hit = (n != self) & (FAST || r2 > cut);
-            const bool interact = leaf ? hit : far;
-            open = !leaf & !far;
             // the term is computed for every lane and kept only where it interacts
             const float D = r2 * rr;
             const float s = FDIV ? div_rn_mid(gm, D) : gm / D;
             if (QUAD && interact && !leaf) {
                 const float y = FDIV ? rcp_rn_mid(D) : 1.0f / D;
-                const QuadTerm qt = quad_term(qm0, qm1, dx, dy, dz, r2, rr, D, y, s, G);
+                const QuadTerm qt = quad_term(v.qm0, v.qm1, dx, dy, dz, r2, rr, D, y, s, G);
                 accumulate_quad<KAHAN>(F, dx, dy, dz, qt);
                 if (MODE == 2) phi += qt.phi - (double)gm / (double)rr;
             } else if (KAHAN) {
@@ -414,18 +434,19 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
                 F.y = __builtin_fmaf(sk, dy, F.y);
                 F.z = __builtin_fmaf(sk, dz, F.z);
             }
-            if (MODE == 1) {
-                st.visits += 1;
-                st.leafs += leaf && interact;
-                st.monos += !leaf && interact;
-                st.opens += open;
-            }
             if (MODE == 2 && interact && !(QUAD && !leaf)) phi -= (double)gm / (double)rr;
         }
-        // only top nodes have layer lt (the rest of the tree lies below it)
-        if (TOPC && open && layer == tv.lt) resume = next;
-        n = open ? last : next;
-        if (n == kNextEnd) break;
+        if (MODE == 1) {
+            st.visits += 1;
+            st.leafs += leaf && interact;
+            st.monos += !leaf && interact;
+            st.opens += open;
+        }
+        return more;
+    };
+    Node A, B;
+    fetch(root, A);
+    while (visit(A, B) && visit(B, A)) {
     }
 }
 
