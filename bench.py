@@ -578,12 +578,13 @@ def main():
                                         "1024 SIMDs and the kernel's busy cycles "
                                         "(GRBM_GUI_ACTIVE per XCD; rocprofv3, profiles/)",
                      "ta_busy_frac": ta_frac,
-                     "limiter": "the walk's dependent chain per visit (node load, distance, "
-                                "square root, division, next link): VALU issue and the "
-                                "texture address unit both busy about three quarters of the "
-                                "kernel's cycles with the top tree layers read from LDS (the "
-                                "address unit bound it while every node came from memory: "
-                                "busy 83%; profiles/force_memory_pipeline_two_disks_500k.md)",
+                     "limiter": "VALU issue and the texture address unit together: with "
+                                "the top tree layers read from LDS a global visit costs the "
+                                "CU's one address unit about as many cycles as its VALU work "
+                                "costs each SIMD, and both are busy three quarters of the "
+                                "kernel's cycles or more (the address unit alone bound it "
+                                "while every node came from memory: busy 83%; "
+                                "profiles/force_memory_pipeline_two_disks_500k.md)",
                      "interactions_per_step": inter, "flop_per_interaction": FLOP_PER_INTERACTION,
                      "visits_per_step": 0.5 * (st0["visits"] + st1["visits"])},
         "stages": stages,
