@@ -996,10 +996,15 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
     __shared__ float4 s_val[kFusedRange];     // the current layer's nodes (centre, mass)
     __shared__ uint32_t s_lc[8];              // heads of layer lf per candidate tile
     __shared__ float s_mom[QUAD ? 6 * kFusedRange : 1];  // and their second moments
-    __shared__ uint16_t s_rank[kFusedRange];  // per position: nodes of the layer before it
-    __shared__ uint16_t s_pos[kFusedRange];   // per node of the layer: its start position
+    // per position: nodes of the layer before it; per node of the layer: its start position
+    // (two of each, by layer parity, so that one layer's are written while the last layer's
+    // are still read)
+    __shared__ uint16_t s_rank[2][kFusedRange];
+    __shared__ uint16_t s_pos[2][kFusedRange];
     __shared__ uint8_t s_h[kFusedRange];      // per position: its head level
     __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
+    constexpr uint32_t NW = kFusedThreads / 64;
+    __shared__ uint32_t s_wcnt[kFusedMaxLayers][NW + 1];  // heads per wave, then their scan
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t a0 = t * kFusedTile;
     const uint32_t b0 = min(a0 + kFusedTile, a.nt);
@@ -1036,16 +1041,40 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
     // levels of the range in LDS: the node stores of every layer read them
     s_h[q0] = (uint8_t)h0;
     s_h[q1] = (uint8_t)h1;
+    // Every layer's node ranks up front: wave w holds positions 128 w .. 128 w + 127 (two per
+    // lane, in order), so a position's rank among the layer's heads is the heads of the waves
+    // before it (a scan over the 16 wave counts) plus those of the lanes below it (ballots).
+    const int hv0 = h0 == kNoHead ? -1 : (int)h0, hv1 = h1 == kNoHead ? -1 : (int)h1;
+    const uint32_t wave = tid >> 6, lane = tid & 63u;
+    for (uint32_t l = 0; l < lf; l++) {
+        const uint64_t b0 = __ballot(hv0 >= (int)l), b1 = __ballot(hv1 >= (int)l);
+        if (lane == 0) s_wcnt[l][wave] = (uint32_t)(__popcll(b0) + __popcll(b1));
+    }
+    lds_barrier();
+    if (tid < lf) {  // per layer, the exclusive scan of the wave counts and the total
+        uint32_t c = 0;
+        for (uint32_t w = 0; w < NW; w++) {
+            const uint32_t v = s_wcnt[tid][w];
+            s_wcnt[tid][w] = c;
+            c += v;
+        }
+        s_wcnt[tid][NW] = c;
+    }
+    lds_barrier();
     uint32_t tot_prev = 0;
     for (uint32_t l = 0; l < lf; l++) {
-        const bool f0 = h0 != kNoHead && h0 >= l, f1 = h1 != kNoHead && h1 >= l;
-        uint32_t tot;
-        // s_tmp is next written after this layer's further barriers
-        const uint32_t r0 = lds_block_scan<kFusedThreads, false>(
-            (f0 ? 1u : 0u) + (f1 ? 1u : 0u), s_tmp, &tot);
+        const bool f0 = hv0 >= (int)l, f1 = hv1 >= (int)l;
+        const uint64_t b0 = __ballot(f0), b1 = __ballot(f1);
+        const uint32_t tot = s_wcnt[l][NW];
+        const uint32_t r0 = s_wcnt[l][wave] + lanes_below(b0) + lanes_below(b1);
         const uint32_t r1 = r0 + (f0 ? 1u : 0u);
-        if (f0) s_pos[r0] = (uint16_t)q0;
-        if (f1) s_pos[r1] = (uint16_t)q1;
+        uint16_t* const sp = s_pos[l & 1u];
+        const uint16_t* const srk = s_rank[(l + 1u) & 1u];  // the layer below's ranks
+        if (f0) sp[r0] = (uint16_t)q0;
+        if (f1) sp[r1] = (uint16_t)q1;
+        // per position, the nodes of this layer before it (the next layer's child ranks)
+        if (q0 < np) s_rank[l & 1u][q0] = (uint16_t)r0;
+        if (q1 < np) s_rank[l & 1u][q1] = (uint16_t)r1;
         if (l == 0) {  // the leaf of every position of the tile
             const uint32_t base0 = fbase[t];
             if (q0 < own) a.leaf_of_sorted[a0 + q0] = base0 + r0 + (f0 ? 1u : 0u) - 1u;
@@ -1064,8 +1093,8 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
             const uint32_t k = tid + i * kFusedThreads;
             done[i] = false;
             if (k >= tot) continue;
-            st[i] = s_pos[k];
-            const uint32_t nx = k + 1 < tot ? s_pos[k + 1] : np;
+            st[i] = sp[k];
+            const uint32_t nx = k + 1 < tot ? sp[k + 1] : np;
             done[i] = k + 1 < tot || end_ok;
             if (l == 0) {
                 c0[i] = a0 + st[i];
@@ -1077,8 +1106,8 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
                     else mom_particles<Mom32>(a, c0[i], c1[i], cm[i], S[i]);
                 }
             } else {
-                c0[i] = s_rank[st[i]];
-                c1[i] = nx < np ? s_rank[nx] : tot_prev;
+                c0[i] = srk[st[i]];
+                c1[i] = nx < np ? srk[nx] : tot_prev;
                 if (fp64) agg_nodes<Agg64>(s_val, c0[i], c1[i], cm[i]);
                 else agg_nodes<Agg32>(s_val, c0[i], c1[i], cm[i]);
                 if (QUAD) {
@@ -1139,13 +1168,10 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
                 atomicAdd(&lsup[(tt / kSuper) * kSuperStride], s_lc[tid]);
             }
         }
-        // per position, the nodes of this layer before it (the next layer's child ranks)
-        if (q0 < np) s_rank[q0] = (uint16_t)r0;
-        if (q1 < np) s_rank[q1] = (uint16_t)r1;
         tot_prev = tot;
-        // no barrier here: the next layer's scan passes two before anything reads s_val,
-        // s_rank or s_pos, and its first LDS write (s_tmp) was last read before this layer's
-        // own later barriers
+        // no barrier here: the next layer writes the other parity's s_pos and s_rank (last
+        // read before this layer's barrier after the aggregation) and passes a barrier before
+        // anything reads s_val
     }
     // Leftovers (a tile ran past its halo; the scan raised wlow[1] before this launch): the
     // last block to finish builds them.  Each block's nodes are released by its count
