@@ -752,7 +752,8 @@ __global__ __launch_bounds__(kTopThreads) void k_layers_top(TreeArgs a, uint32_t
 //    first node index) and the layer's size;
 //  * k_fused_build: one block per tile builds every layer of its positions in LDS, leaves from
 //    the particles and each layer from the one below, and writes the nodes that start in its
-//    tile.  It reads past its tile up to the next head at layer lf-1 (at most kFusedHalo
+//    tile, with their next pointers (from its own ranks of every layer below the top, in place
+//    of k_next_fill, which runs only without fused layers).  It reads past its tile up to the next head at layer lf-1 (at most kFusedHalo
 //    positions), so the nodes it owns are complete; a node that runs past that halo is left to
 //  * fused_fix: the fused build's last block to finish builds those nodes, layer by layer,
 //    from the finished layer below.
@@ -1004,7 +1005,8 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
     __shared__ uint8_t s_h[kFusedRange];      // per position: its head level
     __shared__ uint32_t s_tmp[kFusedThreads / 64 + 1];
     constexpr uint32_t NW = kFusedThreads / 64;
-    __shared__ uint32_t s_wcnt[kFusedMaxLayers][NW + 1];  // heads per wave, then their scan
+    __shared__ uint32_t s_wcnt[kMaxLayers][NW + 1];  // heads per wave, then their scan
+    __shared__ uint32_t s_pnext[kFusedTile];  // the next of the nodes starting at each position
     const uint32_t t = blockIdx.x, tid = threadIdx.x;
     const uint32_t a0 = t * kFusedTile;
     const uint32_t b0 = min(a0 + kFusedTile, a.nt);
@@ -1044,14 +1046,23 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
     // Every layer's node ranks up front: wave w holds positions 128 w .. 128 w + 127 (two per
     // lane, in order), so a position's rank among the layer's heads is the heads of the waves
     // before it (a scan over the 16 wave counts) plus those of the lanes below it (ballots).
+    // Every layer below the top (nl = bits), for the next pointers: the nodes starting at a
+    // head position s of level H < nl have as next the node of layer H holding s - 1, whose
+    // index is off[H] plus the layer's nodes before the tile plus s's rank at layer H, minus
+    // one; kNextEnd where s starts a system (H = bits).
+    const uint32_t nl = a.bits;
     const int hv0 = h0 == kNoHead ? -1 : (int)h0, hv1 = h1 == kNoHead ? -1 : (int)h1;
     const uint32_t wave = tid >> 6, lane = tid & 63u;
-    for (uint32_t l = 0; l < lf; l++) {
+    uint32_t pr0 = 0, pr1 = 0;  // the ranks of q0 and q1 within the wave at their own level
+    for (uint32_t l = 0; l < nl; l++) {
         const uint64_t b0 = __ballot(hv0 >= (int)l), b1 = __ballot(hv1 >= (int)l);
         if (lane == 0) s_wcnt[l][wave] = (uint32_t)(__popcll(b0) + __popcll(b1));
+        const uint32_t c = lanes_below(b0) + lanes_below(b1);
+        if (hv0 == (int)l) pr0 = c;
+        if (hv1 == (int)l) pr1 = c + (hv0 >= (int)l ? 1u : 0u);
     }
     lds_barrier();
-    if (tid < lf) {  // per layer, the exclusive scan of the wave counts and the total
+    if (tid < nl) {  // per layer, the exclusive scan of the wave counts and the total
         uint32_t c = 0;
         for (uint32_t w = 0; w < NW; w++) {
             const uint32_t v = s_wcnt[tid][w];
@@ -1061,6 +1072,25 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
         s_wcnt[tid][NW] = c;
     }
     lds_barrier();
+    {
+        // the next pointers of this tile's head positions, in LDS for the node stores below and
+        // in device memory when the leftover pass will read them
+        const bool keep = a.wlow[1] != 0u;
+        const uint32_t qq[2] = {q0, q1};
+        const int hv[2] = {hv0, hv1};
+        const uint32_t pr[2] = {pr0, pr1};
+#pragma unroll
+        for (uint32_t i = 0; i < 2; i++) {
+            if (qq[i] >= own || hv[i] < 0) continue;
+            const uint32_t H = (uint32_t)hv[i];
+            const uint32_t pn =
+                H >= nl ? kNextEnd
+                        : (uint32_t)(a.off[H] + fbase[(uint64_t)H * tiles + t] + s_wcnt[H][wave] +
+                                     pr[i] - 1u);
+            s_pnext[qq[i]] = pn;
+            if (keep) a.pnext[a0 + qq[i]] = pn;
+        }
+    }
     uint32_t tot_prev = 0;
     for (uint32_t l = 0; l < lf; l++) {
         const bool f0 = hv0 >= (int)l, f1 = hv1 >= (int)l;
@@ -1144,7 +1174,7 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
             tn.last = l == 0 ? ASTRO_INVALID_CHILD : (uint32_t)(a.off[l - 1] + base_prev + c1[i] - 1u);
             if (W > 0 && l == W) tn.last = kNextEnd;  // never opened by a finite target
             tn.info = tn_info(l, l == 0 ? 0u : c1[i] - c0[i], h);
-            tn.next = a.pnext[a0 + st[i]];
+            tn.next = s_pnext[st[i]];
             tn.start = a0 + st[i];
             store_node(a, nd, tn);
             a.levels[nd] = (uint8_t)h;
@@ -1216,8 +1246,10 @@ void launch_heads_next(const TreeArgs& a, uint32_t lf, uint8_t* plevel, uint32_t
                        nclr);
     hipLaunchKernelGGL(k_fused_scan, dim3(nl), dim3(kFusedThreads), 0, s, a,
                        fused_fcnt(fbuf, tiles), fused_fbase(fbuf, tiles), lf, tiles);
-    hipLaunchKernelGGL(k_next_fill, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
-                       fused_fbase(fbuf, tiles), nl, tiles);
+    // with fused layers the fused build computes the next pointers itself
+    if (lf == 0)
+        hipLaunchKernelGGL(k_next_fill, dim3(tiles), dim3(kFusedThreads), 0, s, a, plevel,
+                           fused_fbase(fbuf, tiles), nl, tiles);
 }
 
 void launch_fused_layers(const TreeArgs& a, uint32_t lf, const uint8_t* plevel, uint32_t* fbuf,

@@ -194,9 +194,10 @@ void launch_layer_count(const TreeArgs& a, uint32_t layer, uint32_t* tile_cnt, u
                         hipStream_t s);
 // Before any node is written: the head level of every sorted position (plevel), the node
 // count of every layer below the top per 1,024-position tile and its scan over the tiles
-// (fbuf), and from them a.pnext, the next pointer of the nodes starting at each position.
-// Layers 0 .. lf-1 (lf <= kFusedMaxLayers; 0: none) are then built in two launches
-// (monopole, no reference quirk) by launch_fused_layers.  plevel holds nt bytes and fbuf
+// (fbuf), and, without fused layers (lf = 0), from them a.pnext, the next pointer of the nodes
+// starting at each position.  Layers 0 .. lf-1 (lf <= kFusedMaxLayers; 0: none) are then built
+// in one launch by launch_fused_layers, which computes those next pointers itself (and stores
+// them in a.pnext when its leftover pass will read them).  plevel holds nt bytes and fbuf
 // fused_buf_words(nt) words, both scratch.
 constexpr uint32_t kFusedMaxLayers = 16;
 uint32_t fused_tiles(uint32_t nt);
