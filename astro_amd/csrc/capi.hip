@@ -151,7 +151,7 @@ extern "C" int astro_destroy(astro_ctx* c) {
                     c->d_sys_off, c->phi, c->diag_partial, c->levels, c->qmom, c->order,
                     c->rs_keys, c->rs_idx, c->spl_keys, c->spl_idx, c->rs_count, c->spl_cube,
                     c->spos, c->stype, c->rs_pos, c->rs_type, c->wlow, c->d_sys_b0,
-                    c->d_bkt_sys};
+                    c->d_bkt_sys, c->sysfirst};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
@@ -274,6 +274,8 @@ extern "C" int astro_create(const astro_config* cfg, astro_ctx** out) {
     e = e ? e : dalloc(c->device_bytes, &c->d_sys_b0, c->cfg.max_systems + 1ull);
     e = e ? e : dalloc(c->device_bytes, &c->d_bkt_sys, kResortMaxBuckets);
     e = e ? e : dalloc(c->device_bytes, &c->wlow, 3);
+    if (c->cfg.max_systems > 1)
+        e = e ? e : dalloc(c->device_bytes, &c->sysfirst, (uint64_t)c->cfg.max_systems * kMaxLayers);
     e = e ? e : dalloc(c->device_bytes, &c->spos, NT);  // the sort's positions and types in
     e = e ? e : dalloc(c->device_bytes, &c->stype, NT); // sorted order
     if (e != hipSuccess) {
@@ -885,6 +887,8 @@ static ForceArgs force_args(astro_ctx* c) {
     a.stats = nullptr;
     a.counts = c->layer_counts;
     a.wlow = c->wlow;
+    a.sysfirst = c->nsys > 1 ? c->sysfirst : nullptr;
+    a.nsys = c->nsys;
     for (uint32_t l = 0; l < c->layers; l++) a.off[l] = c->off[l];
     a.layers = c->layers;
     a.top = c->force_top ? 1u : 0u;
@@ -996,6 +1000,7 @@ static TreeArgs tree_args(astro_ctx* c) {
     a.side_shifted = c->params.side_rule == ASTRO_SIDE_SHIFTED ? 1u : 0u;
     a.cubes = c->cubes;
     a.wlow = c->wlow;
+    a.sysfirst = c->nsys > 1 ? c->sysfirst : nullptr;
     return a;
 }
 

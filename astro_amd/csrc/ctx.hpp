@@ -98,6 +98,8 @@ struct astro_ctx {
     // may have left layers out, so a caller's read of the tree builds it again in full
     bool skip_unreached = true;
     uint32_t* wlow = nullptr;
+    uint32_t* sysfirst = nullptr;      // batches: [max_systems][kMaxLayers] each system's first
+                                       // node per layer (the traversal's per-system LDS top)
     bool tree_partial = false;
     uint64_t device_bytes = 0;         // bytes allocated on the device at creation
     uint32_t* scratch = nullptr;       // zeroed per use: look-back statuses, histograms

@@ -455,24 +455,39 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
 // up (base[l]).  Thread 0 decides, every thread then copies its share of the nodes with their
 // links rewritten.  Returns the view the walks read.
 template <int NT, bool QUAD>
-__device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint4* s_lnk, float4* s_q0,
-                            float4* s_q1, uint32_t* s_base, uint32_t* s_meta) {
+__device__ TopView fill_top(const ForceArgs& a, uint32_t S, float4* s_pos, uint4* s_lnk,
+                            float4* s_q0, float4* s_q1, uint32_t* s_base, uint32_t* s_lo,
+                            uint32_t* s_meta) {
     constexpr uint32_t slots = TopCap<NT, QUAD>::kSlots;
     const uint32_t L = a.layers, tid = threadIdx.x;
+    // the cached tree's nodes of each layer: a batch's system S (its first node of the layer
+    // and, through the next system's first node, its count), else the whole layer
+    if (tid < L) {
+        uint32_t lo = 0, c = a.counts[tid];
+        if (a.sysfirst) {
+            lo = a.sysfirst[(size_t)S * kMaxLayers + tid];
+            if (S + 1u < a.nsys) c = a.sysfirst[(size_t)(S + 1u) * kMaxLayers + tid];
+            c -= lo;
+        }
+        s_lo[tid] = lo;
+        s_base[tid] = c;  // (the counts until thread 0 turns them into slot bases)
+    }
+    __syncthreads();
     if (tid == 0) {
         uint32_t lt = L, K = 0;
         const uint32_t wl = a.wlow ? *a.wlow : 0u;  // the lowest layer the build wrote
         for (uint32_t l = L; l-- > 0;) {
             if (l < wl) break;
-            const uint32_t c = a.counts[l];
+            const uint32_t c = s_base[l];
             if (K + c > slots) break;
             K += c;
             lt = l;
         }
         uint32_t b = 0;
         for (uint32_t l = lt; l < L; l++) {
+            const uint32_t c = s_base[l];
             s_base[l] = b;
-            b += a.counts[l];
+            b += c;
         }
         s_meta[0] = lt;
         s_meta[1] = K;
@@ -482,15 +497,16 @@ __device__ TopView fill_top(const ForceArgs& a, float4* s_pos, uint4* s_lnk, flo
     for (uint32_t s = tid; s < K; s += NT) {
         uint32_t l = lt;
         while (l + 1 < L && s >= s_base[l + 1]) l++;
-        const uint32_t n = (uint32_t)a.off[l] + (s - s_base[l]);
+        const uint32_t n = (uint32_t)a.off[l] + s_lo[l] + (s - s_base[l]);
         const TLink k = a.tlink[n];
         s_pos[s] = a.tpos[n];
         uint32_t last = k.last, next = k.next;
-        if (l > lt) last = top_name<QUAD>(last - (uint32_t)a.off[l - 1] + s_base[l - 1]);
+        if (l > lt)
+            last = top_name<QUAD>(last - (uint32_t)a.off[l - 1] - s_lo[l - 1] + s_base[l - 1]);
         else if (l == 0u) last = QUAD ? kPackedNoChild : kNextEnd;  // a leaf: never opened
-        if (next != kNextEnd) {
+        if (next != kNextEnd) {  // (a node's next lies in its own system's tree)
             const uint32_t h = tn_level(k.info);  // the layer of the next node
-            next = top_name<QUAD>(next - (uint32_t)a.off[h] + s_base[h]);
+            next = top_name<QUAD>(next - (uint32_t)a.off[h] - s_lo[h] + s_base[h]);
         }
         if (QUAD) {
             s_q0[s] = a.qmom[2 * (size_t)n];
@@ -523,6 +539,7 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
     __shared__ float4 s_q0[TOPC && QUAD ? kSl : 1u];
     __shared__ float4 s_q1[TOPC && QUAD ? kSl : 1u];
     __shared__ uint32_t s_base[kMaxLayers];
+    __shared__ uint32_t s_lo[kMaxLayers];
     __shared__ uint32_t s_meta[2];
     // XCD-aware mapping: hardware places block b on XCD b % 8.  xcd_chunk 0 gives XCD x the
     // x-th contiguous eighth of the target tiles (best L2 locality); xcd_chunk K deals runs of
@@ -545,8 +562,12 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
                                                          __HIP_MEMORY_SCOPE_AGENT)) &
         ERR_CAPACITY)
         return;
+    // a batch's block caches the top of the system of its first target (nearly always the
+    // system of all of them); its other targets walk from device memory
+    const uint32_t j0 = tile * NT;
+    const uint32_t S = (TOPC && a.sysfirst && j0 < a.ns) ? a.sys_of[j0] : 0u;
     TopView tv{};
-    if (TOPC) tv = fill_top<NT, QUAD>(a, s_pos, s_lnk, s_q0, s_q1, s_base, s_meta);
+    if (TOPC) tv = fill_top<NT, QUAD>(a, S, s_pos, s_lnk, s_q0, s_q1, s_base, s_lo, s_meta);
     const bool active = j < a.ns;
     const uint32_t sys = (active && a.sys_of) ? a.sys_of[j] : 0u;
     // the fast division holds for the whole wave when every lane's system cube lies within
@@ -578,10 +599,14 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
             (type != 2u && j < a.nt) ? (uint32_t)(a.off0 + a.leaf_of_sorted[j]) : 0xFFFFFFFFu;
         uint32_t root = (uint32_t)(a.root_off + sys);
         if (TOPC && tv.lt < a.layers) {
-            // the roots are top nodes; so are the leaves when the whole tree is cached
-            root = top_name<QUAD>(s_base[a.layers - 1u] + sys);
-            if (tv.lt == 0u && self != 0xFFFFFFFFu)
-                self = top_name<QUAD>(self - (uint32_t)a.off0 + s_base[0]);
+            if (sys == S) {
+                // the root is a top node; so are the leaves when the whole tree is cached
+                root = top_name<QUAD>(s_base[a.layers - 1u] + sys - s_lo[a.layers - 1u]);
+                if (tv.lt == 0u && self != 0xFFFFFFFFu)
+                    self = top_name<QUAD>(self - (uint32_t)a.off0 - s_lo[0] + s_base[0]);
+            } else {
+                tv.top_lo = QUAD ? kTopQuad : 0x7FFFFFFFu;  // no global link leads to the top
+            }
         }
         Acc3 F;
         double phi = 0.0;

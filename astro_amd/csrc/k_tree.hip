@@ -400,6 +400,7 @@ __device__ uint32_t emit_leaf(const TreeArgs& a, uint32_t j, uint32_t u) {
     t.start = j;
     store_node(a, n, t);
     a.levels[n] = (uint8_t)h;
+    if (a.sysfirst && h == a.bits) a.sysfirst[(size_t)a.sys_of[j] * kMaxLayers] = u;
     float S[6];
     if (QUAD) {
         if (sizeof(AGG) == sizeof(Agg64)) {
@@ -606,6 +607,8 @@ __device__ __forceinline__ uint32_t emit_parent(const TreeArgs& a, uint32_t l, u
     t.start = u0.start;
     store_node(a, nd, t);
     a.levels[nd] = (uint8_t)level;
+    if (a.sysfirst && level == a.bits)  // a system's first node of this layer
+        a.sysfirst[(size_t)a.sys_of[u0.start] * kMaxLayers + l] = p;
     if (QUAD) {
         float S[6];
         const float4* cq = a.qmom + 2 * (cb + c);
@@ -1178,6 +1181,8 @@ __global__ __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(8
             tn.start = a0 + st[i];
             store_node(a, nd, tn);
             a.levels[nd] = (uint8_t)h;
+            if (a.sysfirst && h == a.bits)  // a system's first node of this layer
+                a.sysfirst[(size_t)a.sys_of[tn.start] * kMaxLayers + l] = gi;
             if (QUAD) store_moments(a, nd, S[i], tn.last, l, tn.next);
         }
         if (l + 1 == lf && lcnt) {

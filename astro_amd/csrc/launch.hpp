@@ -170,6 +170,9 @@ struct TreeArgs {
     const float4* cubes;
     uint32_t* wlow;             // [3]: W, whether any tile left a node for fused_fix, and
                                 //      the fused build's finished-block count (left 0)
+    // Batches (else null): [nsys][kMaxLayers] the index in layer l of system s's first node
+    // (the node starting at the system's first position, a head at every layer)
+    uint32_t* sysfirst;
 };
 
 void launch_bbox_init(uint32_t* minmax, uint32_t nsys, hipStream_t s);
@@ -266,6 +269,8 @@ struct ForceArgs {
     double* phi;                // per-particle FP64 potential, or null
     const uint32_t* counts;     // [layers] valid nodes per layer (device)
     const uint32_t* wlow;       // [1] the lowest layer the build wrote (the LDS top stays above)
+    const uint32_t* sysfirst;   // batches (else null): the build's first node of each system
+    uint32_t nsys;              //   per layer ([nsys][kMaxLayers]), and the system count
     uint64_t off[kMaxLayers];   // layer offsets
     uint32_t layers;
     uint32_t top;               // 1: the top layers from LDS (blocks of 512 or 1,024 threads)
