@@ -708,13 +708,15 @@ def test_wide_offsets_bitexact(astro, orc, small_ctx, bits, multipole):
 
 @pytest.mark.parametrize("top", [0, 1])
 @pytest.mark.parametrize("block", [512, 1024])
-@pytest.mark.parametrize("case", ["plummer_16384", "batched", "whole_tree_in_lds"])
+@pytest.mark.parametrize("case", ["plummer_16384", "batched", "batched_quad", "whole_tree_in_lds"])
 def test_traversal_blocks_and_lds_top(astro, orc, case, block, top):
     # blocks of 512 and 1,024 threads, with and without the LDS copy of the top layers: the
-    # forces equal the oracle's bit for bit (one system, batched systems with their own roots,
-    # and a tree small enough that every layer, leaves included, sits in LDS)
+    # forces equal the oracle's bit for bit (one system, batched systems with their own roots --
+    # a block caches the top of its first target's system, and its targets of other systems
+    # walk from device memory -- and a tree small enough that every layer, leaves included,
+    # sits in LDS)
     say("traversal block", block, "top", top, case)
-    if case == "batched":
+    if case.startswith("batched"):
         from test_gpu_batched import make_batch
         sizes = [3000, 1, 5000, 700]
         parts, pos, vel, col, off = make_batch(astro, sizes, seed0=60)
@@ -723,9 +725,12 @@ def test_traversal_blocks_and_lds_top(astro, orc, case, block, top):
         pos, vel, col = plummer(astro, n, 29)
     bits = 7 if case == "whole_tree_in_lds" else 21  # 8 layers of at most 150 nodes
     P, Po = astro.params_physics(bits), orc.params_physics(bits)
+    if case == "batched_quad":
+        P.multipole = Po.multipole = 2
+        P.theta = Po.theta = 0.55
     with astro.Context(len(pos), bits_per_axis=21, max_systems=4) as ctx:
         ctx.set_params(P)
-        if case == "batched":
+        if case.startswith("batched"):
             ctx.upload_systems(pos, vel, col, off)
         else:
             ctx.upload(pos, vel, col)
@@ -735,7 +740,7 @@ def test_traversal_blocks_and_lds_top(astro, orc, case, block, top):
         ctx.compute_forces()
         ctx.check()
         f = ctx.forces()
-    if case == "batched":
+    if case.startswith("batched"):
         for k, (p, v, c) in enumerate(parts):
             t = orc.build_tree(p, Po)
             want, _ = orc.barnes_hut(t, p, c, Po, threads=8)
