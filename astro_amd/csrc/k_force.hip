@@ -451,8 +451,9 @@ __device__ __forceinline__ void traverse(const ForceArgs& a, const TopView& tv, 
 }
 
 // The LDS top of the tree (TOPC): the cached layers lt .. L-1 are the most, from the root
-// down, whose node counts sum to at most `slots`; slots are given out layer by layer from lt
-// up (base[l]).  Thread 0 decides, every thread then copies its share of the nodes with their
+// down, whose node counts sum to at most `slots` (in a batch, the counts of system S's tree,
+// whose nodes of layer l are the index range lo[l] ..); slots are given out layer by layer
+// from lt up (base[l]).  Thread 0 decides, every thread then copies its share of the nodes with their
 // links rewritten.  Returns the view the walks read.
 template <int NT, bool QUAD>
 __device__ TopView fill_top(const ForceArgs& a, uint32_t S, float4* s_pos, uint4* s_lnk,
