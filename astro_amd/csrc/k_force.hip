@@ -563,10 +563,11 @@ __global__ __launch_bounds__(NT) void k_force_compact(ForceArgs a) {
                                                          __HIP_MEMORY_SCOPE_AGENT)) &
         ERR_CAPACITY)
         return;
-    // a batch's block caches the top of the system of its first target (nearly always the
-    // system of all of them); its other targets walk from device memory
-    const uint32_t j0 = tile * NT;
-    const uint32_t S = (TOPC && a.sysfirst && j0 < a.ns) ? a.sys_of[j0] : 0u;
+    // a batch's block caches the top of the system of its middle target (nearly always the
+    // system of all of them, and of most of them when the block straddles two); its other
+    // targets walk from device memory
+    const uint32_t jm = min(tile * NT + NT / 2u, a.ns - 1u);
+    const uint32_t S = (TOPC && a.sysfirst && a.ns) ? a.sys_of[jm] : 0u;
     TopView tv{};
     if (TOPC) tv = fill_top<NT, QUAD>(a, S, s_pos, s_lnk, s_q0, s_q1, s_base, s_lo, s_meta);
     const bool active = j < a.ns;

@@ -210,7 +210,7 @@ int astro_traversal_stats(astro_ctx* ctx, astro_stats* out);
 int astro_set_force_block(astro_ctx* ctx, uint32_t block);
 /* With blocks of 512 or 1,024 threads, each block first copies the top layers of the tree (as
  * many as fit in its LDS, chosen from the layer counts on the device; in a batch, of the tree of
- * its first target's system) into LDS, and the walks read them there (default on; the results
+ * its middle target's system) into LDS, and the walks read them there (default on; the results
  * are the same bits either way).  Replaces nothing in
  * the reference: BarnesHut.metal:22-101 reads every node from device memory. */
 int astro_set_force_top_cache(astro_ctx* ctx, int on);

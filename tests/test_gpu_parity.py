@@ -712,7 +712,7 @@ def test_wide_offsets_bitexact(astro, orc, small_ctx, bits, multipole):
 def test_traversal_blocks_and_lds_top(astro, orc, case, block, top):
     # blocks of 512 and 1,024 threads, with and without the LDS copy of the top layers: the
     # forces equal the oracle's bit for bit (one system, batched systems with their own roots --
-    # a block caches the top of its first target's system, and its targets of other systems
+    # a block caches the top of its middle target's system, and its targets of other systems
     # walk from device memory -- and a tree small enough that every layer, leaves included,
     # sits in LDS)
     say("traversal block", block, "top", top, case)
